@@ -1,0 +1,124 @@
+"""Writes the golden fixtures under tests/golden/<case>/ from the reference.
+
+Run in a container that holds /root/reference (after ``make -C oracle ref``):
+
+    python tests/golden/make_golden.py
+
+For every case it generates the seeded inputs (mechafm_amd.workloads), runs
+``oracle/_ref/ref_harness`` -- the reference's own objects, built from its
+sources -- and stores
+
+* the input files the reference read (cube files gzipped),
+* ``records.npz``: the full-precision records of every scan position
+  (i j k n_steps x y z fx fy fz rx ry rz r angle e),
+* for grid / external-potential cases, ``eval.npz``: tip positions and the
+  reference's System::evalTipSurfaceForces at them.
+
+Only data is committed; no reference source or binary enters the repository.
+"""
+from __future__ import annotations
+
+import gzip
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from mechafm_amd import workloads as W  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+# name -> (maker, kwargs, eval samples)
+CASES = {
+    "c1_mini": (W.make_c1, dict(mini=True), 0),
+    "c1_rk4": (W.make_c1, dict(mini=True, area="0.5 0.5", integrator="rk4"), 0),
+    "c1_euler": (W.make_c1, dict(mini=True, area="0.5 0.5", integrator="euler"), 0),
+    "c1_sd": (W.make_c1, dict(mini=True, area="0.2 0.2", minimiser="SD", zlow="9.0"), 0),
+    "c1_minterm_e": (W.make_c1, dict(mini=True, area="0.5 0.5", minterm="e"), 0),
+    "c1_minterm_f": (W.make_c1, dict(mini=True, area="0.5 0.5", minterm="f", ftol="0.001"), 0),
+    "c1_maxsteps": (W.make_c1, dict(mini=True, area="0.5 0.5", maxsteps="20"), 0),
+    "c1_pbc": (W.make_c1, dict(mini=True, area="0.5 0.5", vdw_pbc="on", cell_a="14.0 0.0 0.0",
+                               cell_b="0.0 14.0 0.0", cell_c="0.0 0.0 30.0", center="7.0 7.0"), 0),
+    "c2_mini": (W.make_c2, dict(mini=True), 512),
+    "c3_mol": (lambda d, **k: W.make_c3(d, count=3, **k)[2], dict(mini=True), 0),
+    "c4_mini": (W.make_c4, dict(mini=True), 512),
+    "c5_mini": (W.make_c5, dict(mini=True), 0),
+    "c5_param_ev": (W.make_c5, dict(mini=True, units="eV", tip_dummy_coulomb="on", coulomb="on"), 0),
+}
+
+
+def harness(*args: str) -> None:
+    subprocess.check_call([HARNESS, *args], stdout=subprocess.DEVNULL,
+                          env=dict(os.environ, OMP_NUM_THREADS=str(os.cpu_count() or 8)))
+
+
+def param_charge_variant(work: str) -> None:
+    """c5_param_ev: drop the xyz charge column, give charges in the parameter
+    file and add an LJ pair overwrite (exercises parse.cpp:571-583, 639-663)."""
+    xyz = os.path.join(work, "molecule.xyz")
+    lines = open(xyz).read().splitlines()
+    out = lines[:2] + [" ".join(l.split()[:4]) for l in lines[2:] if l.strip()]
+    open(xyz, "w").write("\n".join(out) + "\n")
+    atoms = dict(W.BASE_ATOMS)
+    atoms["C"] = atoms["C"][:3] + (-0.05,)
+    atoms["N"] = atoms["N"][:3] + (-0.15,)
+    atoms["H"] = atoms["H"][:3] + (0.10,)
+    W.write_params(os.path.join(work, "parameters.dat"), atoms, W.HARM, ["O H lj 0.05000 2.90000"])
+
+
+def make_case(name: str) -> None:
+    maker, kw, n_eval = CASES[name]
+    dst = os.path.join(OUT, name)
+    if os.path.isdir(dst):
+        shutil.rmtree(dst)
+    with tempfile.TemporaryDirectory() as work:
+        wl = maker(work, **kw)
+        wdir = os.path.dirname(wl.input_path)
+        if name == "c5_param_ev":
+            param_charge_variant(wdir)
+        rec_bin = os.path.join(wdir, "records.bin")
+        harness("scan", wl.input_path, rec_bin)
+        raw = open(rec_bin, "rb").read()
+        hdr = np.frombuffer(raw[:16], dtype=np.int32)
+        rec = np.frombuffer(raw[16:], dtype=np.float64).reshape(-1, int(hdr[3]))
+        os.makedirs(dst)
+        for fn in sorted(os.listdir(wdir)):
+            src = os.path.join(wdir, fn)
+            if fn.endswith(".bin"):
+                continue
+            if fn.endswith(".cube"):
+                with open(src, "rb") as fi, gzip.open(os.path.join(dst, fn + ".gz"), "wb", 9) as fo:
+                    fo.write(fi.read())
+            else:
+                shutil.copy(src, os.path.join(dst, fn))
+        np.savez_compressed(os.path.join(dst, "records.npz"), shape=hdr[:3], records=rec)
+        if n_eval:
+            rng = np.random.default_rng(99)
+            ax, ay = (float(v) for v in wl.options["area"].split())
+            zl, zh = float(wl.options["zlow"]), float(wl.options["zhigh"])
+            pos = np.stack([rng.uniform(0, ax, n_eval), rng.uniform(0, ay, n_eval),
+                            rng.uniform(zl - 3.76, zh - 3.76, n_eval)], axis=1)
+            pin = os.path.join(wdir, "pos.bin")
+            pout = os.path.join(wdir, "eval.bin")
+            pos.astype(np.float64).tofile(pin)
+            harness("eval", wl.input_path, pin, pout)
+            ev = np.fromfile(pout, dtype=np.float64).reshape(-1, 4)
+            np.savez_compressed(os.path.join(dst, "eval.npz"), pos=pos, fe=ev)
+        print(f"{name}: {rec.shape[0]} records, steps mean {rec[:, 3].mean():.1f} max {rec[:, 3].max():.0f}")
+
+
+def main(names=None) -> None:
+    if not os.path.exists(HARNESS):
+        raise SystemExit("build the reference harness first: make -C oracle ref")
+    for name in names or CASES:
+        make_case(name)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:] or None)
