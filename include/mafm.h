@@ -1,0 +1,206 @@
+/*
+ * mafm.h -- C ABI of libmafm, the MI355X (gfx950) rigid probe-particle scan.
+ *
+ * Plain C types only: host pointers are borrowed for the duration of a call,
+ * every entry point returns mafm_status (no exit(), no exceptions cross the
+ * ABI), one context per host thread, calls on one context are serialised on
+ * the context's own HIP stream.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * MechAFM source tree):
+ *   mafm_set_systems       the tip-surface / tip-dummy part of the Interaction
+ *                          list built by Simulation::buildInteractions
+ *                          (simulation.cpp:201-216, 238-418, 503-515) and held
+ *                          as vector<unique_ptr<Interaction>> (simulation.hpp:76)
+ *   mafm_scan              the pixel loop of Simulation::run
+ *                          (simulation.cpp:78-140) with FIREMinimisation /
+ *                          SDMinimisation (minimiser.cpp:39-129),
+ *                          euler/midpoint/rk4Step (integrators.cpp:8-111) and
+ *                          System::getOutput (system.cpp:28-51)
+ *   mafm_records           the OutputData buffer drained by writeOutput
+ *                          (simulation.cpp:145-199; globals.hpp:27-32)
+ *   mafm_eval_tip_surface  Interaction::eval over the tip-surface list
+ *                          (interactions.hpp:68-76, interactions.cpp:15-51,
+ *                          236-250) as System::evalTipSurfaceForces calls it
+ *   mafm_grid_build        the grid sampling loop of buildTipGridInteractions
+ *                          (simulation.cpp:420-479)
+ *   mafm_grid_upload /     ForceGrid set-up and swapForceValues /
+ *   mafm_grid_download     swapEnergyValues (force_grid.hpp:20-30)
+ *   mafm_potential_grid    ElectrostaticPotentialInteraction's constructor
+ *                          (interactions.cpp:53-234)
+ *   mafm_fft3d             fft_data_grid / ffti_data_grid (fft.hpp:25, 34)
+ */
+#ifndef MAFM_H
+#define MAFM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    MAFM_OK = 0,
+    MAFM_ERR_INVALID = 1,      /* bad argument / descriptor */
+    MAFM_ERR_HIP = 2,          /* a HIP runtime call failed */
+    MAFM_ERR_OUT_OF_GRID = 3,  /* non-periodic grid lookup needed a point past the grid
+                                  (force_grid.cpp:236-240 calls error() there) */
+    MAFM_ERR_STATE = 4,        /* call order: e.g. scan before set_systems */
+    MAFM_ERR_FFT_IMAG = 5,     /* inverse FFT imaginary part above 1e-8 (fft.cpp:79-83) */
+    MAFM_ERR_NO_DEVICE = 6
+} mafm_status;
+
+typedef struct mafm_ctx mafm_ctx;
+
+/* ---- interaction tables ------------------------------------------------ */
+enum { MAFM_VDW_LJ = 0, MAFM_VDW_MORSE = 1 };
+
+/* One tip-surface site: the VDW term (LJInteraction / MorseInteraction,
+ * interactions.hpp:79-119) against one surface atom (or one periodic image of
+ * it, simulation.cpp:298-312) plus the optional Coulomb term on the same atom
+ * (CoulombInteraction, interactions.hpp:121-137).  Sites are given in the
+ * reference's list order: VDW(1,i) then Coulomb(1,i), atom by atom. */
+typedef struct {
+    double x, y, z;    /* atom position + pbc shift, Angstrom */
+    int32_t vdw;       /* MAFM_VDW_LJ or MAFM_VDW_MORSE */
+    int32_t has_coulomb;
+    double p0, p1, p2; /* LJ: es6, es12, unused | Morse: De, a, re */
+    double qq;         /* qbase*q_tip*q_atom when has_coulomb */
+} mafm_site;
+
+/* Tip-dummy terms (simulation.cpp:503-515): VDW(0,1), optional Coulomb(0,1)
+ * and the lateral TipHarmonicInteraction (interactions.cpp:252-266). */
+typedef struct {
+    int32_t vdw;
+    int32_t has_coulomb;
+    double p0, p1, p2;
+    double qq;
+    double k, r0;
+} mafm_tip_dummy;
+
+/* One molecule / surface to scan. */
+typedef struct {
+    int32_t n_sites;
+    int32_t force_grid;     /* handle of a non-periodic force grid replacing the
+                               sites (rigidgrid on), or -1 */
+    const mafm_site* sites;
+    int32_t potential_grid; /* handle of a periodic external-potential grid
+                               evaluated after the sites, or -1 */
+    int32_t tip_pbc;        /* System::setDummyXY wraps the dummy into the cell */
+    mafm_tip_dummy tip_dummy;
+    double tip_dummy_d;     /* initial tip-dummy distance (simulation.cpp:218-225) */
+    double cell[9];         /* row-major cell matrix (columns = cell vectors) */
+    double offset[3];       /* System offset (system.cpp:103-136) */
+} mafm_system_desc;
+
+/* ---- grids ------------------------------------------------------------- */
+typedef struct {
+    int32_t n[3];
+    int32_t periodic;
+    double basis[9];        /* row-major, columns = grid step vectors */
+    double offset[3];       /* position of grid point (0,0,0) */
+    const double* fe;       /* n0*n1*n2 x 4 doubles (fx fy fz e), z fastest;
+                               NULL for mafm_grid_build */
+} mafm_grid_desc;
+
+/* ---- scan -------------------------------------------------------------- */
+enum { MAFM_MIN_SD = 0, MAFM_MIN_FIRE = 1 };
+enum { MAFM_INT_EULER = 0, MAFM_INT_MIDPOINT = 1, MAFM_INT_RK4 = 2 };
+enum { MAFM_TERM_E = 0, MAFM_TERM_F = 1, MAFM_TERM_EF = 2 };
+/* Arithmetic of the pair terms.  PLAIN64: one lane per pixel, IEEE divisions,
+ * operation order of the reference (the bit-parity path and the "plain FP64"
+ * baseline).  FAST64: work-stealing persistent kernel, FP64 with refined
+ * reciprocals.  FAST32: the same kernel with FP32 pair terms and compensated
+ * accumulation; minimiser state and convergence tests stay FP64. */
+enum { MAFM_PREC_PLAIN64 = 0, MAFM_PREC_FAST64 = 1, MAFM_PREC_FAST32 = 2 };
+
+typedef struct {
+    int32_t nx, ny, nz;
+    double dx, dy, dz;
+    double zhigh;
+    int32_t minimiser, integrator, minterm;
+    double etol, ftol, dt;
+    int32_t maxsteps;
+    int32_t precision;
+    /* tuning, 0 = automatic */
+    int32_t lanes_per_pixel;   /* 1,2,4,...,64 lanes share one pixel column */
+    int32_t waves_per_simd;    /* persistent grid size */
+    int32_t compensation;      /* FAST32: 0 none, 1 Kahan (default when 0 and autotune off: 1) */
+    int32_t reserved;
+} mafm_scan_desc;
+
+/* Per scan position (OutputData, globals.hpp:27-32, minus the derived r and
+ * angle which the writer computes from r). */
+typedef struct {
+    double x, y, z;            /* requested pixel x, y; dummy z */
+    double fx, fy, fz;         /* tip-surface force at the relaxed tip */
+    double rx, ry, rz;         /* tip - dummy */
+    double e;                  /* tip-surface energy */
+    int32_t n_steps;           /* minimiser iterations (maxsteps if not converged) */
+    int32_t flags;             /* MAFM_REC_* */
+} mafm_record;
+
+enum { MAFM_REC_UNCONVERGED = 1, MAFM_REC_GRID_CLAMP = 2, MAFM_REC_OUT_OF_GRID = 4 };
+
+typedef struct {
+    double kernel_ms;          /* device time of the scan kernel(s), HIP events */
+    int64_t positions;
+    int64_t total_steps;
+    int64_t unconverged;
+    int64_t grid_warnings;
+    int64_t pair_evals;        /* sites x evaluations (algorithmic work) */
+    int32_t lanes_per_pixel;
+    int32_t waves_per_simd;
+    int32_t grid_blocks;
+    int32_t block_threads;
+} mafm_scan_stats;
+
+typedef struct {
+    char name[64];
+    int32_t compute_units;
+    int32_t clock_mhz;
+    int64_t total_mem;
+    int32_t lds_per_block;
+    int32_t warp_size;
+} mafm_device_props;
+
+/* External electrostatic potential on a periodic grid (the cube data after the
+ * unit scaling of simulation.cpp:348-355 and the origin shift of :365). */
+typedef struct {
+    int32_t n[3];
+    double basis[9];          /* row-major, columns = voxel vectors, Angstrom */
+    double origin[3];
+    const double* potential;  /* n0*n1*n2 values, z fastest */
+    double tip_charge;        /* charge of the tip atom (parameter file) */
+    double gaussian_width;    /* width of the tip's Gaussian charge, 0.5 A in the reference */
+} mafm_potential_desc;
+
+/* ---- entry points ------------------------------------------------------ */
+mafm_status mafm_create(int device, mafm_ctx** out);
+void mafm_destroy(mafm_ctx* ctx);
+const char* mafm_last_error(const mafm_ctx* ctx);
+mafm_status mafm_device_info(mafm_ctx* ctx, mafm_device_props* props);
+
+mafm_status mafm_grid_upload(mafm_ctx* ctx, const mafm_grid_desc* grid, int* handle);
+mafm_status mafm_grid_build(mafm_ctx* ctx, const mafm_system_desc* sys, const mafm_grid_desc* grid,
+                            int* handle);
+mafm_status mafm_grid_download(mafm_ctx* ctx, int handle, double* fe);
+mafm_status mafm_grid_free(mafm_ctx* ctx, int handle);
+/* Energy and force of the tip's Gaussian charge in the potential, by FFT
+ * convolution and spectral gradient, stored as a periodic grid. */
+mafm_status mafm_potential_grid(mafm_ctx* ctx, const mafm_potential_desc* pot, int* handle);
+/* 3D complex DFT, unnormalised forward (e^{-2 pi i jk/n}) or inverse scaled
+ * by 1/(n0 n1 n2); interleaved re/im doubles, z fastest. */
+mafm_status mafm_fft3d(mafm_ctx* ctx, const int32_t n[3], int inverse, const double* in, double* out);
+
+mafm_status mafm_set_systems(mafm_ctx* ctx, int n_systems, const mafm_system_desc* systems);
+mafm_status mafm_scan(mafm_ctx* ctx, const mafm_scan_desc* desc, mafm_scan_stats* stats);
+/* records of all systems: index ((system*nx + i)*ny + j)*nz + k */
+mafm_status mafm_records(mafm_ctx* ctx, size_t first, size_t count, mafm_record* out);
+mafm_status mafm_eval_tip_surface(mafm_ctx* ctx, int system, size_t n, const double* pos, double* fe);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

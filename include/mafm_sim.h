@@ -1,0 +1,67 @@
+/*
+ * mafm_sim.h -- C ABI over the C++ Simulation API of libmafm (host side).
+ *
+ * A mafm_sim is one MechAFM job: the input.scan / parameter / xyz (and cube)
+ * files are parsed exactly as the reference does, the system is centred and
+ * lowered to device tables, and the scan runs through mafm.h on the context
+ * given at open.  The per-height output files are those of the reference.
+ *
+ * Reference call sites each entry replaces:
+ *   mafm_sim_open          parseCommandLine / readInputFile / readXYZFile /
+ *                          readParameterFile (parse.cpp:30-697) and the
+ *                          scan-grid size of openUniverse (mechafm.cpp:47-49)
+ *   mafm_sim_initialize    Simulation::initialize (simulation.cpp:31-65),
+ *                          including buildInteractions and the force-grid /
+ *                          external-potential construction
+ *   mafm_sim_run           Simulation::run (simulation.cpp:67-143)
+ *   mafm_batch_run         many jobs in one launch (no reference counterpart:
+ *                          the reference runs one molecule per process)
+ *   mafm_sim_write_output  openUniverse / writeOutput / closeUniverse
+ *                          (mechafm.cpp:59-101, simulation.cpp:145-199) and
+ *                          the statistics block of finalize (mechafm.cpp:103-154)
+ */
+#ifndef MAFM_SIM_H
+#define MAFM_SIM_H
+
+#include "mafm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mafm_sim mafm_sim;
+
+typedef struct {
+    int32_t nx, ny, nz;           /* scan positions (mechafm.cpp:47-49) */
+    int32_t n_atoms;              /* surface atoms */
+    int32_t n_sites;              /* lowered tip-surface sites (pbc images included) */
+    int32_t rigidgrid, use_external_potential, vdw_pbc, coulomb;
+    int32_t force_grid_n[3];      /* force-grid dimensions when rigidgrid is on */
+    double tip_dummy_d;
+    double offset[3];             /* System offset after centring */
+    double grid_build_ms;         /* device time spent building grids / potential */
+} mafm_sim_info;
+
+typedef struct {
+    int32_t precision;            /* MAFM_PREC_* */
+    int32_t lanes_per_pixel;      /* 0 = automatic */
+    int32_t waves_per_simd;       /* 0 = automatic */
+    int32_t compensation;         /* FAST32 accumulation: 0 default (Kahan), 1 Kahan, 2 none */
+} mafm_run_opts;
+
+mafm_status mafm_sim_open(const char* input_file, const char* output_dir, mafm_ctx* ctx, mafm_sim** out);
+mafm_status mafm_sim_initialize(mafm_sim* sim);
+mafm_status mafm_sim_info_get(mafm_sim* sim, mafm_sim_info* info);
+/* the lowered system (valid until the sim is closed) */
+mafm_status mafm_sim_system(mafm_sim* sim, mafm_system_desc* desc);
+mafm_status mafm_sim_run(mafm_sim* sim, const mafm_run_opts* opts, mafm_scan_stats* stats);
+mafm_status mafm_batch_run(int n, mafm_sim* const* sims, const mafm_run_opts* opts, mafm_scan_stats* stats);
+mafm_status mafm_sim_records(mafm_sim* sim, size_t first, size_t count, mafm_record* out);
+mafm_status mafm_sim_write_output(mafm_sim* sim, int threads);
+const char* mafm_sim_last_error(const mafm_sim* sim);
+void mafm_sim_close(mafm_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,344 @@
+"""ctypes binding of libmafm (include/mafm.h, include/mafm_sim.h).
+
+The library is the product: every call here goes to the gfx950 kernels.  If
+``mechafm_amd/lib/libmafm.so`` is missing this module raises instead of
+falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmafm.so")
+
+MAFM_OK = 0
+STATUS = {0: "OK", 1: "INVALID", 2: "HIP", 3: "OUT_OF_GRID", 4: "STATE", 5: "FFT_IMAG", 6: "NO_DEVICE"}
+PREC = {"plain64": 0, "fast64": 1, "fast32": 2}
+VDW_LJ, VDW_MORSE = 0, 1
+
+
+class Site(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_double), ("y", ctypes.c_double), ("z", ctypes.c_double),
+                ("vdw", ctypes.c_int32), ("has_coulomb", ctypes.c_int32),
+                ("p0", ctypes.c_double), ("p1", ctypes.c_double), ("p2", ctypes.c_double),
+                ("qq", ctypes.c_double)]
+
+
+class TipDummy(ctypes.Structure):
+    _fields_ = [("vdw", ctypes.c_int32), ("has_coulomb", ctypes.c_int32),
+                ("p0", ctypes.c_double), ("p1", ctypes.c_double), ("p2", ctypes.c_double),
+                ("qq", ctypes.c_double), ("k", ctypes.c_double), ("r0", ctypes.c_double)]
+
+
+class SystemDesc(ctypes.Structure):
+    _fields_ = [("n_sites", ctypes.c_int32), ("force_grid", ctypes.c_int32),
+                ("sites", ctypes.POINTER(Site)), ("potential_grid", ctypes.c_int32),
+                ("tip_pbc", ctypes.c_int32), ("tip_dummy", TipDummy), ("tip_dummy_d", ctypes.c_double),
+                ("cell", ctypes.c_double * 9), ("offset", ctypes.c_double * 3)]
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32 * 3), ("periodic", ctypes.c_int32), ("basis", ctypes.c_double * 9),
+                ("offset", ctypes.c_double * 3), ("fe", ctypes.POINTER(ctypes.c_double))]
+
+
+class ScanDesc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+                ("zhigh", ctypes.c_double), ("minimiser", ctypes.c_int32), ("integrator", ctypes.c_int32),
+                ("minterm", ctypes.c_int32), ("etol", ctypes.c_double), ("ftol", ctypes.c_double),
+                ("dt", ctypes.c_double), ("maxsteps", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("lanes_per_pixel", ctypes.c_int32), ("waves_per_simd", ctypes.c_int32),
+                ("compensation", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+RECORD_DTYPE = np.dtype([("x", "f8"), ("y", "f8"), ("z", "f8"), ("fx", "f8"), ("fy", "f8"), ("fz", "f8"),
+                         ("rx", "f8"), ("ry", "f8"), ("rz", "f8"), ("e", "f8"),
+                         ("n_steps", "i4"), ("flags", "i4")])
+
+
+class ScanStats(ctypes.Structure):
+    _fields_ = [("kernel_ms", ctypes.c_double), ("positions", ctypes.c_int64), ("total_steps", ctypes.c_int64),
+                ("unconverged", ctypes.c_int64), ("grid_warnings", ctypes.c_int64), ("pair_evals", ctypes.c_int64),
+                ("lanes_per_pixel", ctypes.c_int32), ("waves_per_simd", ctypes.c_int32),
+                ("grid_blocks", ctypes.c_int32), ("block_threads", ctypes.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class DeviceProps(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 64), ("compute_units", ctypes.c_int32), ("clock_mhz", ctypes.c_int32),
+                ("total_mem", ctypes.c_int64), ("lds_per_block", ctypes.c_int32), ("warp_size", ctypes.c_int32)]
+
+
+class PotentialDesc(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32 * 3), ("basis", ctypes.c_double * 9), ("origin", ctypes.c_double * 3),
+                ("potential", ctypes.POINTER(ctypes.c_double)), ("tip_charge", ctypes.c_double),
+                ("gaussian_width", ctypes.c_double)]
+
+
+class SimInfo(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("n_atoms", ctypes.c_int32), ("n_sites", ctypes.c_int32), ("rigidgrid", ctypes.c_int32),
+                ("use_external_potential", ctypes.c_int32), ("vdw_pbc", ctypes.c_int32),
+                ("coulomb", ctypes.c_int32), ("force_grid_n", ctypes.c_int32 * 3),
+                ("tip_dummy_d", ctypes.c_double), ("offset", ctypes.c_double * 3),
+                ("grid_build_ms", ctypes.c_double)]
+
+
+class RunOpts(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int32), ("lanes_per_pixel", ctypes.c_int32),
+                ("waves_per_simd", ctypes.c_int32), ("compensation", ctypes.c_int32)]
+
+
+# every exported entry point and its signature: (restype, argtypes)
+_P = ctypes.POINTER
+_vp = ctypes.c_void_p
+SIGNATURES = {
+    "mafm_create": (ctypes.c_int, [ctypes.c_int, _P(_vp)]),
+    "mafm_destroy": (None, [_vp]),
+    "mafm_last_error": (ctypes.c_char_p, [_vp]),
+    "mafm_device_info": (ctypes.c_int, [_vp, _P(DeviceProps)]),
+    "mafm_grid_upload": (ctypes.c_int, [_vp, _P(GridDesc), _P(ctypes.c_int)]),
+    "mafm_grid_build": (ctypes.c_int, [_vp, _P(SystemDesc), _P(GridDesc), _P(ctypes.c_int)]),
+    "mafm_grid_download": (ctypes.c_int, [_vp, ctypes.c_int, _P(ctypes.c_double)]),
+    "mafm_grid_free": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "mafm_potential_grid": (ctypes.c_int, [_vp, _P(PotentialDesc), _P(ctypes.c_int)]),
+    "mafm_fft3d": (ctypes.c_int, [_vp, _P(ctypes.c_int32), ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_double)]),
+    "mafm_set_systems": (ctypes.c_int, [_vp, ctypes.c_int, _P(SystemDesc)]),
+    "mafm_scan": (ctypes.c_int, [_vp, _P(ScanDesc), _P(ScanStats)]),
+    "mafm_records": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _vp]),
+    "mafm_eval_tip_surface": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_size_t, _P(ctypes.c_double),
+                                             _P(ctypes.c_double)]),
+    "mafm_sim_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _vp, _P(_vp)]),
+    "mafm_sim_initialize": (ctypes.c_int, [_vp]),
+    "mafm_sim_info_get": (ctypes.c_int, [_vp, _P(SimInfo)]),
+    "mafm_sim_system": (ctypes.c_int, [_vp, _P(SystemDesc)]),
+    "mafm_sim_run": (ctypes.c_int, [_vp, _P(RunOpts), _P(ScanStats)]),
+    "mafm_batch_run": (ctypes.c_int, [ctypes.c_int, _P(_vp), _P(RunOpts), _P(ScanStats)]),
+    "mafm_sim_records": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _vp]),
+    "mafm_sim_write_output": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "mafm_sim_last_error": (ctypes.c_char_p, [_vp]),
+    "mafm_sim_close": (None, [_vp]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libmafm is not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class MafmError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Context:
+    """A libmafm context on one device (mafm_create / mafm_destroy)."""
+
+    def __init__(self, device: int = 0):
+        self.h = ctypes.c_void_p()
+        st = lib().mafm_create(device, ctypes.byref(self.h))
+        if st != MAFM_OK:
+            raise MafmError(st, "mafm_create failed (no HIP device?)")
+        self._keep = []
+
+    def check(self, st: int) -> None:
+        if st != MAFM_OK:
+            raise MafmError(st, lib().mafm_last_error(self.h).decode())
+
+    def close(self) -> None:
+        if self.h:
+            lib().mafm_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_info(self) -> dict:
+        p = DeviceProps()
+        self.check(lib().mafm_device_info(self.h, ctypes.byref(p)))
+        return {"name": p.name.decode(), "compute_units": p.compute_units, "clock_mhz": p.clock_mhz,
+                "total_mem": p.total_mem, "lds_per_block": p.lds_per_block, "warp_size": p.warp_size}
+
+    # ---- grids
+    def grid_upload(self, n, basis, offset, fe: np.ndarray, periodic: bool) -> int:
+        fe = np.ascontiguousarray(fe, dtype=np.float64)
+        g = GridDesc()
+        for a in range(3):
+            g.n[a] = int(n[a])
+            g.offset[a] = float(offset[a])
+        for a, v in enumerate(np.asarray(basis, dtype=np.float64).ravel()):
+            g.basis[a] = v
+        g.periodic = int(periodic)
+        g.fe = _dp(fe)
+        h = ctypes.c_int(-1)
+        self.check(lib().mafm_grid_upload(self.h, ctypes.byref(g), ctypes.byref(h)))
+        return h.value
+
+    def grid_download(self, handle: int, n) -> np.ndarray:
+        out = np.zeros((int(n[0]) * int(n[1]) * int(n[2]), 4))
+        self.check(lib().mafm_grid_download(self.h, handle, _dp(out)))
+        return out
+
+    def potential_grid(self, pot: np.ndarray, basis, origin, tip_charge: float, width: float = 0.5) -> int:
+        pot = np.ascontiguousarray(pot, dtype=np.float64)
+        d = PotentialDesc()
+        for a in range(3):
+            d.n[a] = pot.shape[a]
+            d.origin[a] = float(origin[a])
+        for a, v in enumerate(np.asarray(basis, dtype=np.float64).ravel()):
+            d.basis[a] = v
+        d.potential = _dp(pot)
+        d.tip_charge = tip_charge
+        d.gaussian_width = width
+        h = ctypes.c_int(-1)
+        self.check(lib().mafm_potential_grid(self.h, ctypes.byref(d), ctypes.byref(h)))
+        return h.value
+
+    def fft3d(self, x: np.ndarray, inverse: bool = False) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        n = (ctypes.c_int32 * 3)(*x.shape)
+        out = np.zeros_like(x)
+        self.check(lib().mafm_fft3d(self.h, n, int(inverse), x.view(np.float64).ctypes.data_as(_P(ctypes.c_double)),
+                                    out.view(np.float64).ctypes.data_as(_P(ctypes.c_double))))
+        return out
+
+    # ---- systems and scans
+    def set_systems(self, descs: Sequence[SystemDesc]) -> None:
+        arr = (SystemDesc * len(descs))(*descs)
+        self._keep = [arr]
+        self.check(lib().mafm_set_systems(self.h, len(descs), arr))
+
+    def scan(self, desc: ScanDesc) -> dict:
+        st = ScanStats()
+        self.check(lib().mafm_scan(self.h, ctypes.byref(desc), ctypes.byref(st)))
+        return st.as_dict()
+
+    def records(self, first: int, count: int) -> np.ndarray:
+        out = np.zeros(count, dtype=RECORD_DTYPE)
+        self.check(lib().mafm_records(self.h, first, count, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def eval_tip_surface(self, system: int, pos: np.ndarray) -> np.ndarray:
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        out = np.zeros((len(pos), 4))
+        self.check(lib().mafm_eval_tip_surface(self.h, system, len(pos), _dp(pos), _dp(out)))
+        return out
+
+
+class Simulation:
+    """One MechAFM job (mafm_sim_*): input files -> device scan -> records / files."""
+
+    def __init__(self, input_path: str, ctx: Context, outdir: Optional[str] = None, initialize: bool = True):
+        self.ctx = ctx
+        self.h = ctypes.c_void_p()
+        st = lib().mafm_sim_open(input_path.encode(), (outdir or "").encode(), ctx.h, ctypes.byref(self.h))
+        self.check(st)
+        if initialize:
+            self.check(lib().mafm_sim_initialize(self.h))
+
+    def check(self, st: int) -> None:
+        if st != MAFM_OK:
+            msg = lib().mafm_sim_last_error(self.h).decode() if self.h else "open failed"
+            raise MafmError(st, msg)
+
+    def close(self) -> None:
+        if self.h:
+            lib().mafm_sim_close(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        i = SimInfo()
+        self.check(lib().mafm_sim_info_get(self.h, ctypes.byref(i)))
+        d = {k: getattr(i, k) for k, _ in i._fields_}
+        d["force_grid_n"] = tuple(i.force_grid_n)
+        d["offset"] = tuple(i.offset)
+        return d
+
+    def system(self) -> SystemDesc:
+        d = SystemDesc()
+        self.check(lib().mafm_sim_system(self.h, ctypes.byref(d)))
+        return d
+
+    def sites(self) -> List[Site]:
+        d = self.system()
+        return [d.sites[k] for k in range(d.n_sites)]
+
+    def run(self, precision: str = "fast64", lanes_per_pixel: int = 0, waves_per_simd: int = 0,
+            compensation: int = 0) -> dict:
+        o = RunOpts(PREC[precision], lanes_per_pixel, waves_per_simd, compensation)
+        st = ScanStats()
+        self.check(lib().mafm_sim_run(self.h, ctypes.byref(o), ctypes.byref(st)))
+        return st.as_dict()
+
+    def records(self) -> np.ndarray:
+        i = self.info()
+        n = i["nx"] * i["ny"] * i["nz"]
+        out = np.zeros(n, dtype=RECORD_DTYPE)
+        self.check(lib().mafm_sim_records(self.h, 0, n, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def write_output(self, threads: int = 8) -> None:
+        self.check(lib().mafm_sim_write_output(self.h, threads))
+
+
+def batch_run(sims: Sequence[Simulation], precision: str = "fast32", lanes_per_pixel: int = 0,
+              waves_per_simd: int = 0, compensation: int = 0) -> dict:
+    arr = (ctypes.c_void_p * len(sims))(*[s.h.value for s in sims])
+    o = RunOpts(PREC[precision], lanes_per_pixel, waves_per_simd, compensation)
+    st = ScanStats()
+    rc = lib().mafm_batch_run(len(sims), arr, ctypes.byref(o), ctypes.byref(st))
+    if rc != MAFM_OK:
+        raise MafmError(rc, lib().mafm_sim_last_error(sims[0].h).decode())
+    return st.as_dict()
+
+
+def records_to_table(rec: np.ndarray, shape) -> np.ndarray:
+    """Expand device records to the 16-column layout of the reference's
+    OutputData (i j k n x y z fx fy fz rx ry rz r angle e); r and angle use
+    System::getOutput's formulas (system.cpp:28-38)."""
+    nx, ny, nz = shape
+    idx = np.arange(len(rec))
+    k = idx % nz
+    j = (idx // nz) % ny
+    i = idx // (nz * ny)
+    t = np.zeros((len(rec), 16))
+    t[:, 0], t[:, 1], t[:, 2], t[:, 3] = i, j, k, rec["n_steps"]
+    t[:, 4], t[:, 5], t[:, 6] = rec["x"], rec["y"], rec["z"]
+    t[:, 7], t[:, 8], t[:, 9] = rec["fx"], rec["fy"], rec["fz"]
+    t[:, 10], t[:, 11], t[:, 12] = rec["rx"], rec["ry"], rec["rz"]
+    t[:, 13] = np.sqrt((rec["rx"] ** 2 + rec["ry"] ** 2) + rec["rz"] ** 2)
+    t[:, 14] = np.arctan2(np.sqrt(rec["rx"] ** 2 + rec["ry"] ** 2), rec["rz"]) * (180.0 / 3.14159265358979323846)
+    t[:, 15] = rec["e"]
+    return t

@@ -1,0 +1,581 @@
+// C ABI of libmafm: context, device tables, launches.  Host code only; the
+// kernels live in scan_plain.hip, scan_fast.hip and fft.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mafm.h"
+#include "device_common.hpp"
+
+namespace mafm {
+hipError_t launch_scan_plain(const ScanArgs& a, hipStream_t st);
+hipError_t launch_eval_plain(const ScanArgs& a, int mol, long n, const double* pos, double* out, hipStream_t st);
+hipError_t launch_grid_build(const ScanArgs& a, int mol, const int* n, const double* s, const double* o,
+                             double4* out, hipStream_t st);
+hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_waves, hipStream_t st);
+hipError_t launch_to_f32(const double4* in, float4* out, long n, hipStream_t st);
+}  // namespace mafm
+
+using namespace mafm;
+
+namespace {
+
+struct HostGrid {
+    bool live = false;
+    int n[3] = {0, 0, 0};
+    int periodic = 0;
+    int orthogonal = 1;
+    double basis[9] = {0}, inv[9] = {0}, offset[3] = {0};
+    double4* fe64 = nullptr;
+    float4* fe32 = nullptr;
+};
+
+void inv3(const double* m, double* o) {
+    double det = m[0] * (m[4] * m[8] - m[5] * m[7]);
+    det += m[1] * (m[5] * m[6] - m[3] * m[8]);
+    det += m[2] * (m[3] * m[7] - m[4] * m[6]);
+    double inv_det = 1 / det;
+    double a = m[0], b = m[1], c = m[2], d = m[3], e = m[4], f = m[5], g = m[6], h = m[7], i = m[8];
+    o[0] = inv_det * (e * i - f * h);
+    o[1] = -inv_det * (b * i - c * h);
+    o[2] = inv_det * (b * f - c * e);
+    o[3] = -inv_det * (d * i - f * g);
+    o[4] = inv_det * (a * i - c * g);
+    o[5] = -inv_det * (a * f - c * d);
+    o[6] = inv_det * (d * h - e * g);
+    o[7] = -inv_det * (a * h - b * g);
+    o[8] = inv_det * (a * e - b * d);
+}
+
+// Matrix3::isDiagonal (matrices.hpp:72-86)
+bool is_diag(const double* m) {
+    double diag = 0, nondiag = 0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) (i == j ? diag : nondiag) += std::fabs(m[3 * i + j]);
+    return nondiag < 1.0e-10 * diag;
+}
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t n) {
+        if (n <= cap && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T));
+        if (e == hipSuccess) cap = std::max<size_t>(n, 1);
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+}  // namespace
+
+struct mafm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    int cus = 0;
+    std::vector<HostGrid> grids;
+    DevBuf<DevGrid> d_grids;
+    std::vector<DevMol> mols;
+    DevBuf<DevMol> d_mols;
+    DevBuf<Site64> d_ref, d_seg64;
+    DevBuf<Site32> d_seg32;
+    DevBuf<DevRecord> d_rec;
+    DevBuf<int32_t> d_next;
+    DevBuf<float> d_cum;
+    DevBuf<int64_t> d_counters;
+    DevBuf<double> d_tmp;
+    std::vector<int64_t> mol_cost;
+    size_t n_records = 0;
+    bool have_systems = false;
+};
+
+static mafm_status fail(mafm_ctx* c, mafm_status s, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return s;
+}
+
+#define HIPCHK(ctx, expr)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(ctx, MAFM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));        \
+    } while (0)
+
+static mafm_status sync_grids(mafm_ctx* c) {
+    std::vector<DevGrid> dg(std::max<size_t>(c->grids.size(), 1));
+    for (size_t k = 0; k < c->grids.size(); ++k) {
+        const HostGrid& h = c->grids[k];
+        DevGrid& g = dg[k];
+        memset(&g, 0, sizeof g);
+        for (int d = 0; d < 3; ++d) g.n[d] = h.n[d];
+        g.periodic = h.periodic;
+        g.orthogonal = h.orthogonal;
+        memcpy(g.basis, h.basis, sizeof g.basis);
+        memcpy(g.inv, h.inv, sizeof g.inv);
+        memcpy(g.offset, h.offset, sizeof g.offset);
+        g.fe64 = h.fe64;
+        g.fe32 = h.fe32;
+    }
+    HIPCHK(c, c->d_grids.reserve(dg.size()));
+    HIPCHK(c, hipMemcpyAsync(c->d_grids.p, dg.data(), dg.size() * sizeof(DevGrid), hipMemcpyHostToDevice,
+                             c->stream));
+    return MAFM_OK;
+}
+
+static mafm_status new_grid(mafm_ctx* c, const mafm_grid_desc* g, HostGrid& h) {
+    if (!g) return fail(c, MAFM_ERR_INVALID, "grid descriptor is NULL");
+    for (int d = 0; d < 3; ++d) {
+        if (g->n[d] <= 0) return fail(c, MAFM_ERR_INVALID, "grid dimension %d is %d", d, g->n[d]);
+        h.n[d] = g->n[d];
+        h.offset[d] = g->offset[d];
+    }
+    h.periodic = g->periodic ? 1 : 0;
+    memcpy(h.basis, g->basis, sizeof h.basis);
+    h.orthogonal = is_diag(h.basis);
+    inv3(h.basis, h.inv);
+    size_t np = (size_t)h.n[0] * h.n[1] * h.n[2];
+    HIPCHK(c, hipMalloc(&h.fe64, np * sizeof(double4)));
+    HIPCHK(c, hipMalloc(&h.fe32, np * sizeof(float4)));
+    h.live = true;
+    return MAFM_OK;
+}
+
+static int store_grid(mafm_ctx* c, const HostGrid& h) {
+    for (size_t k = 0; k < c->grids.size(); ++k)
+        if (!c->grids[k].live) {
+            c->grids[k] = h;
+            return (int)k;
+        }
+    c->grids.push_back(h);
+    return (int)c->grids.size() - 1;
+}
+
+static ScanArgs base_args(mafm_ctx* c) {
+    ScanArgs a;
+    memset(&a, 0, sizeof a);
+    a.mols = c->d_mols.p;
+    a.sites_ref = c->d_ref.p;
+    a.seg64 = c->d_seg64.p;
+    a.seg32 = c->d_seg32.p;
+    a.grids = c->d_grids.p;
+    a.counters = c->d_counters.p;
+    a.n_mols = (int)c->mols.size();
+    return a;
+}
+
+// Lower mafm_system_desc tables to the device layout (device_common.hpp).
+static mafm_status build_tables(mafm_ctx* c, int n, const mafm_system_desc* sys, std::vector<DevMol>& mols,
+                                std::vector<Site64>& ref, std::vector<Site64>& seg64,
+                                std::vector<Site32>& seg32, std::vector<int64_t>& cost) {
+    const double LOG2E = 1.4426950408889634;
+    for (int m = 0; m < n; ++m) {
+        const mafm_system_desc& s = sys[m];
+        if (s.n_sites < 0 || (s.n_sites > 0 && !s.sites))
+            return fail(c, MAFM_ERR_INVALID, "system %d: bad site table", m);
+        for (int g : {s.force_grid, s.potential_grid})
+            if (g >= 0 && (g >= (int)c->grids.size() || !c->grids[g].live))
+                return fail(c, MAFM_ERR_INVALID, "system %d: grid handle %d is not live", m, g);
+        DevMol d;
+        memset(&d, 0, sizeof d);
+        d.ref_off = (int)ref.size();
+        d.n_sites = s.n_sites;
+        d.seg_off = (int)seg64.size();
+        d.force_grid = s.force_grid;
+        d.potential_grid = s.potential_grid;
+        d.tip_pbc = s.tip_pbc;
+        d.td_vdw = s.tip_dummy.vdw;
+        d.td_coul = s.tip_dummy.has_coulomb;
+        d.td_p0 = s.tip_dummy.p0;
+        d.td_p1 = s.tip_dummy.p1;
+        d.td_p2 = s.tip_dummy.p2;
+        d.td_qq = s.tip_dummy.qq;
+        d.td_k = s.tip_dummy.k;
+        d.td_r0 = s.tip_dummy.r0;
+        d.tip_dummy_d = s.tip_dummy_d;
+        memcpy(d.cell, s.cell, sizeof d.cell);
+        memcpy(d.offset, s.offset, sizeof d.offset);
+        if (s.tip_pbc) inv3(d.cell, d.cell_inv);
+        // FP32 coordinate origin: centre of the sites' bounding box
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int k = 0; k < s.n_sites; ++k) {
+            const double p[3] = {s.sites[k].x, s.sites[k].y, s.sites[k].z};
+            for (int q = 0; q < 3; ++q) {
+                lo[q] = std::min(lo[q], p[q]);
+                hi[q] = std::max(hi[q], p[q]);
+            }
+        }
+        for (int q = 0; q < 3; ++q) d.origin[q] = s.n_sites ? 0.5 * (lo[q] + hi[q]) : 0.0;
+        for (int k = 0; k < s.n_sites; ++k) {
+            const mafm_site& t = s.sites[k];
+            if (t.vdw != MAFM_VDW_LJ && t.vdw != MAFM_VDW_MORSE)
+                return fail(c, MAFM_ERR_INVALID, "system %d site %d: unknown vdw kind %d", m, k, t.vdw);
+            Site64 r = {t.x, t.y, t.z, t.p0, t.p1, t.p2, t.has_coulomb ? t.qq : 0.0, t.vdw, t.has_coulomb ? 1 : 0};
+            ref.push_back(r);
+        }
+        for (int segk = 0; segk < 4; ++segk) {
+            const int want_vdw = segk < 2 ? MAFM_VDW_LJ : MAFM_VDW_MORSE;
+            const int want_coul = (segk % 2 == 0) ? 1 : 0;
+            int cnt = 0;
+            for (int k = 0; k < s.n_sites; ++k) {
+                const mafm_site& t = s.sites[k];
+                const int coul = (t.has_coulomb && t.qq != 0.0) ? 1 : 0;
+                if (t.vdw != want_vdw || coul != want_coul) continue;
+                seg64.push_back(Site64{t.x, t.y, t.z, t.p0, t.p1, t.p2, coul ? t.qq : 0.0, t.vdw, coul});
+                Site32 f;
+                f.a = make_float4((float)(t.x - d.origin[0]), (float)(t.y - d.origin[1]),
+                                  (float)(t.z - d.origin[2]), (float)t.p0);
+                if (t.vdw == MAFM_VDW_LJ)
+                    f.b = make_float4((float)t.p1, 0.f, (float)(coul ? t.qq : 0.0), 0.f);
+                else
+                    f.b = make_float4((float)(t.p1 * LOG2E), (float)t.p2, (float)(coul ? t.qq : 0.0), (float)t.p1);
+                seg32.push_back(f);
+                ++cnt;
+            }
+            d.n_seg[segk] = cnt;
+        }
+        mols.push_back(d);
+        cost.push_back(s.force_grid >= 0 ? 8 : std::max(1, s.n_sites));
+    }
+    return MAFM_OK;
+}
+
+extern "C" {
+
+mafm_status mafm_create(int device, mafm_ctx** out) {
+    if (!out) return MAFM_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return MAFM_ERR_NO_DEVICE;
+    if (device < 0 || device >= count) return MAFM_ERR_INVALID;
+    mafm_ctx* c = new mafm_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        c->d_counters.reserve(8) != hipSuccess) {
+        delete c;
+        return MAFM_ERR_HIP;
+    }
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device) == hipSuccess) c->cus = p.multiProcessorCount;
+    if (c->cus <= 0) c->cus = 256;
+    *out = c;
+    return MAFM_OK;
+}
+
+void mafm_destroy(mafm_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& g : c->grids) {
+        if (g.fe64) (void)hipFree(g.fe64);
+        if (g.fe32) (void)hipFree(g.fe32);
+    }
+    c->d_grids.release();
+    c->d_mols.release();
+    c->d_ref.release();
+    c->d_seg64.release();
+    c->d_seg32.release();
+    c->d_rec.release();
+    c->d_next.release();
+    c->d_cum.release();
+    c->d_counters.release();
+    c->d_tmp.release();
+    (void)hipEventDestroy(c->ev0);
+    (void)hipEventDestroy(c->ev1);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* mafm_last_error(const mafm_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+mafm_status mafm_device_info(mafm_ctx* c, mafm_device_props* props) {
+    if (!c || !props) return MAFM_ERR_INVALID;
+    hipDeviceProp_t p;
+    HIPCHK(c, hipGetDeviceProperties(&p, c->device));
+    memset(props, 0, sizeof *props);
+    snprintf(props->name, sizeof props->name, "%s", p.gcnArchName);
+    props->compute_units = p.multiProcessorCount;
+    props->clock_mhz = p.clockRate / 1000;
+    props->total_mem = (int64_t)p.totalGlobalMem;
+    props->lds_per_block = (int32_t)p.sharedMemPerBlock;
+    props->warp_size = p.warpSize;
+    return MAFM_OK;
+}
+
+mafm_status mafm_grid_upload(mafm_ctx* c, const mafm_grid_desc* g, int* handle) {
+    if (!c || !handle) return MAFM_ERR_INVALID;
+    if (!g || !g->fe) return fail(c, MAFM_ERR_INVALID, "grid upload needs sample values");
+    HIPCHK(c, hipSetDevice(c->device));
+    HostGrid h;
+    mafm_status s = new_grid(c, g, h);
+    if (s != MAFM_OK) return s;
+    size_t np = (size_t)h.n[0] * h.n[1] * h.n[2];
+    HIPCHK(c, hipMemcpyAsync(h.fe64, g->fe, np * sizeof(double4), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_to_f32(h.fe64, h.fe32, (long)np, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *handle = store_grid(c, h);
+    return sync_grids(c);
+}
+
+mafm_status mafm_grid_download(mafm_ctx* c, int handle, double* fe) {
+    if (!c || !fe) return MAFM_ERR_INVALID;
+    if (handle < 0 || handle >= (int)c->grids.size() || !c->grids[handle].live)
+        return fail(c, MAFM_ERR_INVALID, "grid handle %d is not live", handle);
+    const HostGrid& h = c->grids[handle];
+    size_t np = (size_t)h.n[0] * h.n[1] * h.n[2];
+    HIPCHK(c, hipMemcpyAsync(fe, h.fe64, np * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MAFM_OK;
+}
+
+mafm_status mafm_grid_free(mafm_ctx* c, int handle) {
+    if (!c) return MAFM_ERR_INVALID;
+    if (handle < 0 || handle >= (int)c->grids.size() || !c->grids[handle].live)
+        return fail(c, MAFM_ERR_INVALID, "grid handle %d is not live", handle);
+    for (const DevMol& m : c->mols)
+        if (m.force_grid == handle || m.potential_grid == handle)
+            return fail(c, MAFM_ERR_STATE, "grid %d is used by a loaded system", handle);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HostGrid& h = c->grids[handle];
+    (void)hipFree(h.fe64);
+    (void)hipFree(h.fe32);
+    h = HostGrid();
+    return sync_grids(c);
+}
+
+static mafm_status upload_tables(mafm_ctx* c, const std::vector<DevMol>& mols, const std::vector<Site64>& ref,
+                                 const std::vector<Site64>& seg64, const std::vector<Site32>& seg32) {
+    HIPCHK(c, c->d_mols.reserve(mols.size()));
+    HIPCHK(c, c->d_ref.reserve(ref.size()));
+    HIPCHK(c, c->d_seg64.reserve(seg64.size()));
+    HIPCHK(c, c->d_seg32.reserve(seg32.size()));
+    if (!mols.empty())
+        HIPCHK(c, hipMemcpyAsync(c->d_mols.p, mols.data(), mols.size() * sizeof(DevMol), hipMemcpyHostToDevice, c->stream));
+    if (!ref.empty())
+        HIPCHK(c, hipMemcpyAsync(c->d_ref.p, ref.data(), ref.size() * sizeof(Site64), hipMemcpyHostToDevice, c->stream));
+    if (!seg64.empty())
+        HIPCHK(c, hipMemcpyAsync(c->d_seg64.p, seg64.data(), seg64.size() * sizeof(Site64), hipMemcpyHostToDevice, c->stream));
+    if (!seg32.empty())
+        HIPCHK(c, hipMemcpyAsync(c->d_seg32.p, seg32.data(), seg32.size() * sizeof(Site32), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MAFM_OK;
+}
+
+mafm_status mafm_grid_build(mafm_ctx* c, const mafm_system_desc* sys, const mafm_grid_desc* g, int* handle) {
+    if (!c || !sys || !handle) return MAFM_ERR_INVALID;
+    if (sys->force_grid >= 0) return fail(c, MAFM_ERR_INVALID, "grid build needs the tip-surface sites, not a grid");
+    HIPCHK(c, hipSetDevice(c->device));
+    // sample with a one-molecule table, then restore the loaded systems
+    std::vector<DevMol> mols;
+    std::vector<Site64> ref, seg64;
+    std::vector<Site32> seg32;
+    std::vector<int64_t> cost;
+    mafm_status s = build_tables(c, 1, sys, mols, ref, seg64, seg32, cost);
+    if (s != MAFM_OK) return s;
+    HostGrid h;
+    s = new_grid(c, g, h);
+    if (s != MAFM_OK) return s;
+    if (!h.orthogonal) return fail(c, MAFM_ERR_INVALID, "grid build expects a diagonal spacing basis");
+    DevBuf<DevMol> dm;
+    DevBuf<Site64> dr;
+    HIPCHK(c, dm.reserve(1));
+    HIPCHK(c, dr.reserve(ref.size()));
+    HIPCHK(c, hipMemcpyAsync(dm.p, mols.data(), sizeof(DevMol), hipMemcpyHostToDevice, c->stream));
+    if (!ref.empty())
+        HIPCHK(c, hipMemcpyAsync(dr.p, ref.data(), ref.size() * sizeof(Site64), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_counters.p, 0, 8 * sizeof(int64_t), c->stream));
+    ScanArgs a = base_args(c);
+    a.mols = dm.p;
+    a.sites_ref = dr.p;
+    a.n_mols = 1;
+    const double sp[3] = {h.basis[0], h.basis[4], h.basis[8]};
+    HIPCHK(c, launch_grid_build(a, 0, h.n, sp, h.offset, h.fe64, c->stream));
+    size_t np = (size_t)h.n[0] * h.n[1] * h.n[2];
+    HIPCHK(c, launch_to_f32(h.fe64, h.fe32, (long)np, c->stream));
+    int64_t counters[8];
+    HIPCHK(c, hipMemcpyAsync(counters, c->d_counters.p, sizeof counters, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dm.release();
+    dr.release();
+    if (counters[3] != 0) {
+        (void)hipFree(h.fe64);
+        (void)hipFree(h.fe32);
+        return fail(c, MAFM_ERR_OUT_OF_GRID, "potential-grid lookup left the grid during grid build");
+    }
+    *handle = store_grid(c, h);
+    return sync_grids(c);
+}
+
+mafm_status mafm_set_systems(mafm_ctx* c, int n, const mafm_system_desc* sys) {
+    if (!c || n <= 0 || !sys) return fail(c, MAFM_ERR_INVALID, "need at least one system");
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<DevMol> mols;
+    std::vector<Site64> ref, seg64;
+    std::vector<Site32> seg32;
+    std::vector<int64_t> cost;
+    mafm_status s = build_tables(c, n, sys, mols, ref, seg64, seg32, cost);
+    if (s != MAFM_OK) return s;
+    s = upload_tables(c, mols, ref, seg64, seg32);
+    if (s != MAFM_OK) return s;
+    c->mols = mols;
+    c->mol_cost = cost;
+    c->have_systems = true;
+    return MAFM_OK;
+}
+
+mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* stats) {
+    if (!c || !d) return MAFM_ERR_INVALID;
+    if (!c->have_systems) return fail(c, MAFM_ERR_STATE, "mafm_scan before mafm_set_systems");
+    if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return fail(c, MAFM_ERR_INVALID, "empty scan grid");
+    if (d->minimiser != MAFM_MIN_SD && d->minimiser != MAFM_MIN_FIRE) return fail(c, MAFM_ERR_INVALID, "minimiser");
+    if (d->integrator < 0 || d->integrator > 2) return fail(c, MAFM_ERR_INVALID, "integrator");
+    if (d->minterm < 0 || d->minterm > 2) return fail(c, MAFM_ERR_INVALID, "minterm");
+    if (d->precision < 0 || d->precision > 2) return fail(c, MAFM_ERR_INVALID, "precision");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int n_mols = (int)c->mols.size();
+    const size_t per_mol = (size_t)d->nx * d->ny * d->nz;
+    const size_t n_rec = per_mol * n_mols;
+    for (int m = 0; m < n_mols; ++m) c->mols[m].rec_off = (int64_t)(per_mol * m);
+    HIPCHK(c, hipMemcpyAsync(c->d_mols.p, c->mols.data(), n_mols * sizeof(DevMol), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, c->d_rec.reserve(n_rec));
+    c->n_records = n_rec;
+    ScanArgs a = base_args(c);
+    a.nx = d->nx; a.ny = d->ny; a.nz = d->nz;
+    a.dx = d->dx; a.dy = d->dy; a.dz = d->dz; a.zhigh = d->zhigh;
+    a.minimiser = d->minimiser; a.integrator = d->integrator; a.minterm = d->minterm;
+    a.maxsteps = d->maxsteps; a.etol = d->etol; a.ftol = d->ftol; a.dt = d->dt;
+    a.rec = c->d_rec.p;
+    HIPCHK(c, hipMemsetAsync(c->d_counters.p, 0, 8 * sizeof(int64_t), c->stream));
+
+    const long cols_per_mol = (long)d->nx * d->ny;
+    const long total_cols = cols_per_mol * n_mols;
+    int G = 1, wps = 0, n_waves = 0;
+    if (d->precision != MAFM_PREC_PLAIN64) {
+        wps = d->waves_per_simd > 0 ? d->waves_per_simd : 2;
+        const long lanes = (long)c->cus * 4 * wps * 64;
+        int64_t max_sites = 1;
+        for (const DevMol& m : c->mols) max_sites = std::max<int64_t>(max_sites, m.force_grid >= 0 ? 1 : m.n_sites);
+        if (d->lanes_per_pixel > 0) {
+            G = d->lanes_per_pixel;
+        } else {
+            G = 1;
+            while (G < 64 && (double)total_cols * G < 4.0 * lanes && G * 8 <= max_sites) G <<= 1;
+        }
+        if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16 && G != 32 && G != 64)
+            return fail(c, MAFM_ERR_INVALID, "lanes_per_pixel must be a power of two <= 64");
+        const long groups_per_wave = 64 / G;
+        const long need_waves = (total_cols + groups_per_wave - 1) / groups_per_wave;
+        n_waves = (int)std::min<long>((long)c->cus * 4 * wps, need_waves);
+        n_waves = std::max(n_waves, 1);
+        // per-molecule column counters and cumulative work for the initial wave placement
+        std::vector<int32_t> zero(n_mols, 0);
+        std::vector<float> cum(n_mols + 1, 0.f);
+        double tot = 0;
+        for (int m = 0; m < n_mols; ++m) tot += (double)c->mol_cost[m];
+        double run = 0;
+        for (int m = 0; m < n_mols; ++m) {
+            run += (double)c->mol_cost[m];
+            cum[m + 1] = (float)(run / tot);
+        }
+        cum[n_mols] = 1.0f;
+        HIPCHK(c, c->d_next.reserve(n_mols));
+        HIPCHK(c, c->d_cum.reserve(n_mols + 1));
+        HIPCHK(c, hipMemcpyAsync(c->d_next.p, zero.data(), n_mols * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_cum.p, cum.data(), (n_mols + 1) * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        a.next_col = c->d_next.p;
+        a.mol_cum = c->d_cum.p;
+    }
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if (d->precision == MAFM_PREC_PLAIN64) {
+        HIPCHK(c, launch_scan_plain(a, c->stream));
+    } else {
+        const int comp = d->compensation == 0 ? 1 : (d->compensation == 1 ? 1 : 0);
+        HIPCHK(c, launch_scan_fast(a, d->precision == MAFM_PREC_FAST32, comp, G, n_waves, c->stream));
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    int64_t counters[8];
+    HIPCHK(c, hipMemcpyAsync(counters, c->d_counters.p, sizeof counters, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (stats) {
+        memset(stats, 0, sizeof *stats);
+        stats->kernel_ms = ms;
+        stats->positions = (int64_t)n_rec;
+        stats->total_steps = counters[0];
+        stats->unconverged = counters[1];
+        stats->grid_warnings = counters[2];
+        int64_t evals_per_step = d->minimiser == MAFM_MIN_SD ? 1 : (d->integrator == 1 ? 2 : (d->integrator == 2 ? 4 : 1));
+        int64_t pe = 0;
+        for (int m = 0; m < n_mols; ++m) {
+            int64_t sites = c->mols[m].force_grid >= 0 ? 0 : c->mols[m].n_sites;
+            pe += sites;
+        }
+        // algorithmic pair evaluations: (steps * evals + one getOutput eval) per position
+        stats->pair_evals = n_mols ? (counters[0] * evals_per_step + (int64_t)per_mol * n_mols) * (pe / n_mols) : 0;
+        stats->lanes_per_pixel = G;
+        stats->waves_per_simd = wps;
+        stats->grid_blocks = d->precision == MAFM_PREC_PLAIN64 ? (int)((total_cols + 255) / 256) : n_waves;
+        stats->block_threads = d->precision == MAFM_PREC_PLAIN64 ? 256 : 64;
+    }
+    if (counters[3] != 0)
+        return fail(c, MAFM_ERR_OUT_OF_GRID, "%lld positions needed a force-grid point past the grid",
+                    (long long)counters[3]);
+    return MAFM_OK;
+}
+
+mafm_status mafm_records(mafm_ctx* c, size_t first, size_t count, mafm_record* out) {
+    if (!c || (!out && count)) return MAFM_ERR_INVALID;
+    if (first + count > c->n_records)
+        return fail(c, MAFM_ERR_INVALID, "records [%zu, %zu) beyond the %zu of the last scan", first, first + count,
+                    c->n_records);
+    static_assert(sizeof(mafm_record) == sizeof(DevRecord), "record layout");
+    if (count == 0) return MAFM_OK;
+    HIPCHK(c, hipMemcpyAsync(out, c->d_rec.p + first, count * sizeof(DevRecord), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MAFM_OK;
+}
+
+mafm_status mafm_eval_tip_surface(mafm_ctx* c, int system, size_t n, const double* pos, double* fe) {
+    if (!c || (n && (!pos || !fe))) return MAFM_ERR_INVALID;
+    if (!c->have_systems) return fail(c, MAFM_ERR_STATE, "eval before mafm_set_systems");
+    if (system < 0 || system >= (int)c->mols.size()) return fail(c, MAFM_ERR_INVALID, "system %d", system);
+    if (n == 0) return MAFM_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, c->d_tmp.reserve(7 * n));
+    double* dpos = c->d_tmp.p;
+    double* dout = c->d_tmp.p + 3 * n;
+    HIPCHK(c, hipMemcpyAsync(dpos, pos, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_counters.p, 0, 8 * sizeof(int64_t), c->stream));
+    ScanArgs a = base_args(c);
+    HIPCHK(c, launch_eval_plain(a, system, (long)n, dpos, dout, c->stream));
+    int64_t counters[8];
+    HIPCHK(c, hipMemcpyAsync(fe, dout, 4 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(counters, c->d_counters.p, sizeof counters, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (counters[3] != 0) return fail(c, MAFM_ERR_OUT_OF_GRID, "tip position past the force grid");
+    return MAFM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// Device-side data layout shared by the scan, grid and eval kernels.
+//
+// HBM layout (all per context, uploaded once by mafm_set_systems):
+//   DevMol[n_mols]        one record per molecule, read with wave-uniform
+//                         indices so the compiler keeps it in SGPRs
+//   Site64 sites_ref[]    tip-surface sites in the reference's list order
+//                         (VDW(1,i) then Coulomb(1,i), simulation.cpp:297-312)
+//   Site64 seg64[]        the same sites regrouped per molecule into four
+//                         homogeneous segments: LJ+Coulomb, LJ, Morse+Coulomb,
+//                         Morse, so the hot loops carry no per-site branch
+//   Site32 seg32[]        FP32 copy of seg64, coordinates relative to the
+//                         molecule's origin (keeps FP32 rounding of the
+//                         tip-atom separation at ~1e-6 A)
+//   grids                 double4 / float4 {fx, fy, fz, e} per grid point,
+//                         index ix*ny*nz + iy*nz + iz (simulation.cpp:474)
+//   DevRecord[P*Z]        one per scan position
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mafm {
+
+struct Site64 {
+    double x, y, z;
+    double p0, p1, p2;
+    double qq;
+    int32_t vdw;      // 0 LJ, 1 Morse
+    int32_t coul;     // Coulomb term present
+};
+
+struct Site32 {
+    float4 a;         // x, y, z (relative to DevMol::origin), p0 (es6 | De)
+    float4 b;         // p1 (es12 | a*log2(e)), p2 (- | re), qq, p1 raw (Morse a)
+};
+
+struct DevGrid {
+    int32_t n[3];
+    int32_t periodic;
+    int32_t orthogonal;
+    int32_t pad;
+    double basis[9];
+    double inv[9];
+    double offset[3];
+    const double4* fe64;
+    const float4* fe32;
+};
+
+struct DevMol {
+    int32_t ref_off, n_sites;              // sites_ref range
+    int32_t seg_off;                       // seg64 / seg32 start
+    int32_t n_seg[4];                      // LJ+C, LJ, Morse+C, Morse
+    int32_t force_grid, potential_grid;    // indices into the grid table or -1
+    int32_t tip_pbc;
+    int32_t td_vdw, td_coul;
+    double td_p0, td_p1, td_p2, td_qq, td_k, td_r0;
+    double tip_dummy_d;
+    double cell[9], cell_inv[9], offset[3];
+    double origin[3];
+    int64_t rec_off;                       // first record of this molecule
+};
+
+struct DevRecord {
+    double x, y, z;
+    double fx, fy, fz;
+    double rx, ry, rz;
+    double e;
+    int32_t n_steps;
+    int32_t flags;
+};
+
+struct ScanArgs {
+    int32_t nx, ny, nz;
+    int32_t n_mols;
+    double dx, dy, dz, zhigh;
+    int32_t minimiser, integrator, minterm, maxsteps;
+    double etol, ftol, dt;
+    const DevMol* mols;
+    const Site64* sites_ref;
+    const Site64* seg64;
+    const Site32* seg32;
+    const DevGrid* grids;
+    DevRecord* rec;
+    // work-stealing state (FAST kernels)
+    int32_t* next_col;        // per molecule: next column to hand out
+    const float* mol_cum;     // cumulative work fraction per molecule (n_mols + 1)
+    int64_t* counters;        // [0] total steps [1] unconverged [2] grid warnings [3] out-of-grid
+};
+
+enum { REC_UNCONVERGED = 1, REC_GRID_CLAMP = 2, REC_OUT_OF_GRID = 4 };
+
+// Matrix3::multiply for a row-major 3x3 (matrices.hpp:146-153).
+__device__ __host__ inline void mat_vec(const double* m, double x, double y, double z,
+                                        double& ox, double& oy, double& oz) {
+    ox = m[0] * x + m[1] * y + m[2] * z;
+    oy = m[3] * x + m[4] * y + m[5] * z;
+    oz = m[6] * x + m[7] * y + m[8] * z;
+}
+
+}  // namespace mafm

@@ -1,0 +1,82 @@
+// mechafm: the reference's command line (mechafm.cpp:209-233),
+//   mechafm INPUT [OUTDIR] [--precision plain64|fast64|fast32] [--device N]
+// parse -> initialize (interactions, force grid / external potential on the
+// device) -> one device scan -> per-height output files.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "../../../include/mafm.h"
+#include "../../../include/mafm_sim.h"
+
+static int die(const char* what, const char* msg) {
+    fprintf(stderr, "+- ERROR: %s: %s\n", what, msg);
+    return 1;
+}
+
+int main(int argc, char** argv) {
+    std::string input, outdir;
+    mafm_run_opts ro{};
+    ro.precision = MAFM_PREC_FAST64;
+    int device = 0;
+    for (int i = 1; i < argc; ++i) {
+        if (!strcmp(argv[i], "--precision") && i + 1 < argc) {
+            std::string p = argv[++i];
+            if (p == "plain64") ro.precision = MAFM_PREC_PLAIN64;
+            else if (p == "fast64") ro.precision = MAFM_PREC_FAST64;
+            else if (p == "fast32") ro.precision = MAFM_PREC_FAST32;
+            else return die("precision", "expected plain64, fast64 or fast32");
+        } else if (!strcmp(argv[i], "--device") && i + 1 < argc) {
+            device = atoi(argv[++i]);
+        } else if (input.empty()) {
+            input = argv[i];
+        } else if (outdir.empty()) {
+            outdir = argv[i];
+        } else {
+            return die("command line", "Too many command line arguments!");
+        }
+    }
+    if (input.empty()) return die("command line", "Specify an input file to be read!");
+    printf("+- MechAFM rigid probe-particle scan on gfx950 (libmafm)\n");
+    mafm_ctx* ctx = nullptr;
+    mafm_status st = mafm_create(device, &ctx);
+    if (st != MAFM_OK) return die("device", "no usable HIP device");
+    mafm_sim* sim = nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    st = mafm_sim_open(input.c_str(), outdir.c_str(), ctx, &sim);
+    if (st == MAFM_OK) st = mafm_sim_initialize(sim);
+    if (st != MAFM_OK) {
+        int rc = die("set-up", mafm_sim_last_error(sim));
+        mafm_sim_close(sim);
+        mafm_destroy(ctx);
+        return rc;
+    }
+    mafm_sim_info info;
+    mafm_sim_info_get(sim, &info);
+    printf("+- 3D data grid is: %d x %d x %d (%lld in total)\n", info.nx, info.ny, info.nz,
+           (long long)info.nx * info.ny * info.nz);
+    mafm_scan_stats ss{};
+    st = mafm_sim_run(sim, &ro, &ss);
+    if (st == MAFM_OK) st = mafm_sim_write_output(sim, 8);
+    auto t1 = std::chrono::steady_clock::now();
+    if (st != MAFM_OK) {
+        int rc = die("scan", mafm_sim_last_error(sim));
+        mafm_sim_close(sim);
+        mafm_destroy(ctx);
+        return rc;
+    }
+    long long n = (long long)info.nx * info.ny * info.nz;
+    printf("+- Simulation run finished\n+- Statistics:\n");
+    printf("+-     Computed %lld tip positions\n", n);
+    printf("+-     Needed %lld minimization steps in total\n", (long long)ss.total_steps);
+    printf("+-     Which means approximately %.2f minimization steps per tip position\n",
+           (double)ss.total_steps / (double)n);
+    printf("+-     The scan kernel took %.3f seconds (%.0f positions/s)\n", ss.kernel_ms / 1e3,
+           n / (ss.kernel_ms / 1e3));
+    printf("+-     The entire simulation took %.2f seconds\n", std::chrono::duration<double>(t1 - t0).count());
+    mafm_sim_close(sim);
+    mafm_destroy(ctx);
+    return 0;
+}

@@ -1,0 +1,960 @@
+// C++ host side of the scan: parsing, set-up, interaction lowering, run,
+// output.  See simulation.hpp.
+#include "simulation.hpp"
+
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cerrno>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <thread>
+
+namespace mafm {
+
+namespace {
+
+const double SIXTHRT2 = 1.12246204830937298142;   // globals.hpp:14
+const double PI = 3.14159265358979323846;         // globals.hpp:13
+const double FORCE_GRID_MARGIN = 1.5;             // globals.hpp:16
+const double TIP_GAUSSIAN_WIDTH = 0.5;            // globals.hpp:18
+const double HARTREE_TO_EV = 27.211386;           // globals.hpp:21-23
+const double HARTREE_TO_KJ = 2625.5002;
+const double HARTREE_TO_KCAL = 627.50961;
+const double BOHR_TO_ANGST = 0.52917721092;       // cube_io.hpp:17
+
+[[noreturn]] void fail(mafm_status s, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    throw Error(s, buf);
+}
+
+void check(mafm_ctx* ctx, mafm_status s, const char* what) {
+    if (s != MAFM_OK) fail(s, "%s: %s", what, mafm_last_error(ctx));
+}
+
+// checkForComments (parse.cpp:17-27); whitespace-only lines are skipped too.
+bool skip_line(const std::string& line) {
+    if (line.empty() || line[0] == '#' || line[0] == '%' || line[0] == '\n' || line[0] == '\r') return true;
+    for (char ch : line)
+        if (!std::isspace((unsigned char)ch)) return false;
+    return true;
+}
+
+std::vector<std::string> split(const std::string& line) {
+    std::istringstream is(line);
+    std::vector<std::string> t;
+    std::string w;
+    while (is >> w) t.push_back(w);
+    return t;
+}
+
+std::string lower(std::string s) {
+    for (char& ch : s) ch = (char)std::tolower((unsigned char)ch);
+    return s;
+}
+
+double to_d(const std::string& s) { return std::strtod(s.c_str(), nullptr); }   // atof
+
+// isint (utility.cpp:26-37)
+bool is_int(const std::string& s) {
+    for (char ch : s)
+        if (!(std::isdigit((unsigned char)ch) || ch == '-' || ch == '+')) return false;
+    return true;
+}
+
+bool on_off(const std::string& key, const std::string& v) {
+    if (v == "on") return true;
+    if (v == "off") return false;
+    fail(MAFM_ERR_INVALID, "Option %s must be either on or off!", key.c_str());
+}
+
+std::vector<std::string> read_lines(const std::string& path, const char* what) {
+    std::ifstream f(path);
+    if (!f) fail(MAFM_ERR_INVALID, "%s %s does not exist!", what, path.c_str());
+    std::vector<std::string> lines;
+    std::string l;
+    while (std::getline(f, l)) lines.push_back(l);
+    return lines;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- Mat3d
+double Mat3d::determinant() const {   // matrices.hpp:135-143
+    double det = m[0] * (m[4] * m[8] - m[5] * m[7]);
+    det += m[1] * (m[5] * m[6] - m[3] * m[8]);
+    det += m[2] * (m[3] * m[7] - m[4] * m[6]);
+    return det;
+}
+
+Mat3d Mat3d::inverse() const {        // matrices.hpp:89-114
+    Mat3d o;
+    double inv_det = 1 / determinant();
+    double a = m[0], b = m[1], c = m[2], d = m[3], e = m[4], f = m[5], g = m[6], h = m[7], i = m[8];
+    o.m[0] = inv_det * (e * i - f * h);
+    o.m[1] = -inv_det * (b * i - c * h);
+    o.m[2] = inv_det * (b * f - c * e);
+    o.m[3] = -inv_det * (d * i - f * g);
+    o.m[4] = inv_det * (a * i - c * g);
+    o.m[5] = -inv_det * (a * f - c * d);
+    o.m[6] = inv_det * (d * h - e * g);
+    o.m[7] = -inv_det * (a * h - b * g);
+    o.m[8] = inv_det * (a * e - b * d);
+    return o;
+}
+
+bool Mat3d::isDiagonal() const {      // matrices.hpp:72-86
+    double diag = 0, nondiag = 0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) (i == j ? diag : nondiag) += std::fabs(m[3 * i + j]);
+    return nondiag < 1.0e-10 * diag;
+}
+
+// ---------------------------------------------------------------- lowering
+// Tip-surface pair terms become sites in list order; a Coulomb term folds
+// into the site of the same atom that precedes it (the reference emits
+// VDW(1,i) then Coulomb(1,i), simulation.cpp:309-312).
+void LJInteraction::lower(Lowering& out) const {
+    if (i1_ == 1) {
+        const Vec3d& p = (*out.positions)[i2_];
+        mafm_site s{};
+        s.x = p.x + shift_.x; s.y = p.y + shift_.y; s.z = p.z + shift_.z;
+        s.vdw = MAFM_VDW_LJ;
+        s.p0 = es6_; s.p1 = es12_;
+        out.sites.push_back(s);
+    } else if (i1_ == 0 && i2_ == 1) {
+        out.tip_dummy.vdw = MAFM_VDW_LJ;
+        out.tip_dummy.p0 = es6_; out.tip_dummy.p1 = es12_; out.tip_dummy.p2 = 0;
+        out.tip_dummy_vdw_set = true;
+    } else {
+        fail(MAFM_ERR_INVALID, "surface-surface interactions need flexible mode, which is not built");
+    }
+}
+
+void MorseInteraction::lower(Lowering& out) const {
+    if (i1_ == 1) {
+        const Vec3d& p = (*out.positions)[i2_];
+        mafm_site s{};
+        s.x = p.x + shift_.x; s.y = p.y + shift_.y; s.z = p.z + shift_.z;
+        s.vdw = MAFM_VDW_MORSE;
+        s.p0 = de_; s.p1 = a_; s.p2 = re_;
+        out.sites.push_back(s);
+    } else if (i1_ == 0 && i2_ == 1) {
+        out.tip_dummy.vdw = MAFM_VDW_MORSE;
+        out.tip_dummy.p0 = de_; out.tip_dummy.p1 = a_; out.tip_dummy.p2 = re_;
+        out.tip_dummy_vdw_set = true;
+    } else {
+        fail(MAFM_ERR_INVALID, "surface-surface interactions need flexible mode, which is not built");
+    }
+}
+
+void CoulombInteraction::lower(Lowering& out) const {
+    if (i1_ == 1) {
+        if (out.sites.empty()) fail(MAFM_ERR_INVALID, "Coulomb term without a preceding VDW site");
+        mafm_site& s = out.sites.back();
+        const Vec3d& p = (*out.positions)[i2_];
+        if (s.x != p.x || s.y != p.y || s.z != p.z || s.has_coulomb)
+            fail(MAFM_ERR_INVALID, "Coulomb term does not follow the VDW term of its atom");
+        s.has_coulomb = 1;
+        s.qq = qq_;
+    } else if (i1_ == 0 && i2_ == 1) {
+        out.tip_dummy.has_coulomb = 1;
+        out.tip_dummy.qq = qq_;
+    } else {
+        fail(MAFM_ERR_INVALID, "surface-surface interactions need flexible mode, which is not built");
+    }
+}
+
+void TipHarmonicInteraction::lower(Lowering& out) const {
+    if (!(i1_ == 0 && i2_ == 1)) fail(MAFM_ERR_INVALID, "tip harmonic must join dummy (0) and tip (1)");
+    out.tip_dummy.k = k_;
+    out.tip_dummy.r0 = r0_;
+}
+
+// ---------------------------------------------------------------- System
+void System::initialize(int n_atoms) {           // system.cpp:10-25
+    tip_pbc_ = false;
+    n_atoms_ = n_atoms + 2;
+    positions_.assign(n_atoms_, Vec3d());
+    charges_.assign(n_atoms_, 0);
+    fixed_.assign(n_atoms_, 0);
+    types_.assign(n_atoms_, "");
+    fixed_[0] = 1;
+    fixed_[1] = 0;
+}
+
+void System::setMoleculeZ() {                    // system.cpp:103-116
+    double min_z = 10e10;
+    for (int i = 2; i < n_atoms_; ++i)
+        if (positions_[i].z < min_z && types_[i] != "H") min_z = positions_[i].z;
+    for (int i = 2; i < n_atoms_; ++i) positions_[i].z -= min_z;
+    offset_.z = -min_z;
+}
+
+void System::centerMolecule(Vec2d pos) {         // system.cpp:119-136
+    double avgx = 0.0, avgy = 0.0;
+    for (int i = 2; i < n_atoms_; ++i) {
+        avgx += positions_[i].x;
+        avgy += positions_[i].y;
+    }
+    avgx /= n_atoms_ - 2;
+    avgy /= n_atoms_ - 2;
+    double dx = pos.x - avgx, dy = pos.y - avgy;
+    for (int i = 2; i < n_atoms_; ++i) {
+        positions_[i].x += dx;
+        positions_[i].y += dy;
+    }
+    offset_.x = dx;
+    offset_.y = dy;
+}
+
+void System::setUnitCell(const std::vector<Vec3d>& v) {   // system.cpp:170-177
+    for (int i = 0; i < 3; ++i) {
+        cell_.at(0, i) = v[i].x;
+        cell_.at(1, i) = v[i].y;
+        cell_.at(2, i) = v[i].z;
+    }
+    tip_pbc_ = true;
+}
+
+void System::rotateCoordAxes(const std::string& seq) {    // system.cpp:68-90
+    for (auto& p : positions_) {
+        Vec3d q = p;
+        if (seq == "ZXY") p = Vec3d(q.z, q.x, q.y);
+        else if (seq == "YZX") p = Vec3d(q.y, q.z, q.x);
+        else fail(MAFM_ERR_INVALID, "Cannot rotate coordinate axes to %s!", seq.c_str());
+    }
+}
+
+// ---------------------------------------------------------------- cube
+Cube readCube(const std::string& path, bool with_values) {   // cube_io.cpp:10-139
+    FILE* f = fopen(path.c_str(), "rb");
+    if (!f) fail(MAFM_ERR_INVALID, "Could not open cube file %s", path.c_str());
+    std::string buf;
+    {
+        fseek(f, 0, SEEK_END);
+        long len = ftell(f);
+        fseek(f, 0, SEEK_SET);
+        buf.resize((size_t)std::max(len, 0L));
+        size_t got = len > 0 ? fread(&buf[0], 1, (size_t)len, f) : 0;
+        fclose(f);
+        buf.resize(got);
+    }
+    const char* p = buf.c_str();
+    const char* end = p + buf.size();
+    for (int l = 0; l < 2; ++l) {                 // two comment lines
+        while (p < end && *p != '\n') ++p;
+        if (p < end) ++p;
+    }
+    auto num = [&](const char* what) -> double {
+        char* q = nullptr;
+        errno = 0;
+        double v = std::strtod(p, &q);
+        if (q == p) fail(MAFM_ERR_INVALID, "Could not read %s from the cube file. Check the format of the file.", what);
+        p = q;
+        return v;
+    };
+    Cube c;
+    c.n_atoms = (int)num("atom count");
+    c.origin.x = num("origin"); c.origin.y = num("origin"); c.origin.z = num("origin");
+    for (int a = 0; a < 3; ++a) {
+        c.n[a] = (int)num("voxel count");
+        c.voxel[a].x = num("voxel vector");
+        c.voxel[a].y = num("voxel vector");
+        c.voxel[a].z = num("voxel vector");
+    }
+    if (c.n[0] < 0) {                            // Angstrom
+        for (int a = 0; a < 3; ++a) c.n[a] = -c.n[a];
+    } else {                                     // Bohr
+        for (int a = 0; a < 3; ++a) c.voxel[a] = BOHR_TO_ANGST * c.voxel[a];
+        c.origin = BOHR_TO_ANGST * c.origin;
+    }
+    for (int a = 0; a < std::abs(c.n_atoms); ++a)
+        for (int q = 0; q < 5; ++q) num("atom record");
+    if (with_values) {
+        size_t total = (size_t)c.n[0] * c.n[1] * c.n[2];
+        c.values.resize(total);
+        for (size_t i = 0; i < total; ++i) c.values[i] = num("volumetric data");
+    }
+    return c;
+}
+
+// ---------------------------------------------------------------- Simulation
+Simulation::~Simulation() {
+    for (int h : owned_grids_) (void)mafm_grid_free(ctx_, h);
+}
+
+void Simulation::parseCommandLine(const std::string& input, const std::string& output) {   // parse.cpp:30-80
+    InputOptions& o = options_;
+    o.inputfile = input;
+    size_t sl = input.rfind('/');
+    o.inputfolder = sl != std::string::npos ? input.substr(0, sl + 1) : "";
+    o.outputfolder = output;
+    if (!o.outputfolder.empty() && o.outputfolder.back() != '/') o.outputfolder += '/';
+    if (!o.outputfolder.empty()) {
+        std::string acc;
+        for (size_t i = 0; i < o.outputfolder.size(); ++i) {
+            acc += o.outputfolder[i];
+            if (o.outputfolder[i] == '/') mkdir(acc.c_str(), 0755);
+        }
+    }
+}
+
+void Simulation::readInputFile() {                // parse.cpp:83-405
+    InputOptions& o = options_;
+    for (const std::string& line : read_lines(o.inputfile, "The file")) {
+        if (skip_line(line)) continue;
+        std::vector<std::string> t = split(line);
+        const std::string key = lower(t[0]);
+        const std::string val = t.size() > 1 ? t[1] : "";
+        auto need = [&](size_t n) {
+            if (t.size() < n) fail(MAFM_ERR_INVALID, "Option %s needs %zu values", key.c_str(), n - 1);
+        };
+        if (key == "xyzfile") o.xyzfile = o.inputfolder + val;
+        else if (key == "paramfile") o.paramfile = o.inputfolder + val;
+        else if (key == "e_potential_file") o.e_potential_file = o.inputfolder + val;
+        else if (key == "tipatom") o.tipatom = val;
+        else if (key == "dummyatom") o.dummyatom = val;
+        else if (key == "area") { need(3); o.area = {to_d(t[1]), to_d(t[2])}; }
+        else if (key == "center") { need(3); o.center = {to_d(t[1]), to_d(t[2])}; }
+        else if (key == "zhigh") o.zhigh = to_d(val);
+        else if (key == "zlow") o.zlow = to_d(val);
+        else if (key == "dx") o.dx = to_d(val);
+        else if (key == "dy") o.dy = to_d(val);
+        else if (key == "dz") o.dz = to_d(val);
+        else if (key == "cell_a") { need(4); o.cell_a = Vec3d(to_d(t[1]), to_d(t[2]), to_d(t[3])); }
+        else if (key == "cell_b") { need(4); o.cell_b = Vec3d(to_d(t[1]), to_d(t[2]), to_d(t[3])); }
+        else if (key == "cell_c") { need(4); o.cell_c = Vec3d(to_d(t[1]), to_d(t[2]), to_d(t[3])); }
+        else if (key == "etol") o.etol = to_d(val);
+        else if (key == "ftol") o.ftol = to_d(val);
+        else if (key == "dt") o.dt = to_d(val);
+        else if (key == "maxsteps") o.maxsteps = std::atoi(val.c_str());
+        else if (key == "bufsize") o.bufsize = std::atoi(val.c_str());
+        else if (key == "gzip") o.gzip = on_off(key, val);
+        else if (key == "statistics") o.statistics = on_off(key, val);
+        else if (key == "flexible") o.flexible = on_off(key, val);
+        else if (key == "rigidgrid") o.rigidgrid = on_off(key, val);
+        else if (key == "coulomb") o.coulomb = on_off(key, val);
+        else if (key == "tip_dummy_coulomb") o.tip_dummy_coulomb = on_off(key, val);
+        else if (key == "use_external_potential") o.use_external_potential = on_off(key, val);
+        else if (key == "vdw_pbc") o.vdw_pbc = on_off(key, val);
+        else if (key == "surface_normal") {
+            std::string v = lower(val);
+            if (v == "z") o.normal = NORMAL_Z;
+            else if (v == "y") o.normal = NORMAL_Y;
+            else if (v == "x") o.normal = NORMAL_X;
+            else fail(MAFM_ERR_INVALID, "Option %s must be either X, Y or Z!", key.c_str());
+        } else if (key == "minterm") {
+            if (val == "e") o.minterm = MIN_E;
+            else if (val == "f") o.minterm = MIN_F;
+            else if (val == "ef") o.minterm = MIN_EF;
+            else fail(MAFM_ERR_INVALID, "Option %s must be either e, f or ef!", key.c_str());
+        } else if (key == "units") {
+            InteractionParameters& ip = interaction_parameters_;
+            if (val == "kcal/mol" || val == "kcal") { o.units = U_KCAL; ip.qbase = 332.06371; }
+            else if (val == "kJ/mol" || val == "kJ") { o.units = U_KJ; ip.qbase = 1389.354563; }
+            else if (val == "eV") { o.units = U_EV; ip.qbase = 14.39964901; }
+            else fail(MAFM_ERR_INVALID, "Option %s must be either kcal/mol (default), kJ/mol or eV!", key.c_str());
+            o.units_given = true;
+            ip.qbase_set = true;
+        } else if (key == "minimiser") {
+            if (val == "SD") o.minimiser_type = STEEPEST_DESCENT;
+            else if (val == "FIRE") o.minimiser_type = FIRE;
+            else fail(MAFM_ERR_INVALID, "Unrecognised minimiser type!");
+        } else if (key == "integrator") {
+            if (val == "euler") o.integrator_type = EULER;
+            else if (val == "midpoint") o.integrator_type = MIDPOINT;
+            else if (val == "rk4") o.integrator_type = RK4;
+            else fail(MAFM_ERR_INVALID, "Unrecognised integrator type!");
+        } else {
+            fail(MAFM_ERR_INVALID, "Unknown option %s!", key.c_str());
+        }
+    }
+    if (o.xyzfile.empty()) fail(MAFM_ERR_INVALID, "Specify at least an xyzfile!");
+    if (o.paramfile.empty()) fail(MAFM_ERR_INVALID, "Specify at least a parameter file!");
+    if (o.tipatom.empty()) fail(MAFM_ERR_INVALID, "Specify at least a tip atom!");
+    if (o.dummyatom.empty()) fail(MAFM_ERR_INVALID, "Specify at least a dummy atom!");
+    if (o.minterm == NOT_SET)
+        fail(MAFM_ERR_INVALID, "Specify at least a minimization termination criterion (e, f, or ef)!");
+    if (o.center.x == -1 && o.center.y == -1) o.center = {o.area.x / 2, o.area.y / 2};
+    if (o.rigidgrid && o.flexible) fail(MAFM_ERR_INVALID, "Cannot use a flexible molecule with a static force grid!");
+    if (o.coulomb && o.use_external_potential)
+        fail(MAFM_ERR_INVALID, "Cannot use Coulomb interaction and external electrostatic potential at the same time!");
+    if (o.use_external_potential && o.flexible)
+        fail(MAFM_ERR_INVALID, "External potential can be used only for non-flexible systems!");
+    if (o.use_external_potential && o.e_potential_file.empty())
+        fail(MAFM_ERR_INVALID, "If you want to use external electrostatic potential, you must specify a file that contains it!");
+    if (o.vdw_pbc && o.coulomb)
+        fail(MAFM_ERR_INVALID, "Implementation of Coulomb interaction does not support any periodic boundary conditions!");
+    if (o.vdw_pbc && !o.use_external_potential &&
+        (o.cell_a == Vec3d() || o.cell_b == Vec3d() || o.cell_c == Vec3d()))
+        fail(MAFM_ERR_INVALID, "The unit cell vectors must be given if periodic vdW is used.");
+    if (o.flexible) fail(MAFM_ERR_INVALID, "flexible on is not supported: this build scans rigid molecules");
+    if (!(o.dx > 0 && o.dy > 0 && o.dz > 0)) fail(MAFM_ERR_INVALID, "dx, dy and dz must be positive");
+    // openUniverse (mechafm.cpp:47-49)
+    n_points_[0] = (int)std::floor(o.area.x / o.dx) + 1;
+    n_points_[1] = (int)std::floor(o.area.y / o.dy) + 1;
+    n_points_[2] = (int)std::floor((o.zhigh - o.zlow) / o.dz) + 1;
+    if (n_points_[0] <= 0 || n_points_[1] <= 0 || n_points_[2] <= 0)
+        fail(MAFM_ERR_INVALID, "empty scan grid %d x %d x %d", n_points_[0], n_points_[1], n_points_[2]);
+}
+
+void Simulation::readXYZFile() {                  // parse.cpp:477-585
+    InputOptions& o = options_;
+    std::vector<std::string> lines = read_lines(o.xyzfile, "No such file:");
+    int n_atoms = 0;
+    bool first = true, realxyz = false;
+    for (const std::string& line : lines) {
+        if (skip_line(line)) continue;
+        if (first) {
+            std::vector<std::string> t = split(line);
+            if (is_int(t[0])) {
+                n_atoms = std::atoi(t[0].c_str());
+                realxyz = true;
+                break;
+            }
+            first = false;
+        }
+        ++n_atoms;
+    }
+    if (n_atoms <= 0) fail(MAFM_ERR_INVALID, "no atoms in %s", o.xyzfile.c_str());
+    system.initialize(n_atoms);
+    system.types_[0] = o.dummyatom;
+    system.types_[1] = o.tipatom;
+    int atom_i = 2, ncols = 0;
+    for (size_t n = 0; n < lines.size(); ++n) {
+        if (realxyz && n < 2) continue;
+        if (skip_line(lines[n])) continue;
+        std::vector<std::string> t = split(lines[n]);
+        if (ncols == 0) ncols = (int)t.size();
+        if (ncols < 4 || ncols > 6 || (int)t.size() < ncols)
+            fail(MAFM_ERR_INVALID, "Invalid number of columns in the xyz file.");
+        if (atom_i >= system.n_atoms_) fail(MAFM_ERR_INVALID, "more atoms in %s than declared", o.xyzfile.c_str());
+        system.types_[atom_i] = t[0];
+        system.positions_[atom_i] = Vec3d(to_d(t[1]), to_d(t[2]), to_d(t[3]));
+        system.charges_[atom_i] = ncols >= 5 ? to_d(t[4]) : 0.0;
+        system.fixed_[atom_i] = 1;              // rigid: every surface atom fixed (parse.cpp:563)
+        ++atom_i;
+    }
+    if (atom_i != system.n_atoms_) fail(MAFM_ERR_INVALID, "fewer atoms in %s than declared", o.xyzfile.c_str());
+    double charge_check = 0;
+    for (int i = 0; i < system.n_atoms_; ++i) charge_check += std::pow(system.charges_[i], 2);
+    o.xyz_charges = !(std::fabs(charge_check) < 1e-10);
+}
+
+void Simulation::readParameterFile() {            // parse.cpp:588-697
+    InputOptions& o = options_;
+    InteractionParameters& p = interaction_parameters_;
+    for (const std::string& line : read_lines(o.paramfile, "No parameter file")) {
+        if (skip_line(line)) continue;
+        std::vector<std::string> t = split(line);
+        if (t[0] == "atom") {
+            if (t.size() < 6) fail(MAFM_ERR_INVALID, "atom line needs name eps sig mass charge");
+            AtomParameters& a = p.atom_parameters[t[1]];
+            a.eps = to_d(t[2]);
+            a.sig = to_d(t[3]);
+            a.mass = to_d(t[4]);
+            a.q = to_d(t[5]);
+            if (o.dummyatom == t[1]) system.charges_[0] = a.q;
+            if (o.tipatom == t[1]) system.charges_[1] = a.q;
+        } else if (t[0] == "harm") {
+            if (p.harm_set) fail(MAFM_ERR_INVALID, "Parameters for harmonic spring can only be specified once!");
+            if (t.size() < 3) fail(MAFM_ERR_INVALID, "harm needs k and r0");
+            p.tip_dummy_k = to_d(t[1]);
+            p.tip_dummy_r0 = to_d(t[2]);
+            p.harm_set = true;
+        } else if (t[0] == "pair_ovwrt") {
+            if (t.size() < 4) fail(MAFM_ERR_INVALID, "pair_ovwrt needs atom1 atom2 style parameters");
+            OverwriteParameters ow;
+            ow.a = t[1];
+            ow.b = t[2];
+            if (t[3] == "lj") {
+                if (t.size() != 6) fail(MAFM_ERR_INVALID, "Only two parameters (eps, sig) allowed for LJ");
+                ow.morse = false;
+                ow.eps = to_d(t[4]);
+                ow.sig = to_d(t[5]);
+            } else if (t[3] == "morse") {
+                if (t.size() != 7) fail(MAFM_ERR_INVALID, "Only three parameters (De, a, re) allowed for Morse");
+                ow.morse = true;
+                ow.de = to_d(t[4]);
+                ow.alpha = to_d(t[5]);
+                ow.re = to_d(t[6]);
+            } else {
+                fail(MAFM_ERR_INVALID, "Unknown pair style overwrite '%s'", t[3].c_str());
+            }
+            p.overwrite_parameters.push_back(ow);
+        }
+    }
+    if (!p.harm_set) fail(MAFM_ERR_INVALID, "No harmonic spring parameters found in parameter file!");
+}
+
+void Simulation::calculateTipDummyDistance() {    // simulation.cpp:218-225
+    auto& ap = interaction_parameters_.atom_parameters;
+    auto d = ap.find(system.types_[0]);
+    auto t = ap.find(system.types_[1]);
+    if (d == ap.end() || t == ap.end())
+        fail(MAFM_ERR_INVALID, "Parameters for the tip or dummy atom type not found in parameter file!");
+    system.setTipDummyDistance(mixsig(d->second.sig, t->second.sig) * SIXTHRT2);
+}
+
+bool Simulation::findOverwriteParameters(int i1, int i2, OverwriteParameters& op) const {   // :227-236
+    const std::string& a = system.types_[i1];
+    const std::string& b = system.types_[i2];
+    for (const auto& ow : interaction_parameters_.overwrite_parameters) {
+        if ((ow.a == a && ow.b == b) || (ow.a == b && ow.b == a)) {
+            op = ow;
+            return true;
+        }
+    }
+    return false;
+}
+
+void Simulation::addVDWInteraction(int i1, int i2, Vec3d shift) {   // simulation.cpp:238-267
+    OverwriteParameters op;
+    if (findOverwriteParameters(i1, i2, op)) {
+        if (op.morse) {
+            interactions_.emplace_back(new MorseInteraction(i1, i2, op.de, op.alpha, op.re, shift));
+        } else {
+            double es6 = 4 * op.eps * std::pow(op.sig, 6);
+            double es12 = 4 * op.eps * std::pow(op.sig, 12);
+            interactions_.emplace_back(new LJInteraction(i1, i2, es6, es12, shift));
+        }
+        return;
+    }
+    auto& ap = interaction_parameters_.atom_parameters;
+    auto a1 = ap.find(system.types_[i1]);
+    auto a2 = ap.find(system.types_[i2]);
+    if (a1 == ap.end())
+        fail(MAFM_ERR_INVALID, "Parameters for atom type %s not found in parameter file!", system.types_[i1].c_str());
+    if (a2 == ap.end())
+        fail(MAFM_ERR_INVALID, "Parameters for atom type %s not found in parameter file!", system.types_[i2].c_str());
+    double m_eps = mixeps(a1->second.eps, a2->second.eps);
+    double m_sig = mixsig(a1->second.sig, a2->second.sig);
+    double es6 = 4 * m_eps * std::pow(m_sig, 6);
+    double es12 = 4 * m_eps * std::pow(m_sig, 12);
+    interactions_.emplace_back(new LJInteraction(i1, i2, es6, es12, shift));
+}
+
+void Simulation::addCoulombInteraction(int i1, int i2) {   // simulation.cpp:269-295
+    double q1, q2;
+    if (options_.xyz_charges) {
+        q1 = system.charges_[i1];
+        q2 = system.charges_[i2];
+    } else {
+        auto& ap = interaction_parameters_.atom_parameters;
+        auto a1 = ap.find(system.types_[i1]);
+        auto a2 = ap.find(system.types_[i2]);
+        if (a1 == ap.end() || a2 == ap.end())
+            fail(MAFM_ERR_INVALID, "Parameters for atom type not found in parameter file!");
+        q1 = a1->second.q;
+        q2 = a2->second.q;
+    }
+    if (q1 != 0 && q2 != 0) {
+        if (!interaction_parameters_.qbase_set)
+            fail(MAFM_ERR_INVALID, "Coulomb terms need a `units` line (it sets the Coulomb prefactor)");
+        double qq = interaction_parameters_.qbase * q1 * q2;
+        interactions_.emplace_back(new CoulombInteraction(i1, i2, qq));
+    }
+}
+
+ForceGrid Simulation::buildElectrostaticPotential() {      // simulation.cpp:323-417
+    Cube cube = readCube(options_.e_potential_file, true);
+    double scale;
+    if (options_.units == U_EV) scale = -HARTREE_TO_EV;
+    else if (options_.units == U_KJ) scale = -HARTREE_TO_KJ;
+    else scale = -HARTREE_TO_KCAL;
+    for (double& v : cube.values) v *= scale;
+    if (options_.normal != NORMAL_Z)
+        fail(MAFM_ERR_INVALID, "an external potential needs surface_normal z (the reference's grid rotation for "
+                               "x asks for an axis order it does not implement)");
+    mafm_potential_desc pd{};
+    ForceGrid g;
+    for (int a = 0; a < 3; ++a) {
+        pd.n[a] = cube.n[a];
+        g.n[a] = cube.n[a];
+        g.basis.at(0, a) = cube.voxel[a].x;
+        g.basis.at(1, a) = cube.voxel[a].y;
+        g.basis.at(2, a) = cube.voxel[a].z;
+    }
+    memcpy(pd.basis, g.basis.m, sizeof pd.basis);
+    g.offset = cube.origin + system.getOffset();
+    pd.origin[0] = g.offset.x; pd.origin[1] = g.offset.y; pd.origin[2] = g.offset.z;
+    pd.potential = cube.values.data();
+    pd.tip_charge = system.charges_[1];
+    pd.gaussian_width = TIP_GAUSSIAN_WIDTH;
+    g.periodic = true;
+    check(ctx_, mafm_potential_grid(ctx_, &pd, &g.handle), "external potential");
+    owned_grids_.push_back(g.handle);
+    return g;
+}
+
+void Simulation::buildTipSurfaceInteractions() {  // simulation.cpp:297-418
+    if (options_.vdw_pbc) {
+        const Mat3d& cell = system.getUnitCell();
+        for (int a = -1; a <= 1; ++a)
+            for (int b = -1; b <= 1; ++b) {
+                Vec3d shift = (double)a * cell.column(0) + (double)b * cell.column(1);
+                for (int i = 2; i < system.n_atoms_; ++i) addVDWInteraction(1, i, shift);
+            }
+    } else {
+        for (int i = 2; i < system.n_atoms_; ++i) {
+            addVDWInteraction(1, i);
+            if (options_.coulomb) addCoulombInteraction(1, i);
+        }
+    }
+    if (options_.use_external_potential) {
+        ForceGrid g = buildElectrostaticPotential();
+        interactions_.emplace_back(new ElectrostaticPotentialInteraction(g));
+    }
+}
+
+void Simulation::buildTipGridInteractions() {     // simulation.cpp:420-501
+    if (!interactions_.empty()) fail(MAFM_ERR_STATE, "Interaction list is not empty when building force grid!");
+    buildTipSurfaceInteractions();
+    const InputOptions& o = options_;
+    const double sp[3] = {o.dx / 2, o.dy / 2, o.dz / 2};
+    int border[3], n[3];
+    for (int a = 0; a < 3; ++a) border[a] = (int)std::ceil(FORCE_GRID_MARGIN / sp[a]);
+    n[0] = (int)std::floor(o.area.x / sp[0]) + 2 * border[0] + 1;
+    n[1] = (int)std::floor(o.area.y / sp[1]) + 2 * border[1] + 1;
+    n[2] = (int)std::floor((o.zhigh - o.zlow) / sp[2]) + 2 * border[2] + 1;
+    Vec3d offset(-border[0] * sp[0], -border[1] * sp[1], o.zlow - system.getTipDummyDistance() - border[2] * sp[2]);
+    // lower the tip-surface list (sites + optional potential grid) and sample it on the device
+    Lowering low;
+    low.positions = &system.positions_;
+    for (const auto& it : interactions_) it->lower(low);
+    mafm_system_desc sd{};
+    sd.n_sites = (int32_t)low.sites.size();
+    sd.sites = low.sites.data();
+    sd.force_grid = -1;
+    sd.potential_grid = low.potential_grid;
+    sd.tip_pbc = system.tipPbc() ? 1 : 0;
+    memcpy(sd.cell, system.getUnitCell().m, sizeof sd.cell);
+    sd.offset[0] = system.getOffset().x; sd.offset[1] = system.getOffset().y; sd.offset[2] = system.getOffset().z;
+    mafm_grid_desc gd{};
+    ForceGrid fg;
+    for (int a = 0; a < 3; ++a) {
+        gd.n[a] = n[a];
+        fg.n[a] = n[a];
+        force_grid_n_[a] = n[a];
+        fg.basis.at(a, a) = sp[a];
+    }
+    memcpy(gd.basis, fg.basis.m, sizeof gd.basis);
+    gd.offset[0] = offset.x; gd.offset[1] = offset.y; gd.offset[2] = offset.z;
+    gd.periodic = 0;
+    fg.offset = offset;
+    fg.periodic = false;
+    check(ctx_, mafm_grid_build(ctx_, &sd, &gd, &fg.handle), "force grid build");
+    owned_grids_.push_back(fg.handle);
+    interactions_.clear();
+    interactions_.emplace_back(new GridInteraction(fg));
+}
+
+void Simulation::buildTipDummyInteractions() {    // simulation.cpp:503-515
+    addVDWInteraction(0, 1);
+    if (options_.tip_dummy_coulomb) addCoulombInteraction(0, 1);
+    interactions_.emplace_back(new TipHarmonicInteraction(0, 1, interaction_parameters_.tip_dummy_k,
+                                                          interaction_parameters_.tip_dummy_r0));
+}
+
+void Simulation::buildInteractions() {            // simulation.cpp:201-216
+    interactions_.clear();
+    if (options_.rigidgrid) buildTipGridInteractions();
+    else buildTipSurfaceInteractions();
+    buildTipDummyInteractions();
+}
+
+void Simulation::initialize() {                   // simulation.cpp:31-65
+    if (options_.normal == NORMAL_X) system.rotateCoordAxes("YZX");
+    else if (options_.normal == NORMAL_Y) system.rotateCoordAxes("ZXY");
+    if (options_.vdw_pbc) {
+        std::vector<Vec3d> cv;
+        if (options_.use_external_potential) {
+            Cube c = readCube(options_.e_potential_file, false);
+            for (int a = 0; a < 3; ++a) cv.push_back((double)c.n[a] * c.voxel[a]);
+        } else {
+            cv = {options_.cell_a, options_.cell_b, options_.cell_c};
+        }
+        system.setUnitCell(cv);
+    }
+    system.centerMolecule(options_.center);
+    system.setMoleculeZ();
+    calculateTipDummyDistance();
+    buildInteractions();
+    lowerSystem();
+    initialized_ = true;
+}
+
+void Simulation::lowerSystem() {
+    low_ = Lowering();
+    low_.positions = &system.positions_;
+    for (const auto& it : interactions_) it->lower(low_);
+    if (!low_.tip_dummy_vdw_set) fail(MAFM_ERR_STATE, "no tip-dummy VDW term in the interaction list");
+    desc_ = mafm_system_desc{};
+    desc_.n_sites = (int32_t)low_.sites.size();
+    desc_.sites = low_.sites.data();
+    desc_.force_grid = low_.force_grid;
+    desc_.potential_grid = low_.potential_grid;
+    desc_.tip_pbc = (system.tipPbc() && !options_.rigidgrid) ? 1 : 0;   // setTipPbc(false), simulation.cpp:98-99
+    desc_.tip_dummy = low_.tip_dummy;
+    desc_.tip_dummy_d = system.getTipDummyDistance();
+    memcpy(desc_.cell, system.getUnitCell().m, sizeof desc_.cell);
+    desc_.offset[0] = system.getOffset().x;
+    desc_.offset[1] = system.getOffset().y;
+    desc_.offset[2] = system.getOffset().z;
+}
+
+mafm_scan_desc Simulation::scanDesc(const mafm_run_opts& r) const {
+    const InputOptions& o = options_;
+    mafm_scan_desc d{};
+    d.nx = n_points_[0];
+    d.ny = n_points_[1];
+    d.nz = n_points_[2];
+    d.dx = o.dx; d.dy = o.dy; d.dz = o.dz;
+    d.zhigh = o.zhigh;
+    d.minimiser = o.minimiser_type == FIRE ? MAFM_MIN_FIRE : MAFM_MIN_SD;
+    d.integrator = o.integrator_type == EULER ? MAFM_INT_EULER : (o.integrator_type == RK4 ? MAFM_INT_RK4 : MAFM_INT_MIDPOINT);
+    d.minterm = o.minterm == MIN_E ? MAFM_TERM_E : (o.minterm == MIN_F ? MAFM_TERM_F : MAFM_TERM_EF);
+    d.etol = o.etol; d.ftol = o.ftol; d.dt = o.dt;
+    d.maxsteps = o.maxsteps;
+    d.precision = r.precision;
+    d.lanes_per_pixel = r.lanes_per_pixel;
+    d.waves_per_simd = r.waves_per_simd;
+    d.compensation = r.compensation == 2 ? 2 : 1;
+    return d;
+}
+
+void Simulation::run(const mafm_run_opts& r, mafm_scan_stats* stats) {   // simulation.cpp:67-143
+    if (!initialized_) fail(MAFM_ERR_STATE, "run before initialize");
+    check(ctx_, mafm_set_systems(ctx_, 1, &desc_), "set system");
+    mafm_scan_desc d = scanDesc(r);
+    check(ctx_, mafm_scan(ctx_, &d, &last_stats_), "scan");
+    record_base_ = 0;
+    if (stats) *stats = last_stats_;
+}
+
+void Simulation::fetchRecords() {
+    size_t n = (size_t)n_points_[0] * n_points_[1] * n_points_[2];
+    records_.resize(n);
+    check(ctx_, mafm_records(ctx_, record_base_, n, records_.data()), "records");
+}
+
+// openUniverse / writeOutput / closeUniverse (mechafm.cpp:59-101,
+// simulation.cpp:145-199): one text file per height, fixed printf formats,
+// records in (i, j) order; each file is formatted on its own thread.
+void Simulation::writeOutput(int threads) {
+    if (records_.empty()) fetchRecords();
+    const InputOptions& o = options_;
+    const int nx = n_points_[0], ny = n_points_[1], nz = n_points_[2];
+    std::vector<std::string> errors(nz);
+    auto write_one = [&](int k) {
+        char name[512];
+        double z = o.zhigh - k * o.dz;
+        snprintf(name, sizeof name, "%sscan-%06.3f.dat", o.outputfolder.c_str(), z);
+        std::string body;
+        body.reserve((size_t)nx * ny * 112);
+        char line[256];
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j) {
+                const mafm_record& r = records_[((size_t)i * ny + j) * nz + k];
+                double rl = std::sqrt(std::pow(r.rx, 2) + std::pow(r.ry, 2) + std::pow(r.rz, 2));
+                double ang = std::atan2(std::sqrt(std::pow(r.rx, 2) + std::pow(r.ry, 2)), r.rz) * (180.0 / PI);
+                int len = snprintf(line, sizeof line,
+                                   "%d %d %d %6.3f %6.3f %6.3f %8.4f %8.4f %8.4f %6.3f %6.3f %6.3f %6.3f %8.4f %8.4f %d\n",
+                                   k, i, j, r.x, r.y, r.z, r.fx, r.fy, r.fz, r.rx, r.ry, r.rz, rl, ang, r.e, r.n_steps);
+                body.append(line, (size_t)len);
+            }
+        FILE* f;
+        if (o.gzip) {
+            std::string cmd = std::string("gzip -6 > '") + name + ".gz'";
+            f = popen(cmd.c_str(), "w");
+        } else {
+            f = fopen(name, "w");
+        }
+        if (!f) {
+            errors[k] = std::string("cannot open output ") + name;
+            return;
+        }
+        if (fwrite(body.data(), 1, body.size(), f) != body.size()) errors[k] = std::string("short write to ") + name;
+        if (o.gzip) pclose(f); else fclose(f);
+    };
+    threads = std::max(1, std::min(threads, nz));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t)
+        pool.emplace_back([&, t]() {
+            for (int k = t; k < nz; k += threads) write_one(k);
+        });
+    for (auto& th : pool) th.join();
+    for (const auto& e : errors)
+        if (!e.empty()) fail(MAFM_ERR_INVALID, "%s", e.c_str());
+    if (o.statistics) {                           // finalize (mechafm.cpp:129-152)
+        int64_t n_pts = (int64_t)nx * ny * nz;
+        std::string path = o.outputfolder + "statistics.txt";
+        FILE* f = fopen(path.c_str(), "w");
+        if (f) {
+            fprintf(f, "Simulation run finished\n");
+            fprintf(f, "Statistics:\n");
+            fprintf(f, "    Computed %lld tip positions\n", (long long)n_pts);
+            fprintf(f, "    Needed %lld minimization steps in total\n", (long long)last_stats_.total_steps);
+            fprintf(f, "    Which means approximately %.2f minimization steps per tip position\n",
+                    (double)last_stats_.total_steps / n_pts);
+            fprintf(f, "    The scan kernel time is %.2f seconds\n", last_stats_.kernel_ms / 1000.0);
+            fclose(f);
+        }
+    }
+}
+
+}  // namespace mafm
+
+// ================================================================ C ABI
+using mafm::Error;
+using mafm::Simulation;
+
+struct mafm_sim {
+    Simulation sim;
+    std::string err;
+    explicit mafm_sim(mafm_ctx* c) : sim(c) {}
+};
+
+template <typename F>
+static mafm_status guarded(mafm_sim* s, F&& f) {
+    try {
+        f();
+        return MAFM_OK;
+    } catch (const Error& e) {
+        if (s) s->err = e.what();
+        return e.status;
+    } catch (const std::exception& e) {
+        if (s) s->err = e.what();
+        return MAFM_ERR_INVALID;
+    }
+}
+
+extern "C" {
+
+mafm_status mafm_sim_open(const char* input, const char* outdir, mafm_ctx* ctx, mafm_sim** out) {
+    if (!input || !ctx || !out) return MAFM_ERR_INVALID;
+    *out = nullptr;
+    mafm_sim* s = new mafm_sim(ctx);
+    mafm_status st = guarded(s, [&]() {
+        s->sim.parseCommandLine(input, outdir ? outdir : "");
+        s->sim.readInputFile();
+        s->sim.readXYZFile();
+        s->sim.readParameterFile();
+    });
+    *out = s;   // returned even on failure so the caller can read the message
+    return st;
+}
+
+mafm_status mafm_sim_initialize(mafm_sim* s) {
+    if (!s) return MAFM_ERR_INVALID;
+    return guarded(s, [&]() { s->sim.initialize(); });
+}
+
+mafm_status mafm_sim_info_get(mafm_sim* s, mafm_sim_info* info) {
+    if (!s || !info) return MAFM_ERR_INVALID;
+    memset(info, 0, sizeof *info);
+    const Simulation& m = s->sim;
+    info->nx = m.n_points_[0];
+    info->ny = m.n_points_[1];
+    info->nz = m.n_points_[2];
+    info->n_atoms = m.system.n_atoms_ - 2;
+    info->n_sites = m.systemDesc().n_sites;
+    info->rigidgrid = m.options_.rigidgrid;
+    info->use_external_potential = m.options_.use_external_potential;
+    info->vdw_pbc = m.options_.vdw_pbc;
+    info->coulomb = m.options_.coulomb;
+    for (int a = 0; a < 3; ++a) info->force_grid_n[a] = m.force_grid_n_[a];
+    info->tip_dummy_d = m.system.getTipDummyDistance();
+    info->offset[0] = m.system.getOffset().x;
+    info->offset[1] = m.system.getOffset().y;
+    info->offset[2] = m.system.getOffset().z;
+    info->grid_build_ms = m.grid_build_ms_;
+    return MAFM_OK;
+}
+
+mafm_status mafm_sim_system(mafm_sim* s, mafm_system_desc* d) {
+    if (!s || !d) return MAFM_ERR_INVALID;
+    if (!s->sim.initialized_) {
+        s->err = "system requested before initialize";
+        return MAFM_ERR_STATE;
+    }
+    *d = s->sim.systemDesc();
+    return MAFM_OK;
+}
+
+mafm_status mafm_sim_run(mafm_sim* s, const mafm_run_opts* o, mafm_scan_stats* stats) {
+    if (!s) return MAFM_ERR_INVALID;
+    mafm_run_opts def{};
+    def.precision = MAFM_PREC_FAST64;
+    return guarded(s, [&]() {
+        s->sim.records_.clear();
+        s->sim.run(o ? *o : def, stats);
+    });
+}
+
+mafm_status mafm_batch_run(int n, mafm_sim* const* sims, const mafm_run_opts* o, mafm_scan_stats* stats) {
+    if (n <= 0 || !sims || !sims[0]) return MAFM_ERR_INVALID;
+    mafm_run_opts def{};
+    def.precision = MAFM_PREC_FAST32;
+    return guarded(sims[0], [&]() {
+        mafm_ctx* ctx = sims[0]->sim.ctx_;
+        std::vector<mafm_system_desc> descs;
+        mafm_scan_desc d0 = sims[0]->sim.scanDesc(o ? *o : def);
+        for (int m = 0; m < n; ++m) {
+            Simulation& s = sims[m]->sim;
+            if (!s.initialized_) throw Error(MAFM_ERR_STATE, "batch member not initialised");
+            if (s.ctx_ != ctx) throw Error(MAFM_ERR_INVALID, "batch members must share one context");
+            mafm_scan_desc d = s.scanDesc(o ? *o : def);
+            if (d.nx != d0.nx || d.ny != d0.ny || d.nz != d0.nz || d.dx != d0.dx || d.dy != d0.dy ||
+                d.dz != d0.dz || d.zhigh != d0.zhigh || d.minimiser != d0.minimiser ||
+                d.integrator != d0.integrator || d.minterm != d0.minterm || d.etol != d0.etol ||
+                d.ftol != d0.ftol || d.dt != d0.dt || d.maxsteps != d0.maxsteps)
+                throw Error(MAFM_ERR_INVALID, "batch members must share the scan options");
+            descs.push_back(s.systemDesc());
+        }
+        mafm_status st = mafm_set_systems(ctx, n, descs.data());
+        if (st != MAFM_OK) throw Error(st, mafm_last_error(ctx));
+        mafm_scan_stats ss{};
+        st = mafm_scan(ctx, &d0, &ss);
+        if (st != MAFM_OK) throw Error(st, mafm_last_error(ctx));
+        const size_t per = (size_t)d0.nx * d0.ny * d0.nz;
+        for (int m = 0; m < n; ++m) {
+            sims[m]->sim.records_.clear();
+            sims[m]->sim.record_base_ = per * m;
+            sims[m]->sim.last_stats_ = ss;
+        }
+        if (stats) *stats = ss;
+    });
+}
+
+mafm_status mafm_sim_records(mafm_sim* s, size_t first, size_t count, mafm_record* out) {
+    if (!s) return MAFM_ERR_INVALID;
+    return guarded(s, [&]() {
+        mafm_status st = mafm_records(s->sim.ctx_, s->sim.record_base_ + first, count, out);
+        if (st != MAFM_OK) throw Error(st, mafm_last_error(s->sim.ctx_));
+    });
+}
+
+mafm_status mafm_sim_write_output(mafm_sim* s, int threads) {
+    if (!s) return MAFM_ERR_INVALID;
+    return guarded(s, [&]() {
+        s->sim.fetchRecords();
+        s->sim.writeOutput(threads);
+    });
+}
+
+const char* mafm_sim_last_error(const mafm_sim* s) { return s ? s->err.c_str() : "no simulation"; }
+
+void mafm_sim_close(mafm_sim* s) { delete s; }
+
+}  // extern "C"

@@ -1,0 +1,292 @@
+// C++ host API of the MI355X MechAFM scan: the reference's Simulation /
+// System / Interaction / parser types (simulation.hpp, system.hpp,
+// interactions.hpp, parse.hpp), rebuilt around the device tables of mafm.h.
+//
+// What stays as in the reference: option names and defaults, file formats,
+// the set-up sequence (centre, lowest heavy atom at z=0, tip-dummy distance),
+// the interaction list and its order, the output files.  What changes:
+// Interaction objects are descriptors that lower themselves into device site
+// tables instead of evaluating on the CPU, and Simulation::run hands the whole
+// pixel / z / minimiser loop to one device launch.  Errors are thrown as
+// mafm::Error and turned into mafm_status at the C ABI (no exit()).
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/mafm.h"
+#include "../../../include/mafm_sim.h"
+
+namespace mafm {
+
+struct Error : std::runtime_error {
+    mafm_status status;
+    Error(mafm_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+struct Vec3d {
+    double x = 0, y = 0, z = 0;
+    Vec3d() = default;
+    Vec3d(double a, double b, double c) : x(a), y(b), z(c) {}
+    Vec3d operator+(const Vec3d& o) const { return {x + o.x, y + o.y, z + o.z}; }
+    Vec3d operator-(const Vec3d& o) const { return {x - o.x, y - o.y, z - o.z}; }
+    bool operator==(const Vec3d& o) const { return x == o.x && y == o.y && z == o.z; }
+};
+inline Vec3d operator*(double a, const Vec3d& v) { return {a * v.x, a * v.y, a * v.z}; }
+
+struct Vec2d {
+    double x = 0, y = 0;
+};
+
+// Row-major 3x3 (Matrix3, matrices.hpp): at(i, j) = m[3*i + j], columns are
+// the basis / cell vectors.
+struct Mat3d {
+    double m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double& at(int i, int j) { return m[3 * i + j]; }
+    double at(int i, int j) const { return m[3 * i + j]; }
+    Vec3d column(int j) const { return {m[j], m[3 + j], m[6 + j]}; }
+    double determinant() const;
+    Mat3d inverse() const;
+    bool isDiagonal() const;
+};
+
+enum MinimizationCriteria { MIN_E, MIN_F, MIN_EF, NOT_SET };
+enum SurfNormal { NORMAL_X, NORMAL_Y, NORMAL_Z };
+enum Units { U_KCAL, U_KJ, U_EV };
+enum MinimiserType { STEEPEST_DESCENT, FIRE };
+enum IntegratorType { EULER, MIDPOINT, RK4 };
+
+// InputOptions (simulation.hpp:30-60), defaults of parse.cpp:96-134.
+struct InputOptions {
+    std::string inputfolder, outputfolder, inputfile, xyzfile, paramfile, e_potential_file;
+    std::string tipatom, dummyatom;
+    Vec2d area{10, 10}, center{-1, -1};
+    double dx = 0.1, dy = 0.1, dz = 0.1;
+    double zlow = 6.0, zhigh = 10.0;
+    bool vdw_pbc = false;
+    Vec3d cell_a, cell_b, cell_c;
+    SurfNormal normal = NORMAL_Z;
+    Units units = U_KCAL;
+    bool units_given = false;
+    bool coulomb = false, tip_dummy_coulomb = false, use_external_potential = false;
+    int maxsteps = 5000;
+    MinimizationCriteria minterm = NOT_SET;
+    double etol = 0.01, ftol = 0.01, dt = 0.001;
+    int bufsize = 1000;
+    bool gzip = true, statistics = false, flexible = false, rigidgrid = false;
+    bool xyz_charges = false;
+    MinimiserType minimiser_type = FIRE;
+    IntegratorType integrator_type = MIDPOINT;
+};
+
+struct AtomParameters {
+    double eps = 0, sig = 0, q = 0, mass = 0;
+};
+
+struct OverwriteParameters {
+    std::string a, b;   // unordered pair
+    bool morse = false;
+    double eps = 0, sig = 0, de = 0, alpha = 0, re = 0;
+};
+
+// InteractionParameters (interactions.hpp:47-55), rigid subset.
+struct InteractionParameters {
+    double qbase = 0;
+    bool qbase_set = false;
+    double tip_dummy_k = 0, tip_dummy_r0 = 0;
+    bool harm_set = false;
+    std::map<std::string, AtomParameters> atom_parameters;
+    std::vector<OverwriteParameters> overwrite_parameters;
+};
+
+// Mixing rules (interactions.hpp:58-64).
+inline double mixsig(double s1, double s2) { return (s1 + s2) / 2; }
+inline double mixeps(double e1, double e2) { return std::sqrt(e1 * e2); }
+
+// Device-table builder that Interaction::lower() writes into.
+struct Lowering {
+    std::vector<mafm_site> sites;
+    mafm_tip_dummy tip_dummy{};
+    bool tip_dummy_vdw_set = false;
+    int force_grid = -1;
+    int potential_grid = -1;
+    const std::vector<Vec3d>* positions = nullptr;
+};
+
+// Interaction (interactions.hpp:68-76): immutable after build; lowers itself
+// into the device tables.  atom indices: 0 dummy, 1 tip, >= 2 surface.
+class Interaction {
+  public:
+    virtual ~Interaction() = default;
+    virtual bool isTipSurface() const = 0;
+    virtual void lower(Lowering& out) const = 0;
+};
+
+class LJInteraction : public Interaction {
+  public:
+    LJInteraction(int i1, int i2, double es6, double es12, Vec3d shift = Vec3d())
+        : i1_(i1), i2_(i2), es6_(es6), es12_(es12), shift_(shift) {}
+    bool isTipSurface() const override { return i1_ == 1; }
+    void lower(Lowering& out) const override;
+
+  private:
+    int i1_, i2_;
+    double es6_, es12_;
+    Vec3d shift_;
+};
+
+class MorseInteraction : public Interaction {
+  public:
+    MorseInteraction(int i1, int i2, double de, double a, double re, Vec3d shift = Vec3d())
+        : i1_(i1), i2_(i2), de_(de), a_(a), re_(re), shift_(shift) {}
+    bool isTipSurface() const override { return i1_ == 1; }
+    void lower(Lowering& out) const override;
+
+  private:
+    int i1_, i2_;
+    double de_, a_, re_;
+    Vec3d shift_;
+};
+
+class CoulombInteraction : public Interaction {
+  public:
+    CoulombInteraction(int i1, int i2, double qq) : i1_(i1), i2_(i2), qq_(qq) {}
+    bool isTipSurface() const override { return i1_ == 1; }
+    void lower(Lowering& out) const override;
+
+  private:
+    int i1_, i2_;
+    double qq_;
+};
+
+class TipHarmonicInteraction : public Interaction {
+  public:
+    TipHarmonicInteraction(int i1, int i2, double k, double r0) : i1_(i1), i2_(i2), k_(k), r0_(r0) {}
+    bool isTipSurface() const override { return false; }
+    void lower(Lowering& out) const override;
+
+  private:
+    int i1_, i2_;
+    double k_, r0_;
+};
+
+// ForceGrid bound to a device grid handle (force_grid.hpp:15-51).
+struct ForceGrid {
+    int handle = -1;
+    int n[3] = {0, 0, 0};
+    bool periodic = false;
+    Mat3d basis;
+    Vec3d offset;
+};
+
+class GridInteraction : public Interaction {
+  public:
+    explicit GridInteraction(const ForceGrid& g) : grid_(g) {}
+    bool isTipSurface() const override { return true; }
+    void lower(Lowering& out) const override { out.force_grid = grid_.handle; }
+    const ForceGrid& grid() const { return grid_; }
+
+  private:
+    ForceGrid grid_;
+};
+
+class ElectrostaticPotentialInteraction : public Interaction {
+  public:
+    explicit ElectrostaticPotentialInteraction(const ForceGrid& g) : grid_(g) {}
+    bool isTipSurface() const override { return true; }
+    void lower(Lowering& out) const override { out.potential_grid = grid_.handle; }
+    const ForceGrid& grid() const { return grid_; }
+
+  private:
+    ForceGrid grid_;
+};
+
+// System (system.hpp:13-77): the frozen surface plus dummy (0) and tip (1).
+class System {
+  public:
+    void initialize(int n_atoms);
+    void centerMolecule(Vec2d pos);
+    void setMoleculeZ();
+    void setUnitCell(const std::vector<Vec3d>& cell_vectors);
+    void rotateCoordAxes(const std::string& seq);
+    const Mat3d& getUnitCell() const { return cell_; }
+    const Vec3d& getOffset() const { return offset_; }
+    void setTipDummyDistance(double d) { tip_dummy_d_ = d; }
+    double getTipDummyDistance() const { return tip_dummy_d_; }
+    bool tipPbc() const { return tip_pbc_; }
+
+    int n_atoms_ = 0;   // including tip and dummy
+    std::vector<Vec3d> positions_;
+    std::vector<double> charges_;
+    std::vector<int> fixed_;
+    std::vector<std::string> types_;
+
+  private:
+    bool tip_pbc_ = false;
+    double tip_dummy_d_ = 0;
+    Vec3d offset_;
+    Mat3d cell_;
+};
+
+// Gaussian cube reader (cube_io.hpp:27-59).
+struct Cube {
+    int n[3] = {0, 0, 0};
+    Vec3d voxel[3];
+    Vec3d origin;
+    int n_atoms = 0;
+    std::vector<double> values;   // x-major, z fastest
+};
+Cube readCube(const std::string& path, bool with_values);
+
+class Simulation {
+  public:
+    explicit Simulation(mafm_ctx* ctx) : ctx_(ctx) {}
+    ~Simulation();
+    void parseCommandLine(const std::string& input, const std::string& output);
+    void readInputFile();
+    void readXYZFile();
+    void readParameterFile();
+    void initialize();
+    void buildInteractions();
+    // lower the interaction list into a device system description
+    void lowerSystem();
+    const mafm_system_desc& systemDesc() const { return desc_; }
+    mafm_scan_desc scanDesc(const mafm_run_opts& o) const;
+    void run(const mafm_run_opts& opts, mafm_scan_stats* stats);
+    void fetchRecords();
+    void writeOutput(int threads);
+
+    InputOptions options_;
+    InteractionParameters interaction_parameters_;
+    System system;
+    std::vector<std::unique_ptr<Interaction>> interactions_;
+    int n_points_[3] = {0, 0, 0};
+    std::vector<mafm_record> records_;
+    mafm_scan_stats last_stats_{};
+    double grid_build_ms_ = 0;
+    int force_grid_n_[3] = {0, 0, 0};
+    mafm_ctx* ctx_;
+    bool initialized_ = false;
+    // set by batch runs: this simulation's records start here in the context
+    size_t record_base_ = 0;
+
+  private:
+    void calculateTipDummyDistance();
+    bool findOverwriteParameters(int i1, int i2, OverwriteParameters& op) const;
+    void addVDWInteraction(int i1, int i2, Vec3d shift = Vec3d());
+    void addCoulombInteraction(int i1, int i2);
+    void buildTipSurfaceInteractions();
+    void buildTipGridInteractions();
+    void buildTipDummyInteractions();
+    ForceGrid buildElectrostaticPotential();
+    std::vector<int> owned_grids_;
+    Lowering low_;
+    mafm_system_desc desc_{};
+};
+
+}  // namespace mafm

@@ -1,0 +1,594 @@
+// Fast scan: persistent, work-stealing, one force evaluation per loop trip.
+//
+// Work decomposition
+//   * A wave (64 lanes, one wave per workgroup) owns one molecule at a time;
+//     the molecule index is wave-uniform, so its DevMol record and, for G == 1,
+//     every site it streams are read with scalar loads into SGPRs and fed to
+//     the VALU as broadcast operands (no LDS, no bank conflicts, no shuffles).
+//   * G lanes form a pixel group; the G lanes split the molecule's sites and
+//     combine their partial sums with a butterfly of lane shuffles.  G = 1 for
+//     large scans, G > 1 lets small scans (few pixel columns) fill the chip.
+//   * Each group steals pixel columns from its molecule's atomic counter
+//     (one atomic per wave per refill, ballot + prefix count).  When a
+//     molecule runs dry the wave moves to the molecule with work left.
+//   * The minimiser / integrator of simulation.cpp:102-122 runs as a per-group
+//     state machine in registers.  Every trip of the main loop performs exactly
+//     one evaluation of the interaction list per group: an integrator
+//     sub-step (midpoint 2, RK4 4, Euler 1, SD 1 per iteration) or the final
+//     tip-surface evaluation of System::getOutput.  Groups never wait for
+//     each other at convergence, only at the end of the launch.
+//
+// Arithmetic
+//   * Minimiser state, convergence tests (including the etol energy
+//     difference), tip-dummy terms and the integrator updates are FP64 and
+//     follow minimiser.cpp:67-129 / integrators.cpp:8-111.
+//   * Pair terms: F32 = FP32 with hardware reciprocal / reciprocal square root
+//     and Kahan-compensated accumulation (COMP == 1) or plain FP32 sums
+//     (COMP == 0); F64 = FP64 with Newton-refined hardware reciprocals.
+#include "device_common.hpp"
+
+namespace mafm {
+namespace {
+
+constexpr int PH_OUT = 7;
+
+__device__ inline double rcp64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+__device__ inline double rsq64(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double hx = 0.5 * x;
+    y = y * fma(-hx * y, y, 1.5);
+    y = y * fma(-hx * y, y, 1.5);
+    return y;
+}
+
+template <bool COMP>
+struct Acc32 {
+    float s[4];
+    float c[4];
+    __device__ void init() {
+        for (int q = 0; q < 4; ++q) { s[q] = 0.f; c[q] = 0.f; }
+    }
+    __device__ inline void add(int q, float x) {
+        if (COMP) {
+            float y = x - c[q];
+            float t = s[q] + y;
+            c[q] = (t - s[q]) - y;
+            s[q] = t;
+        } else {
+            s[q] += x;
+        }
+    }
+    __device__ double get(int q) const { return COMP ? (double)s[q] - (double)c[q] : (double)s[q]; }
+};
+
+// Sum of all tip-surface site terms of molecule M at tip position (px,py,pz),
+// sites [gl, gl+G, ...] of each segment.
+template <int G, int COMP>
+__device__ void site_sums32(const ScanArgs& a, const DevMol& M, double px, double py, double pz, int gl,
+                            double out[4]) {
+    const float tx = (float)(px - M.origin[0]);
+    const float ty = (float)(py - M.origin[1]);
+    const float tz = (float)(pz - M.origin[2]);
+    Acc32<COMP == 1> acc;
+    acc.init();
+    const Site32* s = a.seg32 + M.seg_off;
+    // LJ + Coulomb
+    {
+        const int n = M.n_seg[0];
+#pragma unroll 2
+        for (int t = gl; t < n; t += G) {
+            const float4 sa = s[t].a, sb = s[t].b;
+            float dx = tx - sa.x, dy = ty - sa.y, dz = tz - sa.z;
+            float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            float ir2 = __builtin_amdgcn_rcpf(r2);
+            float ir = __builtin_amdgcn_rsqf(r2);
+            float ir6 = ir2 * ir2 * ir2;
+            float tb = sa.w * ir6;
+            float ta = sb.x * ir6 * ir6;
+            float ec = sb.z * ir;
+            float fs = (fmaf(12.f, ta, -6.f * tb) + ec) * ir2;
+            acc.add(0, fs * dx);
+            acc.add(1, fs * dy);
+            acc.add(2, fs * dz);
+            acc.add(3, (ta - tb) + ec);
+        }
+        s += n;
+    }
+    // LJ
+    {
+        const int n = M.n_seg[1];
+#pragma unroll 2
+        for (int t = gl; t < n; t += G) {
+            const float4 sa = s[t].a;
+            const float es12 = s[t].b.x;
+            float dx = tx - sa.x, dy = ty - sa.y, dz = tz - sa.z;
+            float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            float ir2 = __builtin_amdgcn_rcpf(r2);
+            float ir6 = ir2 * ir2 * ir2;
+            float tb = sa.w * ir6;
+            float ta = es12 * ir6 * ir6;
+            float fs = fmaf(12.f, ta, -6.f * tb) * ir2;
+            acc.add(0, fs * dx);
+            acc.add(1, fs * dy);
+            acc.add(2, fs * dz);
+            acc.add(3, ta - tb);
+        }
+        s += n;
+    }
+    // Morse (+ Coulomb): E = De (x^2 - 2x + 1), |F| = 2 De a (x^2 - x), x = exp(-a (r - re))
+    for (int seg = 2; seg < 4; ++seg) {
+        const int n = M.n_seg[seg];
+        for (int t = gl; t < n; t += G) {
+            const float4 sa = s[t].a, sb = s[t].b;
+            float dx = tx - sa.x, dy = ty - sa.y, dz = tz - sa.z;
+            float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            float ir = __builtin_amdgcn_rsqf(r2);
+            float r = r2 * ir;
+            float x = exp2f(sb.x * (sb.y - r));
+            float em = sa.w * (x - 1.f) * (x - 1.f);
+            float fs = 2.f * sa.w * sb.w * x * (x - 1.f) * ir;
+            if (seg == 2) {
+                float ec = sb.z * ir;
+                fs += ec * ir * ir;
+                em += ec;
+            }
+            acc.add(0, fs * dx);
+            acc.add(1, fs * dy);
+            acc.add(2, fs * dz);
+            acc.add(3, em);
+        }
+        s += n;
+    }
+    for (int q = 0; q < 4; ++q) out[q] = acc.get(q);
+}
+
+template <int G>
+__device__ void site_sums64(const ScanArgs& a, const DevMol& M, double tx, double ty, double tz, int gl,
+                            double out[4]) {
+    double fx = 0, fy = 0, fz = 0, e = 0;
+    const Site64* s = a.seg64 + M.seg_off;
+    {
+        const int n = M.n_seg[0];
+#pragma unroll 2
+        for (int t = gl; t < n; t += G) {
+            const Site64 st = s[t];
+            double dx = tx - st.x, dy = ty - st.y, dz = tz - st.z;
+            double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            double ir2 = rcp64(r2);
+            double ir = rsq64(r2);
+            double ir6 = ir2 * ir2 * ir2;
+            double tb = st.p0 * ir6;
+            double ta = st.p1 * ir6 * ir6;
+            double ec = st.qq * ir;
+            double fs = (fma(12.0, ta, -6.0 * tb) + ec) * ir2;
+            fx = fma(fs, dx, fx); fy = fma(fs, dy, fy); fz = fma(fs, dz, fz);
+            e += (ta - tb) + ec;
+        }
+        s += n;
+    }
+    {
+        const int n = M.n_seg[1];
+#pragma unroll 2
+        for (int t = gl; t < n; t += G) {
+            const Site64 st = s[t];
+            double dx = tx - st.x, dy = ty - st.y, dz = tz - st.z;
+            double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            double ir2 = rcp64(r2);
+            double ir6 = ir2 * ir2 * ir2;
+            double tb = st.p0 * ir6;
+            double ta = st.p1 * ir6 * ir6;
+            double fs = fma(12.0, ta, -6.0 * tb) * ir2;
+            fx = fma(fs, dx, fx); fy = fma(fs, dy, fy); fz = fma(fs, dz, fz);
+            e += ta - tb;
+        }
+        s += n;
+    }
+    for (int seg = 2; seg < 4; ++seg) {
+        const int n = M.n_seg[seg];
+        for (int t = gl; t < n; t += G) {
+            const Site64 st = s[t];
+            double dx = tx - st.x, dy = ty - st.y, dz = tz - st.z;
+            double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            double ir = rsq64(r2);
+            double r = r2 * ir;
+            double x = exp(-st.p1 * (r - st.p2));
+            double em = st.p0 * (x * x - 2 * x + 1);
+            double fs = 2 * st.p0 * st.p1 * (x * x - x) * ir;
+            if (seg == 2) {
+                double ec = st.qq * ir;
+                fs += ec * ir * ir;
+                em += ec;
+            }
+            fx = fma(fs, dx, fx); fy = fma(fs, dy, fy); fz = fma(fs, dz, fz);
+            e += em;
+        }
+        s += n;
+    }
+    out[0] = fx; out[1] = fy; out[2] = fz; out[3] = e;
+}
+
+// Trilinear lookup of ForceGrid::interpolate (force_grid.cpp:88-155) with the
+// clamp / wrap rules of getGridPoint / getGridPointIndex / pbcGridPoint
+// (force_grid.cpp:158-275); FP64 weights, FP32 or FP64 samples.
+template <bool F32>
+__device__ void grid_lookup(const DevGrid& g, double px, double py, double pz, double out[4], int& flags) {
+    double q[3];
+    if (g.orthogonal) {
+        q[0] = (px - g.offset[0]) / g.basis[0];
+        q[1] = (py - g.offset[1]) / g.basis[4];
+        q[2] = (pz - g.offset[2]) / g.basis[8];
+    } else {
+        mat_vec(g.inv, px - g.offset[0], py - g.offset[1], pz - g.offset[2], q[0], q[1], q[2]);
+    }
+    int gp[3];
+    for (int d = 0; d < 3; ++d) {
+        gp[d] = (int)floor(q[d]);
+        if (!g.periodic) {
+            if (gp[d] < 0) { gp[d] = 0; flags |= REC_GRID_CLAMP; }
+            else if (gp[d] >= g.n[d]) { gp[d] = g.n[d] - 1; flags |= REC_GRID_CLAMP; }
+        }
+    }
+    int i0[3], i1[3];
+    for (int d = 0; d < 3; ++d) {
+        int a0 = gp[d], a1 = gp[d] + 1;
+        if (g.periodic) {
+            a0 -= (int)floor((double)a0 / g.n[d]) * g.n[d];
+            a1 -= (int)floor((double)a1 / g.n[d]) * g.n[d];
+        } else if (a1 >= g.n[d]) {
+            flags |= REC_OUT_OF_GRID;
+            a1 = g.n[d] - 1;
+        }
+        // final clamp: keeps every gather inside the grid even for non-finite positions
+        i0[d] = a0 < 0 ? 0 : (a0 >= g.n[d] ? g.n[d] - 1 : a0);
+        i1[d] = a1 < 0 ? 0 : (a1 >= g.n[d] ? g.n[d] - 1 : a1);
+    }
+    const double w[3] = {q[0] - gp[0], q[1] - gp[1], q[2] - gp[2]};
+    double acc[4] = {0, 0, 0, 0};
+    for (int c = 0; c < 8; ++c) {
+        const int ix = (c & 4) ? i1[0] : i0[0];
+        const int iy = (c & 2) ? i1[1] : i0[1];
+        const int iz = (c & 1) ? i1[2] : i0[2];
+        const double wt = ((c & 4) ? w[0] : 1 - w[0]) * ((c & 2) ? w[1] : 1 - w[1]) * ((c & 1) ? w[2] : 1 - w[2]);
+        const long idx = ((long)ix * g.n[1] + iy) * g.n[2] + iz;
+        double v0, v1, v2, v3;
+        if (F32) {
+            float4 v = g.fe32[idx];
+            v0 = v.x; v1 = v.y; v2 = v.z; v3 = v.w;
+        } else {
+            double4 v = g.fe64[idx];
+            v0 = v.x; v1 = v.y; v2 = v.z; v3 = v.w;
+        }
+        acc[0] = fma(wt, v0, acc[0]);
+        acc[1] = fma(wt, v1, acc[1]);
+        acc[2] = fma(wt, v2, acc[2]);
+        acc[3] = fma(wt, v3, acc[3]);
+    }
+    for (int k = 0; k < 4; ++k) out[k] += acc[k];
+}
+
+// Tip-dummy VDW, Coulomb and lateral spring on the tip (simulation.cpp:503-515,
+// interactions.cpp:15-51, 252-266), FP64.
+__device__ void tip_dummy(const DevMol& M, double dx_, double dy_, double dz_, double tx, double ty, double tz,
+                          double out[4]) {
+    const double rx = dx_ - tx, ry = dy_ - ty, rz = dz_ - tz;
+    const double r2 = rx * rx + ry * ry + rz * rz;
+    double s, e;
+    if (M.td_vdw == 0) {
+        double r6 = r2 * r2 * r2;
+        double ta = M.td_p1 / (r6 * r6);
+        double tb = M.td_p0 / r6;
+        e = ta - tb;
+        s = (12 * ta - 6 * tb) / r2;
+    } else {
+        double r = sqrt(r2);
+        double x = exp(-M.td_p1 * (r - M.td_p2));
+        e = M.td_p0 * (x * x - 2 * x + 1);
+        s = 2 * M.td_p0 * M.td_p1 * (x * x - x) / r;
+    }
+    if (M.td_coul) {
+        double r = sqrt(r2);
+        e += M.td_qq / r;
+        s += M.td_qq / (r * r * r);
+    }
+    double fx = -s * rx, fy = -s * ry, fz = -s * rz;
+    const double rl = sqrt(rx * rx + ry * ry);
+    const double dr = rl - M.td_r0;
+    e += M.td_k * dr * dr;
+    if (rl > 1e-10) {
+        const double h = -2 * M.td_k * dr / rl;
+        fx -= h * rx;
+        fy -= h * ry;
+    }
+    out[0] += fx; out[1] += fy; out[2] += fz; out[3] += e;
+}
+
+template <int G>
+__device__ inline double group_sum(double v) {
+#pragma unroll
+    for (int m = 1; m < G; m <<= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+__device__ inline bool converged(const ScanArgs& a, double fx, double fy, double fz, double ediff) {
+    const double fl = sqrt(fx * fx + fy * fy + fz * fz);
+    switch (a.minterm) {
+        case 0: return fabs(ediff) < a.etol;
+        case 1: return fl < a.ftol;
+        default: return fabs(ediff) < a.etol && fl < a.ftol;
+    }
+}
+
+__device__ int pick_molecule(const ScanArgs& a, int wave, int n_waves) {
+    const float target = (wave + 0.5f) / n_waves;
+    int lo = 0, hi = a.n_mols - 1;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a.mol_cum[mid + 1] > target) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
+template <int G, bool F32, int COMP>
+__global__ void __launch_bounds__(64) k_scan_fast(ScanArgs a, int n_waves) {
+    const int lane = threadIdx.x;
+    const int gl = lane % G;
+    const int gbase = lane - gl;
+    const bool leader = gl == 0;
+    const int n_cols = a.nx * a.ny;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+    int m = __builtin_amdgcn_readfirstlane(pick_molecule(a, blockIdx.x, n_waves));
+
+    // group state (identical on the G lanes of a group)
+    int col = -1, k = 0, ph = 0, n_tot = 1, nnn = 0, flags = 0, i = 0, j = 0;
+    double dmx = 0, dmy = 0, dmz = 0, tx = 0, ty = 0, tz = 0, vx = 0, vy = 0, vz = 0;
+    double fx = 0, fy = 0, fz = 0, e = 0, prev_e = 0, dt = a.dt, alpha = 0.1;
+    double ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0, cx = 0, cy = 0, cz = 0, ce = 0;
+    double Px = 0, Py = 0, Pz = 1.0e3;
+    bool ts = false;
+    int64_t steps = 0, unconv = 0, warns = 0, oogs = 0;
+    const double dt_max = 10 * a.dt;
+
+    for (;;) {
+        // ---- refill: groups without a column steal one from molecule m
+        const bool need = leader && col == -1;
+        const uint64_t need_mask = __ballot(need);
+        if (need_mask) {
+            const int first = __ffsll((unsigned long long)need_mask) - 1;
+            int base = 0;
+            if (lane == first) base = atomicAdd(&a.next_col[m], __popcll(need_mask));
+            base = __shfl(base, first, 64);
+            if (need) {
+                const int c = base + __popcll(need_mask & lt_mask);
+                col = c < n_cols ? c : -2;
+            }
+            const int gcol = (G > 1) ? __shfl(col, gbase, 64) : col;
+            const bool fresh = (col == -1) && gcol >= 0;
+            col = gcol;
+            if (fresh || (need && col >= 0)) {
+                // new pixel column: System copy, setDummyXY, setDummyZ (simulation.cpp:93-101)
+                const DevMol& M = a.mols[m];
+                i = col / a.ny;
+                j = col - i * a.ny;
+                const double x = i * a.dx, y = j * a.dy;
+                double px = x, py = y;
+                if (M.tip_pbc) {
+                    double qx, qy, qz;
+                    mat_vec(M.cell_inv, x - M.offset[0], y - M.offset[1], 0 - M.offset[2], qx, qy, qz);
+                    qx -= floor(qx);
+                    qy -= floor(qy);
+                    double ox, oy, oz;
+                    mat_vec(M.cell, qx, qy, qz, ox, oy, oz);
+                    px = ox + M.offset[0];
+                    py = oy + M.offset[1];
+                }
+                dmx = px; dmy = py; dmz = a.zhigh;
+                tx = px; ty = py; tz = a.zhigh - M.tip_dummy_d;
+                vx = vy = vz = 0;
+                fx = fy = fz = 0;
+                e = 0;
+                k = 0;
+                // begin z-step
+                n_tot = 1; dt = a.dt; alpha = 0.1; nnn = 0; flags = 0;
+                if (n_tot >= a.maxsteps) { ph = PH_OUT; ts = true; Px = tx; Py = ty; Pz = tz; }
+                else { prev_e = e; ph = 0; ts = false; Px = tx; Py = ty; Pz = tz; }
+            }
+        }
+        if (!__any(col >= 0)) {
+            // molecule m exhausted for this wave: move to one with columns left
+            int nm = -1;
+            for (int q = 1; q <= a.n_mols; ++q) {
+                int cand = m + q;
+                if (cand >= a.n_mols) cand -= a.n_mols;
+                if (__atomic_load_n(&a.next_col[cand], __ATOMIC_RELAXED) < n_cols) { nm = cand; break; }
+            }
+            nm = __builtin_amdgcn_readfirstlane(nm);
+            if (nm < 0) break;
+            m = nm;
+            col = -1;
+            continue;
+        }
+
+        // ---- one evaluation of the interaction list at P for every group
+        const DevMol& M = a.mols[m];
+        double r[4];
+        if (M.force_grid >= 0) {
+            r[0] = r[1] = r[2] = r[3] = 0;
+            grid_lookup<F32>(a.grids[M.force_grid], Px, Py, Pz, r, flags);
+        } else {
+            if (F32) site_sums32<G, COMP>(a, M, Px, Py, Pz, gl, r);
+            else site_sums64<G>(a, M, Px, Py, Pz, gl, r);
+            if (G > 1) {
+                for (int q = 0; q < 4; ++q) r[q] = group_sum<G>(r[q]);
+            }
+            if (M.potential_grid >= 0) grid_lookup<F32>(a.grids[M.potential_grid], Px, Py, Pz, r, flags);
+        }
+        if (!ts) tip_dummy(M, dmx, dmy, dmz, Px, Py, Pz, r);
+
+        if (col < 0) continue;
+
+        // ---- state machine (FP64)
+        bool step_done = false;
+        if (ph == PH_OUT) {
+            if (leader) {
+                DevRecord rec;
+                rec.x = i * a.dx; rec.y = j * a.dy; rec.z = dmz;
+                rec.fx = r[0]; rec.fy = r[1]; rec.fz = r[2];
+                rec.rx = tx - dmx; rec.ry = ty - dmy; rec.rz = tz - dmz;
+                rec.e = r[3];
+                rec.n_steps = n_tot;
+                rec.flags = flags | (n_tot >= a.maxsteps ? REC_UNCONVERGED : 0);
+                a.rec[M.rec_off + ((long)col * a.nz + k)] = rec;
+                steps += n_tot;
+                unconv += n_tot >= a.maxsteps;
+                warns += (flags & REC_GRID_CLAMP) != 0;
+                oogs += (flags & REC_OUT_OF_GRID) != 0;
+            }
+            dmz -= a.dz;
+            tz -= a.dz;
+            ++k;
+            if (k >= a.nz) {
+                col = -1;
+                continue;
+            }
+            n_tot = 1; dt = a.dt; alpha = 0.1; nnn = 0; flags = 0;
+            if (n_tot >= a.maxsteps) { ph = PH_OUT; ts = true; }
+            else { prev_e = e; ph = 0; ts = false; }
+            Px = tx; Py = ty; Pz = tz;
+            continue;
+        }
+
+        if (a.minimiser == 0) {
+            // SDMinimisation (minimiser.cpp:39-65): evaluate, test, move
+            fx = r[0]; fy = r[1]; fz = r[2]; e = r[3];
+            if (converged(a, fx, fy, fz, e - prev_e)) {
+                ph = PH_OUT;
+            } else {
+                tx += a.dt * fx; ty += a.dt * fy; tz += a.dt * fz;
+                ++n_tot;
+                ph = n_tot >= a.maxsteps ? PH_OUT : 0;
+                prev_e = e;
+            }
+            ts = ph == PH_OUT;
+            Px = tx; Py = ty; Pz = tz;
+            continue;
+        }
+
+        if (a.integrator == 1) {                       // midpointStep
+            if (ph == 0) {
+                Px = tx + dt / 2 * vx; Py = ty + dt / 2 * vy; Pz = tz + dt / 2 * vz;
+                ax = vx + dt / 2 * r[0]; ay = vy + dt / 2 * r[1]; az = vz + dt / 2 * r[2];
+                ph = 1;
+            } else {
+                tx += dt * ax; ty += dt * ay; tz += dt * az;
+                vx += dt * r[0]; vy += dt * r[1]; vz += dt * r[2];
+                fx = r[0]; fy = r[1]; fz = r[2]; e = r[3];
+                step_done = true;
+            }
+        } else if (a.integrator == 0) {                // eulerStep
+            tx += dt * vx; ty += dt * vy; tz += dt * vz;
+            vx += dt * r[0]; vy += dt * r[1]; vz += dt * r[2];
+            fx = r[0]; fy = r[1]; fz = r[2]; e = r[3];
+            step_done = true;
+        } else {                                       // rk4Step
+            if (ph == 0) {
+                cx = r[0]; cy = r[1]; cz = r[2]; ce = r[3];
+                bx = vx; by = vy; bz = vz;
+                Px = tx + dt / 2 * vx; Py = ty + dt / 2 * vy; Pz = tz + dt / 2 * vz;
+                ax = vx + dt / 2 * r[0]; ay = vy + dt / 2 * r[1]; az = vz + dt / 2 * r[2];
+                ph = 1;
+            } else if (ph == 1) {
+                cx += 2 * r[0]; cy += 2 * r[1]; cz += 2 * r[2]; ce += 2 * r[3];
+                bx += 2 * ax; by += 2 * ay; bz += 2 * az;
+                Px = tx + dt / 2 * ax; Py = ty + dt / 2 * ay; Pz = tz + dt / 2 * az;
+                ax = vx + dt / 2 * r[0]; ay = vy + dt / 2 * r[1]; az = vz + dt / 2 * r[2];
+                ph = 2;
+            } else if (ph == 2) {
+                cx += 2 * r[0]; cy += 2 * r[1]; cz += 2 * r[2]; ce += 2 * r[3];
+                bx += 2 * ax; by += 2 * ay; bz += 2 * az;
+                Px = tx + dt * ax; Py = ty + dt * ay; Pz = tz + dt * az;
+                ax = vx + dt * r[0]; ay = vy + dt * r[1]; az = vz + dt * r[2];
+                ph = 3;
+            } else {
+                cx += r[0]; cy += r[1]; cz += r[2]; ce += r[3];
+                bx += ax; by += ay; bz += az;
+                tx += dt / 6 * bx; ty += dt / 6 * by; tz += dt / 6 * bz;
+                vx += dt / 6 * cx; vy += dt / 6 * cy; vz += dt / 6 * cz;
+                fx = cx / 6; fy = cy / 6; fz = cz / 6; e = ce / 6;
+                step_done = true;
+            }
+        }
+        if (!step_done) continue;
+
+        // FIREMinimisation iteration tail (minimiser.cpp:102-126)
+        if (converged(a, fx, fy, fz, e - prev_e)) {
+            ph = PH_OUT;
+        } else {
+            const double p = vx * fx + vy * fy + vz * fz;
+            const double fl = sqrt(fx * fx + fy * fy + fz * fz);
+            double hx = 0, hy = 0, hz = 0;
+            if (fl > 1e-10) { hx = fx / fl; hy = fy / fl; hz = fz / fl; }
+            const double vl = sqrt(vx * vx + vy * vy + vz * vz);
+            vx = (1 - alpha) * vx + alpha * hx * vl;
+            vy = (1 - alpha) * vy + alpha * hy * vl;
+            vz = (1 - alpha) * vz + alpha * hz * vl;
+            if (p < 0) {
+                vx = vy = vz = 0;
+                nnn = 0;
+                dt *= 0.5;
+                alpha = 0.1;
+            } else {
+                if (nnn > 5) {
+                    dt = fmin(dt * 1.1, dt_max);
+                    alpha *= 0.99;
+                }
+                ++nnn;
+            }
+            ++n_tot;
+            ph = n_tot >= a.maxsteps ? PH_OUT : 0;
+            prev_e = e;
+        }
+        ts = ph == PH_OUT;
+        Px = tx; Py = ty; Pz = tz;
+    }
+    if (leader) {
+        if (steps) atomicAdd((unsigned long long*)&a.counters[0], (unsigned long long)steps);
+        if (unconv) atomicAdd((unsigned long long*)&a.counters[1], (unsigned long long)unconv);
+        if (warns) atomicAdd((unsigned long long*)&a.counters[2], (unsigned long long)warns);
+        if (oogs) atomicAdd((unsigned long long*)&a.counters[3], (unsigned long long)oogs);
+    }
+}
+
+template <int G, bool F32, int COMP>
+hipError_t launch_one(const ScanArgs& a, int n_waves, hipStream_t st) {
+    hipLaunchKernelGGL((k_scan_fast<G, F32, COMP>), dim3(n_waves), dim3(64), 0, st, a, n_waves);
+    return hipGetLastError();
+}
+
+template <bool F32, int COMP>
+hipError_t launch_g(const ScanArgs& a, int G, int n_waves, hipStream_t st) {
+    switch (G) {
+        case 1: return launch_one<1, F32, COMP>(a, n_waves, st);
+        case 2: return launch_one<2, F32, COMP>(a, n_waves, st);
+        case 4: return launch_one<4, F32, COMP>(a, n_waves, st);
+        case 8: return launch_one<8, F32, COMP>(a, n_waves, st);
+        case 16: return launch_one<16, F32, COMP>(a, n_waves, st);
+        case 32: return launch_one<32, F32, COMP>(a, n_waves, st);
+        case 64: return launch_one<64, F32, COMP>(a, n_waves, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_waves, hipStream_t st) {
+    if (!f32) return launch_g<false, 0>(a, G, n_waves, st);
+    return comp ? launch_g<true, 1>(a, G, n_waves, st) : launch_g<true, 0>(a, G, n_waves, st);
+}
+
+}  // namespace mafm

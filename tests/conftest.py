@@ -1,0 +1,59 @@
+import gzip
+import os
+import shutil
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+
+
+def golden_cases():
+    return sorted(d for d in os.listdir(GOLDEN) if os.path.isdir(os.path.join(GOLDEN, d)))
+
+
+def materialize(case: str, dst: str) -> str:
+    """Copy a golden case's input files into dst (gunzipping cubes); returns
+    the path of its input.scan."""
+    src = os.path.join(GOLDEN, case)
+    os.makedirs(dst, exist_ok=True)
+    for fn in os.listdir(src):
+        p = os.path.join(src, fn)
+        if fn.endswith(".npz"):
+            continue
+        if fn.endswith(".gz"):
+            with gzip.open(p, "rb") as fi, open(os.path.join(dst, fn[:-3]), "wb") as fo:
+                fo.write(fi.read())
+        else:
+            shutil.copy(p, os.path.join(dst, fn))
+    return os.path.join(dst, "input.scan")
+
+
+def golden_records(case: str):
+    d = np.load(os.path.join(GOLDEN, case, "records.npz"))
+    return tuple(int(v) for v in d["shape"]), d["records"]
+
+
+def golden_eval(case: str):
+    p = os.path.join(GOLDEN, case, "eval.npz")
+    if not os.path.exists(p):
+        return None
+    d = np.load(p)
+    return d["pos"], d["fe"]
+
+
+@pytest.fixture(scope="session")
+def gpu_ctx():
+    from mechafm_amd import abi
+    ctx = abi.Context(0)
+    yield ctx
+    ctx.close()
