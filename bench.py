@@ -1,0 +1,203 @@
+"""Throughput of the rigid probe-particle scan on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3]
+
+A step is one full scan of the workload (every pixel column, every height,
+FIRE + midpoint to convergence) in one launch of the work-stealing kernel,
+with the molecules' tables already resident in HBM.  Default workload: the
+batched one (64 seeded planar molecules, N ~ U[20, 400], LJ + Coulomb,
+64 x 64 px x 40 z each), FP32 pair terms with Kahan-compensated sums and
+FP64 minimiser state.
+
+Printed JSON line: metric/value (relaxed tip positions per second, whole
+job), the roofline of the scan kernel (algorithmic FP32 VALU FLOPs per launch
+over its HIP-event duration, against the 157.3 TFLOP/s FP32 vector peak),
+the plain FP64 GPU scan on the same workload (speed and the fraction of
+positions within 1e-4 A / 1e-3 relative Fz of it), and a CPU baseline: the
+reference program timed on this host on a sample of the same workload.
+"""
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_VALU_PEAK = 157.3e12      # MI355X FP32 vector, FLOP/s
+# algorithmic FLOPs per pair term and per evaluation / iteration overheads
+PAIR_FLOPS = {(0, 0): 26, (0, 1): 31, (1, 0): 28, (1, 1): 33}   # (vdw, coulomb)
+TIP_DUMMY_FLOPS = 50
+ITER_FLOPS = 52
+EVALS_PER_ITER = {0: 1, 1: 2, 2: 4}                             # euler, midpoint, rk4
+
+
+def workloads(name, tmp):
+    from mechafm_amd import workloads as W
+    if name == "c3":
+        return W.make_c3(tmp)
+    return [W.MAKERS[name](tmp)]
+
+
+def pair_cost(sim):
+    return sum(PAIR_FLOPS[(s.vdw, int(s.has_coulomb and s.qq != 0.0))] for s in sim.sites())
+
+
+def algorithmic_flops(sims, recs, integrator=1):
+    e = EVALS_PER_ITER[integrator]
+    tot = 0.0
+    for s, r in zip(sims, recs):
+        c = pair_cost(s)
+        steps = float(r["n_steps"].astype(np.int64).sum())
+        tot += steps * (e * (c + TIP_DUMMY_FLOPS) + ITER_FLOPS) + len(r) * c
+    return tot
+
+
+def run_batch(abi, sims, prec, **kw):
+    if len(sims) == 1:
+        return sims[0].run(prec, **kw)
+    return abi.batch_run(sims, prec, **kw)
+
+
+def autotune(abi, sims, prec, candidates):
+    best = None
+    for G, wps in candidates:
+        st = run_batch(abi, sims, prec, lanes_per_pixel=G, waves_per_simd=wps)
+        print(f"# tune G={G} waves/SIMD={wps}: {st['kernel_ms']:.1f} ms", file=sys.stderr, flush=True)
+        if best is None or st["kernel_ms"] < best[2]:
+            best = (G, wps, st["kernel_ms"])
+    return best
+
+
+def cpu_baseline(name, tmp, budget_s=25.0):
+    """The reference program (oracle/_ref/mechafm-omp, built from the
+    reference sources) on a sample of the same workload: every molecule, the
+    same scan area and heights, every 8th pixel in x and y (dx = dy = 0.8)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "mechafm-omp")
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    from mechafm_amd import workloads as W
+    sample_dir = os.path.join(tmp, "cpu_sample")
+    if name == "c3":
+        wls = W.make_c3(sample_dir, dx="0.8", dy="0.8")
+    else:
+        wls = [W.MAKERS[name](sample_dir, dx="0.8", dy="0.8")]
+    positions, elapsed = 0, 0.0
+    kind = "reference" if os.path.exists(exe) else "port"
+    for w in wls:
+        shape = W.scan_shape(w.options)
+        out = os.path.join(tmp, "cpu_out")
+        t0 = time.perf_counter()
+        if kind == "reference":
+            subprocess.run([exe, w.input_path, out], check=True, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL, env=dict(os.environ, OMP_NUM_THREADS=str(threads)))
+        else:
+            from oracle import oracle as O
+            O.scan(O.setup(w.input_path))
+        elapsed += time.perf_counter() - t0
+        positions += shape[0] * shape[1] * shape[2]
+        shutil.rmtree(out, ignore_errors=True)
+        if elapsed > budget_s:
+            break
+    return {"value": positions / elapsed, "unit": "positions/s", "cores": threads, "kind": kind,
+            "sample": f"{positions} positions: {len(wls)} system(s) of the workload at every 8th pixel in "
+                      f"x and y (dx = dy = 0.8 A), all heights; {elapsed:.1f} s wall on {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--precision", default="fast32", choices=["fast32", "fast64", "plain64"])
+    ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (pixel columns would shard "
+              "across GPUs with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from mechafm_amd import abi
+    tmp = tempfile.mkdtemp(prefix="mafm_bench_")
+    try:
+        t0 = time.perf_counter()
+        wls = workloads(args.config, tmp)
+        ctx = abi.Context(0)
+        dev = ctx.device_info()
+        sims = [abi.Simulation(w.input_path, ctx) for w in wls]
+        info = sims[0].info()
+        shape = (info["nx"], info["ny"], info["nz"])
+        positions = shape[0] * shape[1] * shape[2] * len(sims)
+        print(f"# workload {args.config}: {len(sims)} systems, {positions} positions, set-up "
+              f"{time.perf_counter() - t0:.1f} s on {dev['name']} ({dev['compute_units']} CUs)",
+              file=sys.stderr, flush=True)
+
+        G, wps = args.lanes, args.waves
+        if args.precision != "plain64" and not args.no_tune and (G == 0 or wps == 0):
+            cands = [(g, w) for g in ([G] if G else [1, 2, 4]) for w in ([wps] if wps else [1, 2])]
+            G, wps, _ = autotune(abi, sims, args.precision, cands)
+        kw = dict(lanes_per_pixel=G, waves_per_simd=wps)
+        for _ in range(args.warmup):
+            run_batch(abi, sims, args.precision, **kw)
+        kms = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            st = run_batch(abi, sims, args.precision, **kw)
+            kms.append(st["kernel_ms"])
+        wall = time.perf_counter() - t0
+        value = positions * args.steps / wall
+        recs = [s.records().copy() for s in sims]
+        flops = algorithmic_flops(sims, recs)
+        k_avg = float(np.mean(kms)) / 1e3
+        achieved = flops / k_avg
+        out = {
+            "metric": "relaxed tip positions/s at 1e-4 A error; % FP32 VALU peak vs plain FP64",
+            "value": value, "unit": "positions/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32" if args.precision == "fast32" else "f64", "data": "synthetic",
+            "config": {"workload": {"c3": "64 seeded planar molecules, N~U[20,400], LJ+Coulomb, "
+                                          "64x64 px x 40 z each, FIRE+midpoint, one batched launch"}.get(
+                args.config, args.config), "systems": len(sims), "positions": positions,
+                "precision": args.precision, "lanes_per_pixel": st["lanes_per_pixel"],
+                "waves_per_simd": st["waves_per_simd"], "total_steps": st["total_steps"],
+                "unconverged": st["unconverged"]},
+            "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": FP32_VALU_PEAK / 1e12,
+                         "unit": "TFLOP/s", "frac": achieved / FP32_VALU_PEAK, "traffic": None,
+                         "kernel_ms": k_avg * 1e3, "flops_per_launch": flops},
+        }
+        traffic = os.path.join(ROOT, "profiles", f"traffic_scan_{args.config}_{args.precision}.json")
+        if os.path.exists(traffic):
+            out["roofline"]["traffic"] = json.load(open(traffic)).get("hbm_bytes_per_launch")
+        if not args.no_plain and args.precision != "plain64":
+            t0 = time.perf_counter()
+            pst = run_batch(abi, sims, "plain64")
+            pwall = time.perf_counter() - t0
+            prec = [s.records() for s in sims]
+            dr = np.concatenate([np.abs(np.stack([a["rx"] - b["rx"], a["ry"] - b["ry"], a["rz"] - b["rz"]])).max(0)
+                                 for a, b in zip(recs, prec)])
+            dfz = np.concatenate([np.abs(a["fz"] - b["fz"]) / np.maximum(np.abs(b["fz"]), 1e-2)
+                                  for a, b in zip(recs, prec)])
+            ok = (dr <= 1e-4) & (dfz <= 1e-3)
+            out["plain_fp64"] = {"value": positions / (pst["kernel_ms"] / 1e3), "kernel_ms": pst["kernel_ms"],
+                                 "wall_s": pwall, "speedup": (pst["kernel_ms"] / 1e3) / k_avg,
+                                 "within_1e-4A_1e-3Fz": float(ok.mean()), "max_dr": float(dr.max()),
+                                 "median_dr": float(np.median(dr))}
+        if not args.no_cpu:
+            out["cpu_baseline"] = cpu_baseline(args.config, tmp)
+        print(json.dumps(out), flush=True)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    main()

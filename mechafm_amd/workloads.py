@@ -105,17 +105,33 @@ def planar_molecule(n_atoms: int, rng: np.random.Generator,
     heavy_types = [t for t, _ in fractions if t != "H" for _ in range(counts[t])]
     rng.shuffle(heavy_types)
     n_heavy = len(heavy_types)
-    pos: List[np.ndarray] = [np.zeros(2)]
+    buf = np.zeros((n_atoms + 1, 2))
+    count = [1]
     deg = [0]
 
+    class _Pos:
+        def __getitem__(self, k):
+            return buf[k]
+
+        def __len__(self):
+            return count[0]
+
+        def append(self, p):
+            buf[count[0]] = p
+            count[0] += 1
+
+    pos = _Pos()
+
     def try_place(parent: int) -> Optional[np.ndarray]:
+        live = buf[: count[0]]
         for _ in range(40):
             ang = rng.uniform(0, 2 * math.pi)
             b = rng.uniform(*bond)
-            p = pos[parent] + b * np.array([math.cos(ang), math.sin(ang)])
-            d = np.linalg.norm(np.array(pos) - p, axis=1)
-            d[parent] = 1e9
-            if d.min() >= max(min_dist, b * 0.999) or d.min() >= 1.3:
+            p = buf[parent] + b * np.array([math.cos(ang), math.sin(ang)])
+            d2 = ((live - p) ** 2).sum(axis=1)
+            d2[parent] = 1e18
+            dmin = math.sqrt(d2.min())
+            if dmin >= max(min_dist, b * 0.999) or dmin >= 1.3:
                 return p
         return None
 
@@ -152,7 +168,7 @@ def planar_molecule(n_atoms: int, rng: np.random.Generator,
         types.append("H")
         n_added += 1
     xyz = np.zeros((len(pos), 3))
-    xyz[:, :2] = np.array(pos)
+    xyz[:, :2] = buf[: len(pos)]
     q = rng.normal(0.0, charge_sigma, len(pos))
     q -= q.mean()
     return Molecule(types, xyz, q)
