@@ -126,7 +126,7 @@ typedef struct {
     /* tuning, 0 = automatic */
     int32_t lanes_per_pixel;   /* 1,2,4,...,64 lanes share one pixel column */
     int32_t waves_per_simd;    /* persistent grid size */
-    int32_t compensation;      /* FAST32: 0 none, 1 Kahan (default when 0 and autotune off: 1) */
+    int32_t compensation;      /* FAST32 sums: 0 or 1 Kahan-compensated, 2 plain FP32 */
     int32_t reserved;
 } mafm_scan_desc;
 
@@ -154,6 +154,8 @@ typedef struct {
     int32_t waves_per_simd;
     int32_t grid_blocks;
     int32_t block_threads;
+    int64_t group_trips;        /* FAST kernels: pixel-group slots x loop trips */
+    int64_t active_group_trips; /* ... of which the group had a column (utilisation) */
 } mafm_scan_stats;
 
 typedef struct {

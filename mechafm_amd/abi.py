@@ -65,7 +65,8 @@ class ScanStats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("positions", ctypes.c_int64), ("total_steps", ctypes.c_int64),
                 ("unconverged", ctypes.c_int64), ("grid_warnings", ctypes.c_int64), ("pair_evals", ctypes.c_int64),
                 ("lanes_per_pixel", ctypes.c_int32), ("waves_per_simd", ctypes.c_int32),
-                ("grid_blocks", ctypes.c_int32), ("block_threads", ctypes.c_int32)]
+                ("grid_blocks", ctypes.c_int32), ("block_threads", ctypes.c_int32),
+                ("group_trips", ctypes.c_int64), ("active_group_trips", ctypes.c_int64)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -18,7 +18,8 @@ hipError_t launch_scan_plain(const ScanArgs& a, hipStream_t st);
 hipError_t launch_eval_plain(const ScanArgs& a, int mol, long n, const double* pos, double* out, hipStream_t st);
 hipError_t launch_grid_build(const ScanArgs& a, int mol, const int* n, const double* s, const double* o,
                              double4* out, hipStream_t st);
-hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_waves, hipStream_t st);
+hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_waves, int lds_sites,
+                            hipStream_t st);
 hipError_t launch_to_f32(const double4* in, float4* out, long n, hipStream_t st);
 }  // namespace mafm
 
@@ -470,7 +471,7 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
 
     const long cols_per_mol = (long)d->nx * d->ny;
     const long total_cols = cols_per_mol * n_mols;
-    int G = 1, wps = 0, n_waves = 0;
+    int G = 1, wps = 0, n_waves = 0, lds_sites = 0;
     if (d->precision != MAFM_PREC_PLAIN64) {
         wps = d->waves_per_simd > 0 ? d->waves_per_simd : 2;
         const long lanes = (long)c->cus * 4 * wps * 64;
@@ -484,6 +485,14 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
         }
         if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16 && G != 32 && G != 64)
             return fail(c, MAFM_ERR_INVALID, "lanes_per_pixel must be a power of two <= 64");
+        // G > 1 stages the largest molecule's sites in one wave's LDS
+        const size_t site_bytes = d->precision == MAFM_PREC_FAST32 ? sizeof(Site32) : sizeof(Site64);
+        if (G > 1 && (size_t)max_sites * site_bytes > 64 * 1024) {
+            if (d->lanes_per_pixel > 0)
+                return fail(c, MAFM_ERR_INVALID, "lanes_per_pixel > 1 needs the sites of one molecule to fit 64 KiB of LDS");
+            G = 1;
+        }
+        lds_sites = (int)max_sites;
         const long groups_per_wave = 64 / G;
         const long need_waves = (total_cols + groups_per_wave - 1) / groups_per_wave;
         n_waves = (int)std::min<long>((long)c->cus * 4 * wps, need_waves);
@@ -510,8 +519,8 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
     if (d->precision == MAFM_PREC_PLAIN64) {
         HIPCHK(c, launch_scan_plain(a, c->stream));
     } else {
-        const int comp = d->compensation == 0 ? 1 : (d->compensation == 1 ? 1 : 0);
-        HIPCHK(c, launch_scan_fast(a, d->precision == MAFM_PREC_FAST32, comp, G, n_waves, c->stream));
+        const int comp = d->compensation == 2 ? 0 : 1;
+        HIPCHK(c, launch_scan_fast(a, d->precision == MAFM_PREC_FAST32, comp, G, n_waves, lds_sites, c->stream));
     }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     int64_t counters[8];
@@ -535,6 +544,8 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
         // algorithmic pair evaluations: (steps * evals + one getOutput eval) per position
         stats->pair_evals = n_mols ? (counters[0] * evals_per_step + (int64_t)per_mol * n_mols) * (pe / n_mols) : 0;
         stats->lanes_per_pixel = G;
+        stats->group_trips = counters[4];
+        stats->active_group_trips = counters[5];
         stats->waves_per_simd = wps;
         stats->grid_blocks = d->precision == MAFM_PREC_PLAIN64 ? (int)((total_cols + 255) / 256) : n_waves;
         stats->block_threads = d->precision == MAFM_PREC_PLAIN64 ? 256 : 64;

@@ -89,8 +89,8 @@ struct ScanArgs {
 enum { REC_UNCONVERGED = 1, REC_GRID_CLAMP = 2, REC_OUT_OF_GRID = 4 };
 
 // Matrix3::multiply for a row-major 3x3 (matrices.hpp:146-153).
-__device__ __host__ inline void mat_vec(const double* m, double x, double y, double z,
-                                        double& ox, double& oy, double& oz) {
+template <typename P>
+__device__ __host__ inline void mat_vec(P m, double x, double y, double z, double& ox, double& oy, double& oz) {
     ox = m[0] * x + m[1] * y + m[2] * z;
     oy = m[3] * x + m[4] * y + m[5] * z;
     oz = m[6] * x + m[7] * y + m[8] * z;

@@ -25,6 +25,8 @@
 //   * Pair terms: F32 = FP32 with hardware reciprocal / reciprocal square root
 //     and Kahan-compensated accumulation (COMP == 1) or plain FP32 sums
 //     (COMP == 0); F64 = FP64 with Newton-refined hardware reciprocals.
+#include <type_traits>
+
 #include "device_common.hpp"
 
 namespace mafm {
@@ -45,6 +47,42 @@ __device__ inline double rsq64(double x) {
     y = y * fma(-hx * y, y, 1.5);
     return y;
 }
+
+// Read-only tables are read through the constant address space: with a
+// wave-uniform index the compiler then emits scalar loads into SGPRs (a plain
+// global pointer would get per-lane vector loads, because the kernel also
+// stores records and the compiler cannot prove the tables unaliased).
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+
+template <typename T>
+__device__ inline cptr<T> as_const(const T* p) {
+    return (cptr<T>)(p);
+}
+
+// Site source of an evaluation: G == 1 streams the molecule's table with
+// scalar loads (every lane needs the same site), G > 1 reads the copy staged
+// in the wave's LDS (the G lanes of a pixel need G different sites).
+template <typename T>
+__device__ inline T load_const(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "dword-sized records only");
+    T out;
+    cptr<uint32_t> src = (cptr<uint32_t>)(p);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = src[i];
+    return out;
+}
+
+template <int G, typename S>
+struct Sites {
+    const S* g;        // global table of the molecule
+    const S* l;        // LDS copy (G > 1)
+    __device__ inline S operator[](int t) const {
+        if (G == 1) return load_const(g + t);
+        return l[t];
+    }
+};
 
 template <bool COMP>
 struct Acc32 {
@@ -69,20 +107,20 @@ struct Acc32 {
 // Sum of all tip-surface site terms of molecule M at tip position (px,py,pz),
 // sites [gl, gl+G, ...] of each segment.
 template <int G, int COMP>
-__device__ void site_sums32(const ScanArgs& a, const DevMol& M, double px, double py, double pz, int gl,
+__device__ void site_sums32(cptr<DevMol> M, Sites<G, Site32> s, double px, double py, double pz, int gl,
                             double out[4]) {
-    const float tx = (float)(px - M.origin[0]);
-    const float ty = (float)(py - M.origin[1]);
-    const float tz = (float)(pz - M.origin[2]);
+    const float tx = (float)(px - M->origin[0]);
+    const float ty = (float)(py - M->origin[1]);
+    const float tz = (float)(pz - M->origin[2]);
     Acc32<COMP == 1> acc;
     acc.init();
-    const Site32* s = a.seg32 + M.seg_off;
     // LJ + Coulomb
     {
-        const int n = M.n_seg[0];
+        const int n = M->n_seg[0];
 #pragma unroll 2
         for (int t = gl; t < n; t += G) {
-            const float4 sa = s[t].a, sb = s[t].b;
+            const Site32 st = s[t];
+            const float4 sa = st.a, sb = st.b;
             float dx = tx - sa.x, dy = ty - sa.y, dz = tz - sa.z;
             float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
             float ir2 = __builtin_amdgcn_rcpf(r2);
@@ -97,15 +135,17 @@ __device__ void site_sums32(const ScanArgs& a, const DevMol& M, double px, doubl
             acc.add(2, fs * dz);
             acc.add(3, (ta - tb) + ec);
         }
-        s += n;
+        s.g += n;
+        s.l += n;
     }
     // LJ
     {
-        const int n = M.n_seg[1];
+        const int n = M->n_seg[1];
 #pragma unroll 2
         for (int t = gl; t < n; t += G) {
-            const float4 sa = s[t].a;
-            const float es12 = s[t].b.x;
+            const Site32 st = s[t];
+            const float4 sa = st.a;
+            const float es12 = st.b.x;
             float dx = tx - sa.x, dy = ty - sa.y, dz = tz - sa.z;
             float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
             float ir2 = __builtin_amdgcn_rcpf(r2);
@@ -118,13 +158,15 @@ __device__ void site_sums32(const ScanArgs& a, const DevMol& M, double px, doubl
             acc.add(2, fs * dz);
             acc.add(3, ta - tb);
         }
-        s += n;
+        s.g += n;
+        s.l += n;
     }
     // Morse (+ Coulomb): E = De (x^2 - 2x + 1), |F| = 2 De a (x^2 - x), x = exp(-a (r - re))
     for (int seg = 2; seg < 4; ++seg) {
-        const int n = M.n_seg[seg];
+        const int n = M->n_seg[seg];
         for (int t = gl; t < n; t += G) {
-            const float4 sa = s[t].a, sb = s[t].b;
+            const Site32 st = s[t];
+            const float4 sa = st.a, sb = st.b;
             float dx = tx - sa.x, dy = ty - sa.y, dz = tz - sa.z;
             float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
             float ir = __builtin_amdgcn_rsqf(r2);
@@ -142,18 +184,18 @@ __device__ void site_sums32(const ScanArgs& a, const DevMol& M, double px, doubl
             acc.add(2, fs * dz);
             acc.add(3, em);
         }
-        s += n;
+        s.g += n;
+        s.l += n;
     }
     for (int q = 0; q < 4; ++q) out[q] = acc.get(q);
 }
 
 template <int G>
-__device__ void site_sums64(const ScanArgs& a, const DevMol& M, double tx, double ty, double tz, int gl,
+__device__ void site_sums64(cptr<DevMol> M, Sites<G, Site64> s, double tx, double ty, double tz, int gl,
                             double out[4]) {
     double fx = 0, fy = 0, fz = 0, e = 0;
-    const Site64* s = a.seg64 + M.seg_off;
     {
-        const int n = M.n_seg[0];
+        const int n = M->n_seg[0];
 #pragma unroll 2
         for (int t = gl; t < n; t += G) {
             const Site64 st = s[t];
@@ -169,10 +211,11 @@ __device__ void site_sums64(const ScanArgs& a, const DevMol& M, double tx, doubl
             fx = fma(fs, dx, fx); fy = fma(fs, dy, fy); fz = fma(fs, dz, fz);
             e += (ta - tb) + ec;
         }
-        s += n;
+        s.g += n;
+        s.l += n;
     }
     {
-        const int n = M.n_seg[1];
+        const int n = M->n_seg[1];
 #pragma unroll 2
         for (int t = gl; t < n; t += G) {
             const Site64 st = s[t];
@@ -186,10 +229,11 @@ __device__ void site_sums64(const ScanArgs& a, const DevMol& M, double tx, doubl
             fx = fma(fs, dx, fx); fy = fma(fs, dy, fy); fz = fma(fs, dz, fz);
             e += ta - tb;
         }
-        s += n;
+        s.g += n;
+        s.l += n;
     }
     for (int seg = 2; seg < 4; ++seg) {
-        const int n = M.n_seg[seg];
+        const int n = M->n_seg[seg];
         for (int t = gl; t < n; t += G) {
             const Site64 st = s[t];
             double dx = tx - st.x, dy = ty - st.y, dz = tz - st.z;
@@ -207,7 +251,8 @@ __device__ void site_sums64(const ScanArgs& a, const DevMol& M, double tx, doubl
             fx = fma(fs, dx, fx); fy = fma(fs, dy, fy); fz = fma(fs, dz, fz);
             e += em;
         }
-        s += n;
+        s.g += n;
+        s.l += n;
     }
     out[0] = fx; out[1] = fy; out[2] = fz; out[3] = e;
 }
@@ -273,34 +318,34 @@ __device__ void grid_lookup(const DevGrid& g, double px, double py, double pz, d
 
 // Tip-dummy VDW, Coulomb and lateral spring on the tip (simulation.cpp:503-515,
 // interactions.cpp:15-51, 252-266), FP64.
-__device__ void tip_dummy(const DevMol& M, double dx_, double dy_, double dz_, double tx, double ty, double tz,
+__device__ void tip_dummy(cptr<DevMol> M, double dx_, double dy_, double dz_, double tx, double ty, double tz,
                           double out[4]) {
     const double rx = dx_ - tx, ry = dy_ - ty, rz = dz_ - tz;
     const double r2 = rx * rx + ry * ry + rz * rz;
     double s, e;
-    if (M.td_vdw == 0) {
+    if (M->td_vdw == 0) {
         double r6 = r2 * r2 * r2;
-        double ta = M.td_p1 / (r6 * r6);
-        double tb = M.td_p0 / r6;
+        double ta = M->td_p1 / (r6 * r6);
+        double tb = M->td_p0 / r6;
         e = ta - tb;
         s = (12 * ta - 6 * tb) / r2;
     } else {
         double r = sqrt(r2);
-        double x = exp(-M.td_p1 * (r - M.td_p2));
-        e = M.td_p0 * (x * x - 2 * x + 1);
-        s = 2 * M.td_p0 * M.td_p1 * (x * x - x) / r;
+        double x = exp(-M->td_p1 * (r - M->td_p2));
+        e = M->td_p0 * (x * x - 2 * x + 1);
+        s = 2 * M->td_p0 * M->td_p1 * (x * x - x) / r;
     }
-    if (M.td_coul) {
+    if (M->td_coul) {
         double r = sqrt(r2);
-        e += M.td_qq / r;
-        s += M.td_qq / (r * r * r);
+        e += M->td_qq / r;
+        s += M->td_qq / (r * r * r);
     }
     double fx = -s * rx, fy = -s * ry, fz = -s * rz;
     const double rl = sqrt(rx * rx + ry * ry);
-    const double dr = rl - M.td_r0;
-    e += M.td_k * dr * dr;
+    const double dr = rl - M->td_r0;
+    e += M->td_k * dr * dr;
     if (rl > 1e-10) {
-        const double h = -2 * M.td_k * dr / rl;
+        const double h = -2 * M->td_k * dr / rl;
         fx -= h * rx;
         fy -= h * ry;
     }
@@ -343,6 +388,23 @@ __global__ void __launch_bounds__(64) k_scan_fast(ScanArgs a, int n_waves) {
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
     int m = __builtin_amdgcn_readfirstlane(pick_molecule(a, blockIdx.x, n_waves));
+    extern __shared__ char lds_raw[];
+    using S = typename std::conditional<F32, Site32, Site64>::type;
+    const S* seg_all = F32 ? (const S*)a.seg32 : (const S*)a.seg64;
+    S* lds_sites = (S*)lds_raw;
+    // G > 1: copy molecule m's site segments into this wave's LDS
+    auto stage = [&](int mol) {
+        if (G > 1) {
+            cptr<DevMol> Ms = as_const(a.mols) + mol;
+            const int n = Ms->n_seg[0] + Ms->n_seg[1] + Ms->n_seg[2] + Ms->n_seg[3];
+            const S* src = seg_all + Ms->seg_off;
+            __syncthreads();
+            for (int t = lane; t < n; t += 64) lds_sites[t] = src[t];
+            __syncthreads();
+        }
+    };
+    stage(m);
+    int64_t trips = 0, active = 0;
 
     // group state (identical on the G lanes of a group)
     int col = -1, k = 0, ph = 0, n_tot = 1, nnn = 0, flags = 0, i = 0, j = 0;
@@ -372,23 +434,23 @@ __global__ void __launch_bounds__(64) k_scan_fast(ScanArgs a, int n_waves) {
             col = gcol;
             if (fresh || (need && col >= 0)) {
                 // new pixel column: System copy, setDummyXY, setDummyZ (simulation.cpp:93-101)
-                const DevMol& M = a.mols[m];
+                cptr<DevMol> M = as_const(a.mols) + m;
                 i = col / a.ny;
                 j = col - i * a.ny;
                 const double x = i * a.dx, y = j * a.dy;
                 double px = x, py = y;
-                if (M.tip_pbc) {
+                if (M->tip_pbc) {
                     double qx, qy, qz;
-                    mat_vec(M.cell_inv, x - M.offset[0], y - M.offset[1], 0 - M.offset[2], qx, qy, qz);
+                    mat_vec(M->cell_inv, x - M->offset[0], y - M->offset[1], 0 - M->offset[2], qx, qy, qz);
                     qx -= floor(qx);
                     qy -= floor(qy);
                     double ox, oy, oz;
-                    mat_vec(M.cell, qx, qy, qz, ox, oy, oz);
-                    px = ox + M.offset[0];
-                    py = oy + M.offset[1];
+                    mat_vec(M->cell, qx, qy, qz, ox, oy, oz);
+                    px = ox + M->offset[0];
+                    py = oy + M->offset[1];
                 }
                 dmx = px; dmy = py; dmz = a.zhigh;
-                tx = px; ty = py; tz = a.zhigh - M.tip_dummy_d;
+                tx = px; ty = py; tz = a.zhigh - M->tip_dummy_d;
                 vx = vy = vz = 0;
                 fx = fy = fz = 0;
                 e = 0;
@@ -411,22 +473,30 @@ __global__ void __launch_bounds__(64) k_scan_fast(ScanArgs a, int n_waves) {
             if (nm < 0) break;
             m = nm;
             col = -1;
+            stage(m);
             continue;
         }
 
         // ---- one evaluation of the interaction list at P for every group
-        const DevMol& M = a.mols[m];
+        ++trips;
+        active += col >= 0;
+        cptr<DevMol> M = as_const(a.mols) + m;
         double r[4];
-        if (M.force_grid >= 0) {
+        if (M->force_grid >= 0) {
             r[0] = r[1] = r[2] = r[3] = 0;
-            grid_lookup<F32>(a.grids[M.force_grid], Px, Py, Pz, r, flags);
+            grid_lookup<F32>(a.grids[M->force_grid], Px, Py, Pz, r, flags);
         } else {
-            if (F32) site_sums32<G, COMP>(a, M, Px, Py, Pz, gl, r);
-            else site_sums64<G>(a, M, Px, Py, Pz, gl, r);
+            if constexpr (F32) {
+                Sites<G, Site32> src{a.seg32 + M->seg_off, (const Site32*)lds_sites};
+                site_sums32<G, COMP>(M, src, Px, Py, Pz, gl, r);
+            } else {
+                Sites<G, Site64> src{a.seg64 + M->seg_off, (const Site64*)lds_sites};
+                site_sums64<G>(M, src, Px, Py, Pz, gl, r);
+            }
             if (G > 1) {
                 for (int q = 0; q < 4; ++q) r[q] = group_sum<G>(r[q]);
             }
-            if (M.potential_grid >= 0) grid_lookup<F32>(a.grids[M.potential_grid], Px, Py, Pz, r, flags);
+            if (M->potential_grid >= 0) grid_lookup<F32>(a.grids[M->potential_grid], Px, Py, Pz, r, flags);
         }
         if (!ts) tip_dummy(M, dmx, dmy, dmz, Px, Py, Pz, r);
 
@@ -443,7 +513,7 @@ __global__ void __launch_bounds__(64) k_scan_fast(ScanArgs a, int n_waves) {
                 rec.e = r[3];
                 rec.n_steps = n_tot;
                 rec.flags = flags | (n_tot >= a.maxsteps ? REC_UNCONVERGED : 0);
-                a.rec[M.rec_off + ((long)col * a.nz + k)] = rec;
+                a.rec[M->rec_off + ((long)col * a.nz + k)] = rec;
                 steps += n_tot;
                 unconv += n_tot >= a.maxsteps;
                 warns += (flags & REC_GRID_CLAMP) != 0;
@@ -556,7 +626,11 @@ __global__ void __launch_bounds__(64) k_scan_fast(ScanArgs a, int n_waves) {
         ts = ph == PH_OUT;
         Px = tx; Py = ty; Pz = tz;
     }
+    if (lane == 0) {
+        atomicAdd((unsigned long long*)&a.counters[4], (unsigned long long)(trips * (64 / G)));
+    }
     if (leader) {
+        if (active) atomicAdd((unsigned long long*)&a.counters[5], (unsigned long long)active);
         if (steps) atomicAdd((unsigned long long*)&a.counters[0], (unsigned long long)steps);
         if (unconv) atomicAdd((unsigned long long*)&a.counters[1], (unsigned long long)unconv);
         if (warns) atomicAdd((unsigned long long*)&a.counters[2], (unsigned long long)warns);
@@ -565,30 +639,33 @@ __global__ void __launch_bounds__(64) k_scan_fast(ScanArgs a, int n_waves) {
 }
 
 template <int G, bool F32, int COMP>
-hipError_t launch_one(const ScanArgs& a, int n_waves, hipStream_t st) {
-    hipLaunchKernelGGL((k_scan_fast<G, F32, COMP>), dim3(n_waves), dim3(64), 0, st, a, n_waves);
+hipError_t launch_one(const ScanArgs& a, int n_waves, size_t lds, hipStream_t st) {
+    hipLaunchKernelGGL((k_scan_fast<G, F32, COMP>), dim3(n_waves), dim3(64), G > 1 ? lds : 0, st, a, n_waves);
     return hipGetLastError();
 }
 
 template <bool F32, int COMP>
-hipError_t launch_g(const ScanArgs& a, int G, int n_waves, hipStream_t st) {
+hipError_t launch_g(const ScanArgs& a, int G, int n_waves, size_t lds, hipStream_t st) {
     switch (G) {
-        case 1: return launch_one<1, F32, COMP>(a, n_waves, st);
-        case 2: return launch_one<2, F32, COMP>(a, n_waves, st);
-        case 4: return launch_one<4, F32, COMP>(a, n_waves, st);
-        case 8: return launch_one<8, F32, COMP>(a, n_waves, st);
-        case 16: return launch_one<16, F32, COMP>(a, n_waves, st);
-        case 32: return launch_one<32, F32, COMP>(a, n_waves, st);
-        case 64: return launch_one<64, F32, COMP>(a, n_waves, st);
+        case 1: return launch_one<1, F32, COMP>(a, n_waves, lds, st);
+        case 2: return launch_one<2, F32, COMP>(a, n_waves, lds, st);
+        case 4: return launch_one<4, F32, COMP>(a, n_waves, lds, st);
+        case 8: return launch_one<8, F32, COMP>(a, n_waves, lds, st);
+        case 16: return launch_one<16, F32, COMP>(a, n_waves, lds, st);
+        case 32: return launch_one<32, F32, COMP>(a, n_waves, lds, st);
+        case 64: return launch_one<64, F32, COMP>(a, n_waves, lds, st);
         default: return hipErrorInvalidValue;
     }
 }
 
 }  // namespace
 
-hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_waves, hipStream_t st) {
-    if (!f32) return launch_g<false, 0>(a, G, n_waves, st);
-    return comp ? launch_g<true, 1>(a, G, n_waves, st) : launch_g<true, 0>(a, G, n_waves, st);
+// lds_sites: largest per-molecule site count (sizes the LDS copy for G > 1)
+hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_waves, int lds_sites,
+                            hipStream_t st) {
+    if (!f32) return launch_g<false, 0>(a, G, n_waves, (size_t)lds_sites * sizeof(Site64), st);
+    const size_t lds = (size_t)lds_sites * sizeof(Site32);
+    return comp ? launch_g<true, 1>(a, G, n_waves, lds, st) : launch_g<true, 0>(a, G, n_waves, lds, st);
 }
 
 }  // namespace mafm
