@@ -65,10 +65,10 @@ def run_batch(abi, sims, prec, **kw):
     return abi.batch_run(sims, prec, **kw)
 
 
-def autotune(abi, sims, prec, candidates):
+def autotune(abi, sims, prec, candidates, compensation=0):
     best = None
     for G, wps in candidates:
-        st = run_batch(abi, sims, prec, lanes_per_pixel=G, waves_per_simd=wps)
+        st = run_batch(abi, sims, prec, lanes_per_pixel=G, waves_per_simd=wps, compensation=compensation)
         print(f"# tune G={G} waves/SIMD={wps}: {st['kernel_ms']:.1f} ms", file=sys.stderr, flush=True)
         if best is None or st["kernel_ms"] < best[2]:
             best = (G, wps, st["kernel_ms"])
@@ -121,6 +121,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--compensation", type=int, default=0,
+                    help="FP32 sums: 0 blocks of 8 folded into FP64, 1 Kahan, 2 plain")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (pixel columns would shard "
@@ -145,8 +147,8 @@ def main():
         G, wps = args.lanes, args.waves
         if args.precision != "plain64" and not args.no_tune and (G == 0 or wps == 0):
             cands = [(g, w) for g in ([G] if G else [1, 2, 4]) for w in ([wps] if wps else [1, 2])]
-            G, wps, _ = autotune(abi, sims, args.precision, cands)
-        kw = dict(lanes_per_pixel=G, waves_per_simd=wps)
+            G, wps, _ = autotune(abi, sims, args.precision, cands, args.compensation)
+        kw = dict(lanes_per_pixel=G, waves_per_simd=wps, compensation=args.compensation)
         for _ in range(args.warmup):
             run_batch(abi, sims, args.precision, **kw)
         kms = []
@@ -170,6 +172,8 @@ def main():
                 args.config, args.config), "systems": len(sims), "positions": positions,
                 "precision": args.precision, "lanes_per_pixel": st["lanes_per_pixel"],
                 "waves_per_simd": st["waves_per_simd"], "total_steps": st["total_steps"],
+                "compensation": {1: "kahan", 2: "none"}.get(args.compensation, "block8-fp64"),
+                "group_utilisation": st["active_group_trips"] / max(1, st["group_trips"]),
                 "unconverged": st["unconverged"]},
             "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": FP32_VALU_PEAK / 1e12,
                          "unit": "TFLOP/s", "frac": achieved / FP32_VALU_PEAK, "traffic": None,

@@ -94,7 +94,8 @@ struct mafm_ctx {
     DevBuf<DevGrid> d_grids;
     std::vector<DevMol> mols;
     DevBuf<DevMol> d_mols;
-    DevBuf<Site64> d_ref, d_seg64;
+    DevBuf<Site64> d_ref;
+    DevBuf<SiteF64> d_seg64;
     DevBuf<Site32> d_seg32;
     DevBuf<DevRecord> d_rec;
     DevBuf<int32_t> d_next;
@@ -177,7 +178,7 @@ static ScanArgs base_args(mafm_ctx* c) {
     memset(&a, 0, sizeof a);
     a.mols = c->d_mols.p;
     a.sites_ref = c->d_ref.p;
-    a.seg64 = c->d_seg64.p;
+    a.segf64 = c->d_seg64.p;
     a.seg32 = c->d_seg32.p;
     a.grids = c->d_grids.p;
     a.counters = c->d_counters.p;
@@ -187,7 +188,7 @@ static ScanArgs base_args(mafm_ctx* c) {
 
 // Lower mafm_system_desc tables to the device layout (device_common.hpp).
 static mafm_status build_tables(mafm_ctx* c, int n, const mafm_system_desc* sys, std::vector<DevMol>& mols,
-                                std::vector<Site64>& ref, std::vector<Site64>& seg64,
+                                std::vector<Site64>& ref, std::vector<SiteF64>& seg64,
                                 std::vector<Site32>& seg32, std::vector<int64_t>& cost) {
     const double LOG2E = 1.4426950408889634;
     for (int m = 0; m < n; ++m) {
@@ -242,14 +243,23 @@ static mafm_status build_tables(mafm_ctx* c, int n, const mafm_system_desc* sys,
                 const mafm_site& t = s.sites[k];
                 const int coul = (t.has_coulomb && t.qq != 0.0) ? 1 : 0;
                 if (t.vdw != want_vdw || coul != want_coul) continue;
-                seg64.push_back(Site64{t.x, t.y, t.z, t.p0, t.p1, t.p2, coul ? t.qq : 0.0, t.vdw, coul});
+                // pre-scaled constants (scan_fast.hip, pair32 / pair64)
+                const double qq = coul ? t.qq : 0.0;
+                SiteF64 g;
+                g.x = t.x; g.y = t.y; g.z = t.z; g.qq = qq;
                 Site32 f;
                 f.a = make_float4((float)(t.x - d.origin[0]), (float)(t.y - d.origin[1]),
-                                  (float)(t.z - d.origin[2]), (float)t.p0);
-                if (t.vdw == MAFM_VDW_LJ)
-                    f.b = make_float4((float)t.p1, 0.f, (float)(coul ? t.qq : 0.0), 0.f);
-                else
-                    f.b = make_float4((float)(t.p1 * LOG2E), (float)t.p2, (float)(coul ? t.qq : 0.0), (float)t.p1);
+                                  (float)(t.z - d.origin[2]), 0.f);
+                if (t.vdw == MAFM_VDW_LJ) {
+                    g.p0 = 6 * t.p0; g.p1 = 6 * t.p1; g.p2 = 0; g.p3 = 0;
+                    f.a.w = (float)g.p0;
+                    f.b = make_float4((float)g.p1, 0.f, (float)qq, 0.f);
+                } else {
+                    g.p0 = 6 * t.p0; g.p1 = t.p1; g.p2 = t.p2; g.p3 = 2 * t.p0 * t.p1;
+                    f.a.w = (float)g.p0;
+                    f.b = make_float4((float)(t.p1 * LOG2E), (float)t.p2, (float)qq, (float)g.p3);
+                }
+                seg64.push_back(g);
                 seg32.push_back(f);
                 ++cnt;
             }
@@ -366,7 +376,7 @@ mafm_status mafm_grid_free(mafm_ctx* c, int handle) {
 }
 
 static mafm_status upload_tables(mafm_ctx* c, const std::vector<DevMol>& mols, const std::vector<Site64>& ref,
-                                 const std::vector<Site64>& seg64, const std::vector<Site32>& seg32) {
+                                 const std::vector<SiteF64>& seg64, const std::vector<Site32>& seg32) {
     HIPCHK(c, c->d_mols.reserve(mols.size()));
     HIPCHK(c, c->d_ref.reserve(ref.size()));
     HIPCHK(c, c->d_seg64.reserve(seg64.size()));
@@ -376,7 +386,7 @@ static mafm_status upload_tables(mafm_ctx* c, const std::vector<DevMol>& mols, c
     if (!ref.empty())
         HIPCHK(c, hipMemcpyAsync(c->d_ref.p, ref.data(), ref.size() * sizeof(Site64), hipMemcpyHostToDevice, c->stream));
     if (!seg64.empty())
-        HIPCHK(c, hipMemcpyAsync(c->d_seg64.p, seg64.data(), seg64.size() * sizeof(Site64), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_seg64.p, seg64.data(), seg64.size() * sizeof(SiteF64), hipMemcpyHostToDevice, c->stream));
     if (!seg32.empty())
         HIPCHK(c, hipMemcpyAsync(c->d_seg32.p, seg32.data(), seg32.size() * sizeof(Site32), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -389,7 +399,8 @@ mafm_status mafm_grid_build(mafm_ctx* c, const mafm_system_desc* sys, const mafm
     HIPCHK(c, hipSetDevice(c->device));
     // sample with a one-molecule table, then restore the loaded systems
     std::vector<DevMol> mols;
-    std::vector<Site64> ref, seg64;
+    std::vector<Site64> ref;
+    std::vector<SiteF64> seg64;
     std::vector<Site32> seg32;
     std::vector<int64_t> cost;
     mafm_status s = build_tables(c, 1, sys, mols, ref, seg64, seg32, cost);
@@ -432,7 +443,8 @@ mafm_status mafm_set_systems(mafm_ctx* c, int n, const mafm_system_desc* sys) {
     if (!c || n <= 0 || !sys) return fail(c, MAFM_ERR_INVALID, "need at least one system");
     HIPCHK(c, hipSetDevice(c->device));
     std::vector<DevMol> mols;
-    std::vector<Site64> ref, seg64;
+    std::vector<Site64> ref;
+    std::vector<SiteF64> seg64;
     std::vector<Site32> seg32;
     std::vector<int64_t> cost;
     mafm_status s = build_tables(c, n, sys, mols, ref, seg64, seg32, cost);
@@ -486,7 +498,7 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
         if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16 && G != 32 && G != 64)
             return fail(c, MAFM_ERR_INVALID, "lanes_per_pixel must be a power of two <= 64");
         // G > 1 stages the largest molecule's sites in one wave's LDS
-        const size_t site_bytes = d->precision == MAFM_PREC_FAST32 ? sizeof(Site32) : sizeof(Site64);
+        const size_t site_bytes = d->precision == MAFM_PREC_FAST32 ? sizeof(Site32) : sizeof(SiteF64);
         if (G > 1 && (size_t)max_sites * site_bytes > 64 * 1024) {
             if (d->lanes_per_pixel > 0)
                 return fail(c, MAFM_ERR_INVALID, "lanes_per_pixel > 1 needs the sites of one molecule to fit 64 KiB of LDS");
@@ -519,7 +531,8 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
     if (d->precision == MAFM_PREC_PLAIN64) {
         HIPCHK(c, launch_scan_plain(a, c->stream));
     } else {
-        const int comp = d->compensation == 2 ? 0 : 1;
+        // 0 / 3: block-compensated (default), 1: Kahan, 2: plain FP32
+        const int comp = d->compensation == 1 ? 1 : (d->compensation == 2 ? 0 : 2);
         HIPCHK(c, launch_scan_fast(a, d->precision == MAFM_PREC_FAST32, comp, G, n_waves, lds_sites, c->stream));
     }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));

@@ -5,10 +5,10 @@
 //                         indices so the compiler keeps it in SGPRs
 //   Site64 sites_ref[]    tip-surface sites in the reference's list order
 //                         (VDW(1,i) then Coulomb(1,i), simulation.cpp:297-312)
-//   Site64 seg64[]        the same sites regrouped per molecule into four
+//   SiteF64 segf64[]      the same sites regrouped per molecule into four
 //                         homogeneous segments: LJ+Coulomb, LJ, Morse+Coulomb,
 //                         Morse, so the hot loops carry no per-site branch
-//   Site32 seg32[]        FP32 copy of seg64, coordinates relative to the
+//   Site32 seg32[]        FP32 copy of segf64, coordinates relative to the
 //                         molecule's origin (keeps FP32 rounding of the
 //                         tip-atom separation at ~1e-6 A)
 //   grids                 double4 / float4 {fx, fy, fz, e} per grid point,
@@ -28,9 +28,18 @@ struct Site64 {
     int32_t coul;     // Coulomb term present
 };
 
+// Fast-path FP64 site, constants pre-scaled like Site32 (p0 = 6 es6 | 6 De,
+// p1 = 6 es12 | a, p2 = - | re, p3 = - | 2 De a).
+struct SiteF64 {
+    double x, y, z;
+    double p0, p1, p2;
+    double qq;
+    double p3;
+};
+
 struct Site32 {
-    float4 a;         // x, y, z (relative to DevMol::origin), p0 (es6 | De)
-    float4 b;         // p1 (es12 | a*log2(e)), p2 (- | re), qq, p1 raw (Morse a)
+    float4 a;         // x, y, z (relative to DevMol::origin), 6 es6 | 6 De
+    float4 b;         // 6 es12 | a log2(e), - | re, qq, - | 2 De a
 };
 
 struct DevGrid {
@@ -76,7 +85,7 @@ struct ScanArgs {
     double etol, ftol, dt;
     const DevMol* mols;
     const Site64* sites_ref;
-    const Site64* seg64;
+    const SiteF64* segf64;
     const Site32* seg32;
     const DevGrid* grids;
     DevRecord* rec;

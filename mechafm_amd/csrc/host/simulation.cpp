@@ -729,7 +729,7 @@ mafm_scan_desc Simulation::scanDesc(const mafm_run_opts& r) const {
     d.precision = r.precision;
     d.lanes_per_pixel = r.lanes_per_pixel;
     d.waves_per_simd = r.waves_per_simd;
-    d.compensation = r.compensation == 2 ? 2 : 1;
+    d.compensation = r.compensation;
     return d;
 }
 

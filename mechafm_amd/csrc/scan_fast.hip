@@ -22,9 +22,10 @@
 //   * Minimiser state, convergence tests (including the etol energy
 //     difference), tip-dummy terms and the integrator updates are FP64 and
 //     follow minimiser.cpp:67-129 / integrators.cpp:8-111.
-//   * Pair terms: F32 = FP32 with hardware reciprocal / reciprocal square root
-//     and Kahan-compensated accumulation (COMP == 1) or plain FP32 sums
-//     (COMP == 0); F64 = FP64 with Newton-refined hardware reciprocals.
+//   * Pair terms: F32 = FP32 with one hardware reciprocal or reciprocal
+//     square root per site, summed in FP32 blocks of 8 folded into FP64
+//     (ACC_BLOCK), Kahan-compensated (ACC_KAHAN) or plain (ACC_NONE);
+//     F64 = FP64 with Newton-refined hardware reciprocals.
 #include <type_traits>
 
 #include "device_common.hpp"
@@ -84,15 +85,29 @@ struct Sites {
     }
 };
 
-template <bool COMP>
+// Accumulators of the FP32 pair sums.
+//   ACC_BLOCK  FP32 partial sums over blocks of 8 sites, folded into FP64
+//   ACC_KAHAN  Kahan-compensated FP32 sum, term by term
+//   ACC_NONE   plain FP32 sum
+enum { ACC_NONE = 0, ACC_KAHAN = 1, ACC_BLOCK = 2 };
+
+template <int MODE>
 struct Acc32 {
-    float s[4];
-    float c[4];
+    float s[4], c[4];
+    double d[4];
     __device__ void init() {
-        for (int q = 0; q < 4; ++q) { s[q] = 0.f; c[q] = 0.f; }
+        for (int q = 0; q < 4; ++q) { s[q] = 0.f; c[q] = 0.f; d[q] = 0.0; }
+    }
+    __device__ inline void begin() {
+        if (MODE == ACC_BLOCK)
+            for (int q = 0; q < 4; ++q) s[q] = 0.f;
+    }
+    __device__ inline void end() {
+        if (MODE == ACC_BLOCK)
+            for (int q = 0; q < 4; ++q) d[q] += (double)s[q];
     }
     __device__ inline void add(int q, float x) {
-        if (COMP) {
+        if (MODE == ACC_KAHAN) {
             float y = x - c[q];
             float t = s[q] + y;
             c[q] = (t - s[q]) - y;
@@ -101,160 +116,198 @@ struct Acc32 {
             s[q] += x;
         }
     }
-    __device__ double get(int q) const { return COMP ? (double)s[q] - (double)c[q] : (double)s[q]; }
+    __device__ double get(int q) const {
+        if (MODE == ACC_BLOCK) return d[q];
+        if (MODE == ACC_KAHAN) return (double)s[q] - (double)c[q];
+        return (double)s[q];
+    }
 };
 
-// Sum of all tip-surface site terms of molecule M at tip position (px,py,pz),
-// sites [gl, gl+G, ...] of each segment.
-template <int G, int COMP>
+// One site term in FP32 (LJ: interactions.cpp:15-27, Morse :29-40, Coulomb
+// :42-51).  Site constants are pre-scaled so the LJ force needs no extra
+// multiply: a.w = 6 es6 | 6 De, b.x = 6 es12 | a log2(e), b.y = - | re,
+// b.z = qq, b.w = - | 2 De a.  Returns the force on the tip and 6x the energy.
+//   SEG 0 LJ + Coulomb, 1 LJ, 2 Morse + Coulomb, 3 Morse
+template <int SEG>
+__device__ inline void pair32(const Site32& st, float tx, float ty, float tz, float& fx, float& fy, float& fz,
+                              float& e6) {
+    const float dx = tx - st.a.x, dy = ty - st.a.y, dz = tz - st.a.z;
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    float fs;
+    if (SEG <= 1) {
+        float ir = 0.f, ir2;
+        if (SEG == 0) {
+            ir = __builtin_amdgcn_rsqf(r2);
+            ir2 = ir * ir;
+        } else {
+            ir2 = __builtin_amdgcn_rcpf(r2);
+        }
+        const float ir6 = ir2 * ir2 * ir2;
+        const float tb = st.a.w * ir6;          // 6 es6 / r^6
+        const float ta = st.b.x * ir6 * ir6;    // 6 es12 / r^12
+        const float el = ta - tb;               // 6 E_LJ; force factor (12 a - 6 b)/r^2 = (ta + el)/r^2
+        if (SEG == 0) {
+            const float ec = st.b.z * ir;       // E_Coulomb; force factor E_C / r^2
+            fs = (ta + el + ec) * ir2;
+            e6 = fmaf(6.f, ec, el);
+        } else {
+            fs = (ta + el) * ir2;
+            e6 = el;
+        }
+    } else {
+        const float ir = __builtin_amdgcn_rsqf(r2);
+        const float r = r2 * ir;
+        const float x = exp2f(st.b.x * (st.b.y - r));
+        const float xm = x - 1.f;
+        fs = st.b.w * x * xm * ir;              // 2 De a x (x - 1) / r
+        e6 = st.a.w * xm * xm;                  // 6 De (x - 1)^2
+        if (SEG == 2) {
+            const float ec = st.b.z * ir;
+            fs = fmaf(ec, ir * ir, fs);
+            e6 = fmaf(6.f, ec, e6);
+        }
+    }
+    fx = fs * dx;
+    fy = fs * dy;
+    fz = fs * dz;
+}
+
+// Sites t = gl, gl + G, ... of one segment: full blocks of 8 per lane without
+// a bound test, then the remainder.
+template <int SEG, int G, int MODE>
+__device__ inline void seg_sum32(const Sites<G, Site32>& s, int n, int gl, float tx, float ty, float tz,
+                                 Acc32<MODE>& acc) {
+    const int full = (n / (8 * G)) * (8 * G);
+    for (int t0 = 0; t0 < full; t0 += 8 * G) {
+        acc.begin();
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            float fx, fy, fz, e;
+            pair32<SEG>(s[t0 + u * G + gl], tx, ty, tz, fx, fy, fz, e);
+            acc.add(0, fx); acc.add(1, fy); acc.add(2, fz); acc.add(3, e);
+        }
+        acc.end();
+    }
+    acc.begin();
+    for (int t = full + gl; t < n; t += G) {
+        float fx, fy, fz, e;
+        pair32<SEG>(s[t], tx, ty, tz, fx, fy, fz, e);
+        acc.add(0, fx); acc.add(1, fy); acc.add(2, fz); acc.add(3, e);
+    }
+    acc.end();
+}
+
+// Sum of all tip-surface site terms of molecule M at tip position (px,py,pz):
+// this lane's share (sites gl, gl+G, ... of each segment).  Coordinates are
+// relative to the molecule's origin so FP32 keeps ~1e-6 A resolution.
+template <int G, int MODE>
 __device__ void site_sums32(cptr<DevMol> M, Sites<G, Site32> s, double px, double py, double pz, int gl,
                             double out[4]) {
     const float tx = (float)(px - M->origin[0]);
     const float ty = (float)(py - M->origin[1]);
     const float tz = (float)(pz - M->origin[2]);
-    Acc32<COMP == 1> acc;
+    Acc32<MODE> acc;
     acc.init();
-    // LJ + Coulomb
-    {
-        const int n = M->n_seg[0];
-#pragma unroll 2
-        for (int t = gl; t < n; t += G) {
-            const Site32 st = s[t];
-            const float4 sa = st.a, sb = st.b;
-            float dx = tx - sa.x, dy = ty - sa.y, dz = tz - sa.z;
-            float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            float ir2 = __builtin_amdgcn_rcpf(r2);
-            float ir = __builtin_amdgcn_rsqf(r2);
-            float ir6 = ir2 * ir2 * ir2;
-            float tb = sa.w * ir6;
-            float ta = sb.x * ir6 * ir6;
-            float ec = sb.z * ir;
-            float fs = (fmaf(12.f, ta, -6.f * tb) + ec) * ir2;
-            acc.add(0, fs * dx);
-            acc.add(1, fs * dy);
-            acc.add(2, fs * dz);
-            acc.add(3, (ta - tb) + ec);
+    int n = M->n_seg[0];
+    seg_sum32<0, G, MODE>(s, n, gl, tx, ty, tz, acc);
+    s.g += n; s.l += n;
+    n = M->n_seg[1];
+    seg_sum32<1, G, MODE>(s, n, gl, tx, ty, tz, acc);
+    s.g += n; s.l += n;
+    n = M->n_seg[2];
+    if (n) seg_sum32<2, G, MODE>(s, n, gl, tx, ty, tz, acc);
+    s.g += n; s.l += n;
+    n = M->n_seg[3];
+    if (n) seg_sum32<3, G, MODE>(s, n, gl, tx, ty, tz, acc);
+    out[0] = acc.get(0);
+    out[1] = acc.get(1);
+    out[2] = acc.get(2);
+    out[3] = acc.get(3) * (1.0 / 6.0);
+}
+
+// The FP64 twin of pair32 (same pre-scaled constants, SiteF64::p3 = 2 De a).
+template <int SEG>
+__device__ inline void pair64(const SiteF64& st, double tx, double ty, double tz, double& fx, double& fy,
+                              double& fz, double& e6) {
+    const double dx = tx - st.x, dy = ty - st.y, dz = tz - st.z;
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    double fs;
+    if (SEG <= 1) {
+        double ir = 0, ir2;
+        if (SEG == 0) {
+            ir = rsq64(r2);
+            ir2 = ir * ir;
+        } else {
+            ir2 = rcp64(r2);
         }
-        s.g += n;
-        s.l += n;
-    }
-    // LJ
-    {
-        const int n = M->n_seg[1];
-#pragma unroll 2
-        for (int t = gl; t < n; t += G) {
-            const Site32 st = s[t];
-            const float4 sa = st.a;
-            const float es12 = st.b.x;
-            float dx = tx - sa.x, dy = ty - sa.y, dz = tz - sa.z;
-            float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            float ir2 = __builtin_amdgcn_rcpf(r2);
-            float ir6 = ir2 * ir2 * ir2;
-            float tb = sa.w * ir6;
-            float ta = es12 * ir6 * ir6;
-            float fs = fmaf(12.f, ta, -6.f * tb) * ir2;
-            acc.add(0, fs * dx);
-            acc.add(1, fs * dy);
-            acc.add(2, fs * dz);
-            acc.add(3, ta - tb);
+        const double ir6 = ir2 * ir2 * ir2;
+        const double tb = st.p0 * ir6;
+        const double ta = st.p1 * ir6 * ir6;
+        const double el = ta - tb;
+        if (SEG == 0) {
+            const double ec = st.qq * ir;
+            fs = (ta + el + ec) * ir2;
+            e6 = fma(6.0, ec, el);
+        } else {
+            fs = (ta + el) * ir2;
+            e6 = el;
         }
-        s.g += n;
-        s.l += n;
-    }
-    // Morse (+ Coulomb): E = De (x^2 - 2x + 1), |F| = 2 De a (x^2 - x), x = exp(-a (r - re))
-    for (int seg = 2; seg < 4; ++seg) {
-        const int n = M->n_seg[seg];
-        for (int t = gl; t < n; t += G) {
-            const Site32 st = s[t];
-            const float4 sa = st.a, sb = st.b;
-            float dx = tx - sa.x, dy = ty - sa.y, dz = tz - sa.z;
-            float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            float ir = __builtin_amdgcn_rsqf(r2);
-            float r = r2 * ir;
-            float x = exp2f(sb.x * (sb.y - r));
-            float em = sa.w * (x - 1.f) * (x - 1.f);
-            float fs = 2.f * sa.w * sb.w * x * (x - 1.f) * ir;
-            if (seg == 2) {
-                float ec = sb.z * ir;
-                fs += ec * ir * ir;
-                em += ec;
-            }
-            acc.add(0, fs * dx);
-            acc.add(1, fs * dy);
-            acc.add(2, fs * dz);
-            acc.add(3, em);
+    } else {
+        const double ir = rsq64(r2);
+        const double r = r2 * ir;
+        const double x = exp(st.p1 * (st.p2 - r));
+        const double xm = x - 1.0;
+        fs = st.p3 * x * xm * ir;
+        e6 = st.p0 * xm * xm;
+        if (SEG == 2) {
+            const double ec = st.qq * ir;
+            fs = fma(ec, ir * ir, fs);
+            e6 = fma(6.0, ec, e6);
         }
-        s.g += n;
-        s.l += n;
     }
-    for (int q = 0; q < 4; ++q) out[q] = acc.get(q);
+    fx = fs * dx;
+    fy = fs * dy;
+    fz = fs * dz;
+}
+
+template <int SEG, int G>
+__device__ inline void seg_sum64(const Sites<G, SiteF64>& s, int n, int gl, double tx, double ty, double tz,
+                                 double acc[4]) {
+    const int full = (n / (4 * G)) * (4 * G);
+    for (int t0 = 0; t0 < full; t0 += 4 * G) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            double fx, fy, fz, e;
+            pair64<SEG>(s[t0 + u * G + gl], tx, ty, tz, fx, fy, fz, e);
+            acc[0] += fx; acc[1] += fy; acc[2] += fz; acc[3] += e;
+        }
+    }
+    for (int t = full + gl; t < n; t += G) {
+        double fx, fy, fz, e;
+        pair64<SEG>(s[t], tx, ty, tz, fx, fy, fz, e);
+        acc[0] += fx; acc[1] += fy; acc[2] += fz; acc[3] += e;
+    }
 }
 
 template <int G>
-__device__ void site_sums64(cptr<DevMol> M, Sites<G, Site64> s, double tx, double ty, double tz, int gl,
+__device__ void site_sums64(cptr<DevMol> M, Sites<G, SiteF64> s, double tx, double ty, double tz, int gl,
                             double out[4]) {
-    double fx = 0, fy = 0, fz = 0, e = 0;
-    {
-        const int n = M->n_seg[0];
-#pragma unroll 2
-        for (int t = gl; t < n; t += G) {
-            const Site64 st = s[t];
-            double dx = tx - st.x, dy = ty - st.y, dz = tz - st.z;
-            double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-            double ir2 = rcp64(r2);
-            double ir = rsq64(r2);
-            double ir6 = ir2 * ir2 * ir2;
-            double tb = st.p0 * ir6;
-            double ta = st.p1 * ir6 * ir6;
-            double ec = st.qq * ir;
-            double fs = (fma(12.0, ta, -6.0 * tb) + ec) * ir2;
-            fx = fma(fs, dx, fx); fy = fma(fs, dy, fy); fz = fma(fs, dz, fz);
-            e += (ta - tb) + ec;
-        }
-        s.g += n;
-        s.l += n;
-    }
-    {
-        const int n = M->n_seg[1];
-#pragma unroll 2
-        for (int t = gl; t < n; t += G) {
-            const Site64 st = s[t];
-            double dx = tx - st.x, dy = ty - st.y, dz = tz - st.z;
-            double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-            double ir2 = rcp64(r2);
-            double ir6 = ir2 * ir2 * ir2;
-            double tb = st.p0 * ir6;
-            double ta = st.p1 * ir6 * ir6;
-            double fs = fma(12.0, ta, -6.0 * tb) * ir2;
-            fx = fma(fs, dx, fx); fy = fma(fs, dy, fy); fz = fma(fs, dz, fz);
-            e += ta - tb;
-        }
-        s.g += n;
-        s.l += n;
-    }
-    for (int seg = 2; seg < 4; ++seg) {
-        const int n = M->n_seg[seg];
-        for (int t = gl; t < n; t += G) {
-            const Site64 st = s[t];
-            double dx = tx - st.x, dy = ty - st.y, dz = tz - st.z;
-            double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-            double ir = rsq64(r2);
-            double r = r2 * ir;
-            double x = exp(-st.p1 * (r - st.p2));
-            double em = st.p0 * (x * x - 2 * x + 1);
-            double fs = 2 * st.p0 * st.p1 * (x * x - x) * ir;
-            if (seg == 2) {
-                double ec = st.qq * ir;
-                fs += ec * ir * ir;
-                em += ec;
-            }
-            fx = fma(fs, dx, fx); fy = fma(fs, dy, fy); fz = fma(fs, dz, fz);
-            e += em;
-        }
-        s.g += n;
-        s.l += n;
-    }
-    out[0] = fx; out[1] = fy; out[2] = fz; out[3] = e;
+    double acc[4] = {0, 0, 0, 0};
+    int n = M->n_seg[0];
+    seg_sum64<0, G>(s, n, gl, tx, ty, tz, acc);
+    s.g += n; s.l += n;
+    n = M->n_seg[1];
+    seg_sum64<1, G>(s, n, gl, tx, ty, tz, acc);
+    s.g += n; s.l += n;
+    n = M->n_seg[2];
+    if (n) seg_sum64<2, G>(s, n, gl, tx, ty, tz, acc);
+    s.g += n; s.l += n;
+    n = M->n_seg[3];
+    if (n) seg_sum64<3, G>(s, n, gl, tx, ty, tz, acc);
+    out[0] = acc[0];
+    out[1] = acc[1];
+    out[2] = acc[2];
+    out[3] = acc[3] * (1.0 / 6.0);
 }
 
 // Trilinear lookup of ForceGrid::interpolate (force_grid.cpp:88-155) with the
@@ -389,8 +442,8 @@ __global__ void __launch_bounds__(64) k_scan_fast(ScanArgs a, int n_waves) {
 
     int m = __builtin_amdgcn_readfirstlane(pick_molecule(a, blockIdx.x, n_waves));
     extern __shared__ char lds_raw[];
-    using S = typename std::conditional<F32, Site32, Site64>::type;
-    const S* seg_all = F32 ? (const S*)a.seg32 : (const S*)a.seg64;
+    using S = typename std::conditional<F32, Site32, SiteF64>::type;
+    const S* seg_all = F32 ? (const S*)a.seg32 : (const S*)a.segf64;
     S* lds_sites = (S*)lds_raw;
     // G > 1: copy molecule m's site segments into this wave's LDS
     auto stage = [&](int mol) {
@@ -490,7 +543,7 @@ __global__ void __launch_bounds__(64) k_scan_fast(ScanArgs a, int n_waves) {
                 Sites<G, Site32> src{a.seg32 + M->seg_off, (const Site32*)lds_sites};
                 site_sums32<G, COMP>(M, src, Px, Py, Pz, gl, r);
             } else {
-                Sites<G, Site64> src{a.seg64 + M->seg_off, (const Site64*)lds_sites};
+                Sites<G, SiteF64> src{a.segf64 + M->seg_off, (const SiteF64*)lds_sites};
                 site_sums64<G>(M, src, Px, Py, Pz, gl, r);
             }
             if (G > 1) {
@@ -663,9 +716,11 @@ hipError_t launch_g(const ScanArgs& a, int G, int n_waves, size_t lds, hipStream
 // lds_sites: largest per-molecule site count (sizes the LDS copy for G > 1)
 hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_waves, int lds_sites,
                             hipStream_t st) {
-    if (!f32) return launch_g<false, 0>(a, G, n_waves, (size_t)lds_sites * sizeof(Site64), st);
+    if (!f32) return launch_g<false, 0>(a, G, n_waves, (size_t)lds_sites * sizeof(SiteF64), st);
     const size_t lds = (size_t)lds_sites * sizeof(Site32);
-    return comp ? launch_g<true, 1>(a, G, n_waves, lds, st) : launch_g<true, 0>(a, G, n_waves, lds, st);
+    if (comp == ACC_KAHAN) return launch_g<true, ACC_KAHAN>(a, G, n_waves, lds, st);
+    if (comp == ACC_NONE) return launch_g<true, ACC_NONE>(a, G, n_waves, lds, st);
+    return launch_g<true, ACC_BLOCK>(a, G, n_waves, lds, st);
 }
 
 }  // namespace mafm
