@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -21,6 +22,18 @@ hipError_t launch_grid_build(const ScanArgs& a, int mol, const int* n, const dou
 hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_waves, int lds_sites,
                             hipStream_t st);
 hipError_t launch_to_f32(const double4* in, float4* out, long n, hipStream_t st);
+bool fft_length_supported(int n);
+void fft_twiddles(int n, std::vector<double>& out);
+hipError_t fft3d_device(double2* data, const int* n, const double2* const* tw, bool inv, hipStream_t st);
+hipError_t launch_real_to_complex(const double* in, double2* out, long n, hipStream_t st);
+hipError_t launch_zero2(double2* out, long n, hipStream_t st);
+hipError_t launch_gauss_tip(double2* rho, const int* n, const int* cut, bool ortho, const double* b, double qn,
+                            double w2, hipStream_t st);
+hipError_t launch_mul_vol(double2* pot, const double2* rho, double vol, long n, hipStream_t st);
+hipError_t launch_spectral_grad(const double2* ek, double2* out, const int* n, int c, const double* kb,
+                                hipStream_t st);
+hipError_t launch_extract(const double2* in, double* fe, int slot, double inv_n, long n, int* flag, hipStream_t st);
+hipError_t launch_scale2(double2* x, double s, long n, hipStream_t st);
 }  // namespace mafm
 
 using namespace mafm;
@@ -105,6 +118,8 @@ struct mafm_ctx {
     std::vector<int64_t> mol_cost;
     size_t n_records = 0;
     bool have_systems = false;
+    std::map<int, DevBuf<double2>> twiddles;   // FFT twiddle tables by length
+    DevBuf<int> d_flag;
 };
 
 static mafm_status fail(mafm_ctx* c, mafm_status s, const char* fmt, ...) {
@@ -312,6 +327,8 @@ void mafm_destroy(mafm_ctx* c) {
     c->d_cum.release();
     c->d_counters.release();
     c->d_tmp.release();
+    for (auto& kv : c->twiddles) kv.second.release();
+    c->d_flag.release();
     (void)hipEventDestroy(c->ev0);
     (void)hipEventDestroy(c->ev1);
     (void)hipStreamDestroy(c->stream);
@@ -600,6 +617,140 @@ mafm_status mafm_eval_tip_surface(mafm_ctx* c, int system, size_t n, const doubl
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (counters[3] != 0) return fail(c, MAFM_ERR_OUT_OF_GRID, "tip position past the force grid");
     return MAFM_OK;
+}
+
+static mafm_status twiddles_for(mafm_ctx* c, int n, const double2** out) {
+    auto it = c->twiddles.find(n);
+    if (it == c->twiddles.end()) {
+        std::vector<double> h;
+        fft_twiddles(n, h);
+        DevBuf<double2> b;
+        HIPCHK(c, b.reserve(n));
+        HIPCHK(c, hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        it = c->twiddles.emplace(n, b).first;
+    }
+    *out = it->second.p;
+    return MAFM_OK;
+}
+
+static mafm_status check_fft_dims(mafm_ctx* c, const int32_t* n) {
+    for (int a = 0; a < 3; ++a)
+        if (!fft_length_supported(n[a]))
+            return fail(c, MAFM_ERR_INVALID, "FFT length %d unsupported (1..2048, prime factors <= 13)", n[a]);
+    return MAFM_OK;
+}
+
+mafm_status mafm_fft3d(mafm_ctx* c, const int32_t n[3], int inverse, const double* in, double* out) {
+    if (!c || !n || !in || !out) return MAFM_ERR_INVALID;
+    mafm_status s = check_fft_dims(c, n);
+    if (s != MAFM_OK) return s;
+    HIPCHK(c, hipSetDevice(c->device));
+    const long N = (long)n[0] * n[1] * n[2];
+    const double2* tw[3];
+    for (int a = 0; a < 3; ++a)
+        if ((s = twiddles_for(c, n[a], &tw[a])) != MAFM_OK) return s;
+    DevBuf<double2> d;
+    HIPCHK(c, d.reserve(N));
+    HIPCHK(c, hipMemcpyAsync(d.p, in, N * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    int nn[3] = {n[0], n[1], n[2]};
+    HIPCHK(c, fft3d_device(d.p, nn, tw, inverse != 0, c->stream));
+    if (inverse) HIPCHK(c, launch_scale2(d.p, 1.0 / (double)N, N, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out, d.p, N * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    d.release();
+    return MAFM_OK;
+}
+
+mafm_status mafm_potential_grid(mafm_ctx* c, const mafm_potential_desc* pd, int* handle) {
+    if (!c || !pd || !handle || !pd->potential) return MAFM_ERR_INVALID;
+    mafm_status s = check_fft_dims(c, pd->n);
+    if (s != MAFM_OK) return s;
+    if (!(pd->gaussian_width > 0)) return fail(c, MAFM_ERR_INVALID, "gaussian width must be positive");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int n[3] = {pd->n[0], pd->n[1], pd->n[2]};
+    const long N = (long)n[0] * n[1] * n[2];
+    const double* B = pd->basis;   // row-major, columns = voxel vectors
+    const bool ortho = is_diag(B);
+    // cutoff of the Gaussian per axis (interactions.cpp:71-95)
+    const double w = pd->gaussian_width, w2 = std::pow(w, 2);
+    const double norm = 1.0 / std::pow(w * std::sqrt(2.0 * 3.14159265358979323846), 3);
+    const int n_min = std::min(std::min(n[0], n[1]), n[2]);
+    int cut[3];
+    for (int a = 0; a < 3; ++a) {
+        cut[a] = n_min;
+        for (int i = 0; i < n_min - 1; ++i) {
+            const double x = i * B[a], y = i * B[3 + a], z = i * B[6 + a];
+            const double r2 = std::pow(x, 2) + std::pow(y, 2) + std::pow(z, 2);
+            if (std::exp(-0.5 * r2 / w2) < 1.0e-10) {
+                cut[a] = i;
+                break;
+            }
+        }
+    }
+    // k-space basis (fft.cpp:41-47): (basis * diag(n))^-1, transposed
+    double bn[9], inv[9], kb[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) bn[3 * i + j] = B[3 * i + j] * n[j];
+    inv3(bn, inv);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) kb[3 * i + j] = inv[3 * j + i];
+    const double vol = B[0] * (B[4] * B[8] - B[5] * B[7]) + B[1] * (B[5] * B[6] - B[3] * B[8]) +
+                       B[2] * (B[3] * B[7] - B[4] * B[6]);
+    const double2* tw[3];
+    for (int a = 0; a < 3; ++a)
+        if ((s = twiddles_for(c, n[a], &tw[a])) != MAFM_OK) return s;
+    DevBuf<double2> pot, rho, tmp;
+    DevBuf<double> real;
+    HIPCHK(c, pot.reserve(N));
+    HIPCHK(c, rho.reserve(N));
+    HIPCHK(c, tmp.reserve(N));
+    HIPCHK(c, real.reserve(N));
+    HIPCHK(c, c->d_flag.reserve(1));
+    HIPCHK(c, hipMemsetAsync(c->d_flag.p, 0, sizeof(int), c->stream));
+    HIPCHK(c, hipMemcpyAsync(real.p, pd->potential, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, launch_real_to_complex(real.p, pot.p, N, c->stream));
+    HIPCHK(c, launch_zero2(rho.p, N, c->stream));
+    HIPCHK(c, launch_gauss_tip(rho.p, n, cut, ortho, B, pd->tip_charge * norm, w2, c->stream));
+    HIPCHK(c, fft3d_device(pot.p, n, tw, false, c->stream));
+    HIPCHK(c, fft3d_device(rho.p, n, tw, false, c->stream));
+    HIPCHK(c, launch_mul_vol(pot.p, rho.p, vol, N, c->stream));   // pot = E_k
+    HostGrid h;
+    mafm_grid_desc gd{};
+    for (int a = 0; a < 3; ++a) {
+        gd.n[a] = n[a];
+        gd.offset[a] = pd->origin[a];
+    }
+    gd.periodic = 1;
+    memcpy(gd.basis, B, sizeof gd.basis);
+    s = new_grid(c, &gd, h);
+    if (s != MAFM_OK) return s;
+    double* fe = reinterpret_cast<double*>(h.fe64);
+    const double inv_n = 1.0 / (double)N;
+    HIPCHK(c, hipMemcpyAsync(tmp.p, pot.p, N * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, fft3d_device(tmp.p, n, tw, true, c->stream));
+    HIPCHK(c, launch_extract(tmp.p, fe, 3, inv_n, N, c->d_flag.p, c->stream));
+    for (int comp = 0; comp < 3; ++comp) {
+        HIPCHK(c, launch_spectral_grad(pot.p, tmp.p, n, comp, kb, c->stream));
+        HIPCHK(c, fft3d_device(tmp.p, n, tw, true, c->stream));
+        HIPCHK(c, launch_extract(tmp.p, fe, comp, inv_n, N, c->d_flag.p, c->stream));
+    }
+    HIPCHK(c, launch_to_f32(h.fe64, h.fe32, N, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    int flag = 0;
+    HIPCHK(c, hipMemcpyAsync(&flag, c->d_flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    pot.release();
+    rho.release();
+    tmp.release();
+    real.release();
+    if (flag) {
+        (void)hipFree(h.fe64);
+        (void)hipFree(h.fe32);
+        return fail(c, MAFM_ERR_FFT_IMAG, "Imaginary part of inverse FFT exceeded threshold 1.0e-8!");
+    }
+    *handle = store_grid(c, h);
+    return sync_grids(c);
 }
 
 }  // extern "C"
