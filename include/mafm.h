@@ -29,6 +29,7 @@
  *   mafm_potential_grid    ElectrostaticPotentialInteraction's constructor
  *                          (interactions.cpp:53-234)
  *   mafm_fft3d             fft_data_grid / ffti_data_grid (fft.hpp:25, 34)
+ *   mafm_adjoint           no counterpart (gradients of an image loss)
  */
 #ifndef MAFM_H
 #define MAFM_H
@@ -179,6 +180,28 @@ typedef struct {
     double gaussian_width;    /* width of the tip's Gaussian charge, 0.5 A in the reference */
 } mafm_potential_desc;
 
+/* Adjoint of the scan (no reference counterpart: MechAFM has no gradients).
+ * For an image loss L(Fz) with g = dL/dFz at every relaxed position, the
+ * implicit function theorem at each relaxed tip (total tip force = 0) gives
+ *   dL/dtheta = sum_pos v . dF_ts/dtheta,  v = g (e_z - w),  J w = grad_p Fz_ts,
+ * J the 3x3 Jacobian of the total tip force.  Returned per site of the
+ * system's table (LJ + Coulomb sites; Morse sites and grids are rejected). */
+typedef struct {
+    int64_t n_positions;
+    const double* tip;        /* (n, 3) relaxed tip positions */
+    const double* dummy;      /* (n, 3) dummy positions */
+    const double* g;          /* (n) dL/dFz */
+    int32_t precision;        /* MAFM_PREC_FAST32 or MAFM_PREC_FAST64 pair arithmetic */
+    int32_t reserved;
+} mafm_adjoint_desc;
+
+typedef struct {
+    double* d_es6;            /* (n_sites) dL/d es6 */
+    double* d_es12;           /* (n_sites) dL/d es12 */
+    double* d_qq;             /* (n_sites) dL/d qq (Coulomb prefactor of the site) */
+    double* d_pos;            /* (n_sites, 3) dL/d site position */
+} mafm_adjoint_out;
+
 /* ---- entry points ------------------------------------------------------ */
 mafm_status mafm_create(int device, mafm_ctx** out);
 void mafm_destroy(mafm_ctx* ctx);
@@ -202,6 +225,7 @@ mafm_status mafm_scan(mafm_ctx* ctx, const mafm_scan_desc* desc, mafm_scan_stats
 /* records of all systems: index ((system*nx + i)*ny + j)*nz + k */
 mafm_status mafm_records(mafm_ctx* ctx, size_t first, size_t count, mafm_record* out);
 mafm_status mafm_eval_tip_surface(mafm_ctx* ctx, int system, size_t n, const double* pos, double* fe);
+mafm_status mafm_adjoint(mafm_ctx* ctx, int system, const mafm_adjoint_desc* desc, mafm_adjoint_out* out);
 
 #ifdef __cplusplus
 }

@@ -54,6 +54,9 @@ mafm_status mafm_sim_initialize(mafm_sim* sim);
 mafm_status mafm_sim_info_get(mafm_sim* sim, mafm_sim_info* info);
 /* the lowered system (valid until the sim is closed) */
 mafm_status mafm_sim_system(mafm_sim* sim, mafm_system_desc* desc);
+/* the scan descriptor this job's options give (for callers that drive
+ * mafm_scan themselves, e.g. with modified site tables) */
+mafm_status mafm_sim_scan_desc(mafm_sim* sim, const mafm_run_opts* opts, mafm_scan_desc* desc);
 mafm_status mafm_sim_run(mafm_sim* sim, const mafm_run_opts* opts, mafm_scan_stats* stats);
 mafm_status mafm_batch_run(int n, mafm_sim* const* sims, const mafm_run_opts* opts, mafm_scan_stats* stats);
 mafm_status mafm_sim_records(mafm_sim* sim, size_t first, size_t count, mafm_record* out);

@@ -83,6 +83,17 @@ class PotentialDesc(ctypes.Structure):
                 ("gaussian_width", ctypes.c_double)]
 
 
+class AdjointDesc(ctypes.Structure):
+    _fields_ = [("n_positions", ctypes.c_int64), ("tip", ctypes.POINTER(ctypes.c_double)),
+                ("dummy", ctypes.POINTER(ctypes.c_double)), ("g", ctypes.POINTER(ctypes.c_double)),
+                ("precision", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class AdjointOut(ctypes.Structure):
+    _fields_ = [("d_es6", ctypes.POINTER(ctypes.c_double)), ("d_es12", ctypes.POINTER(ctypes.c_double)),
+                ("d_qq", ctypes.POINTER(ctypes.c_double)), ("d_pos", ctypes.POINTER(ctypes.c_double))]
+
+
 class SimInfo(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
                 ("n_atoms", ctypes.c_int32), ("n_sites", ctypes.c_int32), ("rigidgrid", ctypes.c_int32),
@@ -116,10 +127,12 @@ SIGNATURES = {
     "mafm_records": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _vp]),
     "mafm_eval_tip_surface": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_size_t, _P(ctypes.c_double),
                                              _P(ctypes.c_double)]),
+    "mafm_adjoint": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     "mafm_sim_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _vp, _P(_vp)]),
     "mafm_sim_initialize": (ctypes.c_int, [_vp]),
     "mafm_sim_info_get": (ctypes.c_int, [_vp, _P(SimInfo)]),
     "mafm_sim_system": (ctypes.c_int, [_vp, _P(SystemDesc)]),
+    "mafm_sim_scan_desc": (ctypes.c_int, [_vp, _P(RunOpts), _P(ScanDesc)]),
     "mafm_sim_run": (ctypes.c_int, [_vp, _P(RunOpts), _P(ScanStats)]),
     "mafm_batch_run": (ctypes.c_int, [ctypes.c_int, _P(_vp), _P(RunOpts), _P(ScanStats)]),
     "mafm_sim_records": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _vp]),
@@ -245,6 +258,19 @@ class Context:
         self.check(lib().mafm_records(self.h, first, count, out.ctypes.data_as(ctypes.c_void_p)))
         return out
 
+    def adjoint(self, system: int, n_sites: int, tip: np.ndarray, dummy: np.ndarray, g: np.ndarray,
+                precision: str = "fast64"):
+        """Site gradients (d_es6, d_es12, d_qq, d_pos) of sum g*Fz by the
+        implicit function theorem at the relaxed tips (mafm_adjoint)."""
+        tip = np.ascontiguousarray(tip, dtype=np.float64)
+        dummy = np.ascontiguousarray(dummy, dtype=np.float64)
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        d = AdjointDesc(len(g), _dp(tip), _dp(dummy), _dp(g), PREC[precision], 0)
+        out = [np.zeros(n_sites), np.zeros(n_sites), np.zeros(n_sites), np.zeros((n_sites, 3))]
+        o = AdjointOut(*[_dp(a) for a in out])
+        self.check(lib().mafm_adjoint(self.h, system, ctypes.byref(d), ctypes.byref(o)))
+        return tuple(out)
+
     def eval_tip_surface(self, system: int, pos: np.ndarray) -> np.ndarray:
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         out = np.zeros((len(pos), 4))
@@ -295,6 +321,13 @@ class Simulation:
     def sites(self) -> List[Site]:
         d = self.system()
         return [d.sites[k] for k in range(d.n_sites)]
+
+    def scan_desc(self, precision: str = "fast64", lanes_per_pixel: int = 0, waves_per_simd: int = 0,
+                  compensation: int = 0) -> ScanDesc:
+        o = RunOpts(PREC[precision], lanes_per_pixel, waves_per_simd, compensation)
+        d = ScanDesc()
+        self.check(lib().mafm_sim_scan_desc(self.h, ctypes.byref(o), ctypes.byref(d)))
+        return d
 
     def run(self, precision: str = "fast64", lanes_per_pixel: int = 0, waves_per_simd: int = 0,
             compensation: int = 0) -> dict:

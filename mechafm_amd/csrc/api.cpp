@@ -34,6 +34,8 @@ hipError_t launch_spectral_grad(const double2* ek, double2* out, const int* n, i
                                 hipStream_t st);
 hipError_t launch_extract(const double2* in, double* fe, int slot, double inv_n, long n, int* flag, hipStream_t st);
 hipError_t launch_scale2(double2* x, double s, long n, hipStream_t st);
+hipError_t launch_adjoint(const ScanArgs& a, int mol, int n_sites, long n, const double* tip, const double* dummy,
+                          const double* g, double* v, double* out6, int* singular, bool f32, hipStream_t st);
 }  // namespace mafm
 
 using namespace mafm;
@@ -751,6 +753,55 @@ mafm_status mafm_potential_grid(mafm_ctx* c, const mafm_potential_desc* pd, int*
     }
     *handle = store_grid(c, h);
     return sync_grids(c);
+}
+
+mafm_status mafm_adjoint(mafm_ctx* c, int system, const mafm_adjoint_desc* d, mafm_adjoint_out* o) {
+    if (!c || !d || !o) return MAFM_ERR_INVALID;
+    if (!c->have_systems) return fail(c, MAFM_ERR_STATE, "adjoint before mafm_set_systems");
+    if (system < 0 || system >= (int)c->mols.size()) return fail(c, MAFM_ERR_INVALID, "system %d", system);
+    const DevMol& m = c->mols[system];
+    if (m.force_grid >= 0 || m.potential_grid >= 0)
+        return fail(c, MAFM_ERR_INVALID, "the adjoint covers pair sites only (no force / potential grids)");
+    if (m.n_seg[2] + m.n_seg[3] > 0) return fail(c, MAFM_ERR_INVALID, "the adjoint covers LJ (+ Coulomb) sites only");
+    const long n = (long)d->n_positions;
+    if (n < 0 || (n > 0 && (!d->tip || !d->dummy || !d->g))) return fail(c, MAFM_ERR_INVALID, "adjoint inputs");
+    const int ns = m.n_sites;
+    if (ns > 0 && (!o->d_es6 || !o->d_es12 || !o->d_qq || !o->d_pos)) return fail(c, MAFM_ERR_INVALID, "adjoint outputs");
+    HIPCHK(c, hipSetDevice(c->device));
+    DevBuf<double> buf;
+    HIPCHK(c, buf.reserve((size_t)10 * n + 6 * (size_t)ns + 1));
+    double* tip = buf.p;
+    double* dummy = tip + 3 * n;
+    double* g = dummy + 3 * n;
+    double* v = g + n;
+    double* out6 = v + 3 * n;
+    HIPCHK(c, c->d_flag.reserve(1));
+    HIPCHK(c, hipMemsetAsync(c->d_flag.p, 0, sizeof(int), c->stream));
+    if (n) {
+        HIPCHK(c, hipMemcpyAsync(tip, d->tip, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(dummy, d->dummy, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(g, d->g, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(c, hipMemsetAsync(out6, 0, 6 * (size_t)ns * sizeof(double), c->stream));
+    ScanArgs a = base_args(c);
+    if (n) HIPCHK(c, launch_adjoint(a, system, ns, n, tip, dummy, g, v, out6, c->d_flag.p,
+                                    d->precision == MAFM_PREC_FAST32, c->stream));
+    std::vector<double> h(6 * (size_t)ns);
+    int singular = 0;
+    if (ns) HIPCHK(c, hipMemcpyAsync(h.data(), out6, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&singular, c->d_flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    buf.release();
+    for (int i = 0; i < ns; ++i) {
+        o->d_es6[i] = h[6 * i];
+        o->d_es12[i] = h[6 * i + 1];
+        o->d_qq[i] = h[6 * i + 2];
+        o->d_pos[3 * i] = h[6 * i + 3];
+        o->d_pos[3 * i + 1] = h[6 * i + 4];
+        o->d_pos[3 * i + 2] = h[6 * i + 5];
+    }
+    if (singular) return fail(c, MAFM_ERR_INVALID, "singular tip-force Jacobian at a relaxed position");
+    return MAFM_OK;
 }
 
 }  // extern "C"

@@ -97,6 +97,19 @@ struct ScanArgs {
 
 enum { REC_UNCONVERGED = 1, REC_GRID_CLAMP = 2, REC_OUT_OF_GRID = 4 };
 
+// Read-only tables are read through the constant address space: with a
+// wave-uniform index the compiler then emits scalar loads into SGPRs (a plain
+// global pointer would get per-lane vector loads, because the kernels also
+// store records and the compiler cannot prove the tables unaliased).
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+using cptr_mol = cptr<DevMol>;
+
+template <typename T>
+__device__ inline cptr<T> as_const(const T* p) {
+    return (cptr<T>)(p);
+}
+
 // Matrix3::multiply for a row-major 3x3 (matrices.hpp:146-153).
 template <typename P>
 __device__ __host__ inline void mat_vec(P m, double x, double y, double z, double& ox, double& oy, double& oz) {

@@ -892,6 +892,14 @@ mafm_status mafm_sim_system(mafm_sim* s, mafm_system_desc* d) {
     return MAFM_OK;
 }
 
+mafm_status mafm_sim_scan_desc(mafm_sim* s, const mafm_run_opts* o, mafm_scan_desc* d) {
+    if (!s || !d) return MAFM_ERR_INVALID;
+    mafm_run_opts def{};
+    def.precision = MAFM_PREC_FAST64;
+    *d = s->sim.scanDesc(o ? *o : def);
+    return MAFM_OK;
+}
+
 mafm_status mafm_sim_run(mafm_sim* s, const mafm_run_opts* o, mafm_scan_stats* stats) {
     if (!s) return MAFM_ERR_INVALID;
     mafm_run_opts def{};

@@ -49,18 +49,6 @@ __device__ inline double rsq64(double x) {
     return y;
 }
 
-// Read-only tables are read through the constant address space: with a
-// wave-uniform index the compiler then emits scalar loads into SGPRs (a plain
-// global pointer would get per-lane vector loads, because the kernel also
-// stores records and the compiler cannot prove the tables unaliased).
-template <typename T>
-using cptr = const __attribute__((address_space(4))) T*;
-
-template <typename T>
-__device__ inline cptr<T> as_const(const T* p) {
-    return (cptr<T>)(p);
-}
-
 // Site source of an evaluation: G == 1 streams the molecule's table with
 // scalar loads (every lane needs the same site), G > 1 reads the copy staged
 // in the wave's LDS (the G lanes of a pixel need G different sites).
