@@ -121,6 +121,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run a device-to-device copy of this many bytes (calibrates HBM byte counters)")
     ap.add_argument("--compensation", type=int, default=0,
                     help="FP32 sums: 0 blocks of 8 folded into FP64, 1 Kahan, 2 plain")
     args = ap.parse_args()
@@ -128,6 +130,17 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (pixel columns would shard "
               "across GPUs with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_bytes:
+        import torch
+        n = args.copy_bytes // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 4, "launches": 3}))
+        return
 
     from mechafm_amd import abi
     tmp = tempfile.mkdtemp(prefix="mafm_bench_")
