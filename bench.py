@@ -46,7 +46,11 @@ def workloads(name, tmp):
 
 
 def pair_cost(sim):
-    return sum(PAIR_FLOPS[(s.vdw, int(s.has_coulomb and s.qq != 0.0))] for s in sim.sites())
+    """Algorithmic FLOPs of one evaluation of a system's sites (every
+    periodic image counts: the reference evaluates 9 x N pairs with vdw_pbc)."""
+    d = sim.system()
+    per_image = sum(PAIR_FLOPS[(s.vdw, int(s.has_coulomb and s.qq != 0.0))] for s in sim.sites())
+    return per_image * max(1, d.n_images)
 
 
 def algorithmic_flops(sims, recs, integrator=1):
@@ -68,7 +72,11 @@ def run_batch(abi, sims, prec, **kw):
 def autotune(abi, sims, prec, candidates, compensation=0):
     best = None
     for G, wps in candidates:
-        st = run_batch(abi, sims, prec, lanes_per_pixel=G, waves_per_simd=wps, compensation=compensation)
+        try:
+            st = run_batch(abi, sims, prec, lanes_per_pixel=G, waves_per_simd=wps, compensation=compensation)
+        except abi.MafmError as e:
+            print(f"# tune G={G} waves/SIMD={wps}: skipped ({e})", file=sys.stderr, flush=True)
+            continue
         print(f"# tune G={G} waves/SIMD={wps}: {st['kernel_ms']:.1f} ms", file=sys.stderr, flush=True)
         if best is None or st["kernel_ms"] < best[2]:
             best = (G, wps, st["kernel_ms"])

@@ -93,6 +93,12 @@ typedef struct {
     double tip_dummy_d;     /* initial tip-dummy distance (simulation.cpp:218-225) */
     double cell[9];         /* row-major cell matrix (columns = cell vectors) */
     double offset[3];       /* System offset (system.cpp:103-136) */
+    /* Periodic images (vdw_pbc, simulation.cpp:298-312): when n_images > 0
+     * the site table is evaluated once per shift, shift-major, each site at
+     * its position + shift (the reference's order).  0 = no images. */
+    int32_t n_images;       /* 0..27 */
+    int32_t reserved;
+    const double* image_shifts;   /* n_images x 3 */
 } mafm_system_desc;
 
 /* ---- grids ------------------------------------------------------------- */

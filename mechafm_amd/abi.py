@@ -38,7 +38,9 @@ class SystemDesc(ctypes.Structure):
     _fields_ = [("n_sites", ctypes.c_int32), ("force_grid", ctypes.c_int32),
                 ("sites", ctypes.POINTER(Site)), ("potential_grid", ctypes.c_int32),
                 ("tip_pbc", ctypes.c_int32), ("tip_dummy", TipDummy), ("tip_dummy_d", ctypes.c_double),
-                ("cell", ctypes.c_double * 9), ("offset", ctypes.c_double * 3)]
+                ("cell", ctypes.c_double * 9), ("offset", ctypes.c_double * 3),
+                ("n_images", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("image_shifts", ctypes.POINTER(ctypes.c_double))]
 
 
 class GridDesc(ctypes.Structure):

@@ -97,9 +97,11 @@ __global__ void __launch_bounds__(256) k_adjoint_vec(const ScanArgs a, int mol, 
     const double tx = tip[3 * p], ty = tip[3 * p + 1], tz = tip[3 * p + 2];
     Sym3 Jts = {0, 0, 0, 0, 0, 0};
     const Site64* s = a.sites_ref + M->ref_off;
+    for (int im = 0; im < M->n_images; ++im)
     for (int i = 0; i < M->n_sites; ++i) {
         const Site64 st = s[i];
-        const Real dx = (Real)(tx - st.x), dy = (Real)(ty - st.y), dz = (Real)(tz - st.z);
+        const double ax = st.x + M->shift[im][0], ay = st.y + M->shift[im][1], az = st.z + M->shift[im][2];
+        const Real dx = (Real)(tx - ax), dy = (Real)(ty - ay), dz = (Real)(tz - az);
         const Real r2 = dx * dx + dy * dy + dz * dz;
         Real f, fp, ir2, ir3;
         lj_coul_radial<Real>(r2, (Real)st.p1, (Real)st.p0, (Real)(st.coul ? st.qq : 0.0), f, fp, ir2, ir3);
@@ -140,8 +142,12 @@ __global__ void __launch_bounds__(256) k_adjoint_site(const ScanArgs a, int mol,
     const Site64 st = a.sites_ref[M->ref_off + site];
     const Real A = (Real)st.p1, B = (Real)st.p0, qq = (Real)(st.coul ? st.qq : 0.0);
     double acc[6] = {0, 0, 0, 0, 0, 0};   // d es6, d es12, d qq, d ax, d ay, d az
-    for (long p = threadIdx.x; p < n; p += blockDim.x) {
-        const Real dx = (Real)(tip[3 * p] - st.x), dy = (Real)(tip[3 * p + 1] - st.y), dz = (Real)(tip[3 * p + 2] - st.z);
+    const int n_img = M->n_images;
+    for (long pi = threadIdx.x; pi < n * n_img; pi += blockDim.x) {
+        const long p = pi / n_img;
+        const int im = (int)(pi - p * n_img);
+        const double ax = st.x + M->shift[im][0], ay = st.y + M->shift[im][1], az = st.z + M->shift[im][2];
+        const Real dx = (Real)(tip[3 * p] - ax), dy = (Real)(tip[3 * p + 1] - ay), dz = (Real)(tip[3 * p + 2] - az);
         const Real r2 = dx * dx + dy * dy + dz * dz;
         Real f, fp, ir2, ir3;
         lj_coul_radial<Real>(r2, A, B, qq, f, fp, ir2, ir3);

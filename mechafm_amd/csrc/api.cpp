@@ -111,7 +111,7 @@ struct mafm_ctx {
     DevBuf<DevMol> d_mols;
     DevBuf<Site64> d_ref;
     DevBuf<SiteF64> d_seg64;
-    DevBuf<Site32> d_seg32;
+    DevBuf<float4> d_seg32a, d_seg32b;
     DevBuf<DevRecord> d_rec;
     DevBuf<int32_t> d_next;
     DevBuf<float> d_cum;
@@ -196,7 +196,8 @@ static ScanArgs base_args(mafm_ctx* c) {
     a.mols = c->d_mols.p;
     a.sites_ref = c->d_ref.p;
     a.segf64 = c->d_seg64.p;
-    a.seg32 = c->d_seg32.p;
+    a.seg32a = c->d_seg32a.p;
+    a.seg32b = c->d_seg32b.p;
     a.grids = c->d_grids.p;
     a.counters = c->d_counters.p;
     a.n_mols = (int)c->mols.size();
@@ -235,6 +236,11 @@ static mafm_status build_tables(mafm_ctx* c, int n, const mafm_system_desc* sys,
         memcpy(d.cell, s.cell, sizeof d.cell);
         memcpy(d.offset, s.offset, sizeof d.offset);
         if (s.tip_pbc) inv3(d.cell, d.cell_inv);
+        if (s.n_images < 0 || s.n_images > 27 || (s.n_images > 0 && !s.image_shifts))
+            return fail(c, MAFM_ERR_INVALID, "system %d: n_images must be 0..27 with shifts", m);
+        d.n_images = s.n_images > 0 ? s.n_images : 1;
+        for (int k = 0; k < d.n_images; ++k)
+            for (int q = 0; q < 3; ++q) d.shift[k][q] = s.n_images > 0 ? s.image_shifts[3 * k + q] : 0.0;
         // FP32 coordinate origin: centre of the sites' bounding box
         double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
         for (int k = 0; k < s.n_sites; ++k) {
@@ -245,6 +251,8 @@ static mafm_status build_tables(mafm_ctx* c, int n, const mafm_system_desc* sys,
             }
         }
         for (int q = 0; q < 3; ++q) d.origin[q] = s.n_sites ? 0.5 * (lo[q] + hi[q]) : 0.0;
+        // FP32 coordinates stay relative to the base image; image shifts are
+        // applied to the tip in FP64 before the conversion
         for (int k = 0; k < s.n_sites; ++k) {
             const mafm_site& t = s.sites[k];
             if (t.vdw != MAFM_VDW_LJ && t.vdw != MAFM_VDW_MORSE)
@@ -270,11 +278,11 @@ static mafm_status build_tables(mafm_ctx* c, int n, const mafm_system_desc* sys,
                 if (t.vdw == MAFM_VDW_LJ) {
                     g.p0 = 6 * t.p0; g.p1 = 6 * t.p1; g.p2 = 0; g.p3 = 0;
                     f.a.w = (float)g.p0;
-                    f.b = make_float4((float)g.p1, 0.f, (float)qq, 0.f);
+                    f.b = make_float4((float)g.p1, (float)qq, 0.f, 0.f);
                 } else {
                     g.p0 = 6 * t.p0; g.p1 = t.p1; g.p2 = t.p2; g.p3 = 2 * t.p0 * t.p1;
                     f.a.w = (float)g.p0;
-                    f.b = make_float4((float)(t.p1 * LOG2E), (float)t.p2, (float)qq, (float)g.p3);
+                    f.b = make_float4((float)(t.p1 * LOG2E), (float)qq, (float)t.p2, (float)g.p3);
                 }
                 seg64.push_back(g);
                 seg32.push_back(f);
@@ -283,7 +291,7 @@ static mafm_status build_tables(mafm_ctx* c, int n, const mafm_system_desc* sys,
             d.n_seg[segk] = cnt;
         }
         mols.push_back(d);
-        cost.push_back(s.force_grid >= 0 ? 8 : std::max(1, s.n_sites));
+        cost.push_back(s.force_grid >= 0 ? 8 : std::max(1, s.n_sites * d.n_images));
     }
     return MAFM_OK;
 }
@@ -323,7 +331,8 @@ void mafm_destroy(mafm_ctx* c) {
     c->d_mols.release();
     c->d_ref.release();
     c->d_seg64.release();
-    c->d_seg32.release();
+    c->d_seg32a.release();
+    c->d_seg32b.release();
     c->d_rec.release();
     c->d_next.release();
     c->d_cum.release();
@@ -399,15 +408,24 @@ static mafm_status upload_tables(mafm_ctx* c, const std::vector<DevMol>& mols, c
     HIPCHK(c, c->d_mols.reserve(mols.size()));
     HIPCHK(c, c->d_ref.reserve(ref.size()));
     HIPCHK(c, c->d_seg64.reserve(seg64.size()));
-    HIPCHK(c, c->d_seg32.reserve(seg32.size()));
+    HIPCHK(c, c->d_seg32a.reserve(seg32.size()));
+    HIPCHK(c, c->d_seg32b.reserve(seg32.size()));
     if (!mols.empty())
         HIPCHK(c, hipMemcpyAsync(c->d_mols.p, mols.data(), mols.size() * sizeof(DevMol), hipMemcpyHostToDevice, c->stream));
     if (!ref.empty())
         HIPCHK(c, hipMemcpyAsync(c->d_ref.p, ref.data(), ref.size() * sizeof(Site64), hipMemcpyHostToDevice, c->stream));
     if (!seg64.empty())
         HIPCHK(c, hipMemcpyAsync(c->d_seg64.p, seg64.data(), seg64.size() * sizeof(SiteF64), hipMemcpyHostToDevice, c->stream));
-    if (!seg32.empty())
-        HIPCHK(c, hipMemcpyAsync(c->d_seg32.p, seg32.data(), seg32.size() * sizeof(Site32), hipMemcpyHostToDevice, c->stream));
+    if (!seg32.empty()) {
+        std::vector<float4> sa(seg32.size()), sb(seg32.size());
+        for (size_t k = 0; k < seg32.size(); ++k) {
+            sa[k] = seg32[k].a;
+            sb[k] = seg32[k].b;
+        }
+        HIPCHK(c, hipMemcpyAsync(c->d_seg32a.p, sa.data(), sa.size() * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_seg32b.p, sb.data(), sb.size() * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MAFM_OK;
 }
@@ -570,7 +588,7 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
         int64_t evals_per_step = d->minimiser == MAFM_MIN_SD ? 1 : (d->integrator == 1 ? 2 : (d->integrator == 2 ? 4 : 1));
         int64_t pe = 0;
         for (int m = 0; m < n_mols; ++m) {
-            int64_t sites = c->mols[m].force_grid >= 0 ? 0 : c->mols[m].n_sites;
+            int64_t sites = c->mols[m].force_grid >= 0 ? 0 : (int64_t)c->mols[m].n_sites * c->mols[m].n_images;
             pe += sites;
         }
         // algorithmic pair evaluations: (steps * evals + one getOutput eval) per position

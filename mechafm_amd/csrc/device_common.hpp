@@ -8,7 +8,7 @@
 //   SiteF64 segf64[]      the same sites regrouped per molecule into four
 //                         homogeneous segments: LJ+Coulomb, LJ, Morse+Coulomb,
 //                         Morse, so the hot loops carry no per-site branch
-//   Site32 seg32[]        FP32 copy of segf64, coordinates relative to the
+//   float4 seg32a[], seg32b[]  FP32 copy of segf64 as two arrays, coordinates relative to the
 //                         molecule's origin (keeps FP32 rounding of the
 //                         tip-atom separation at ~1e-6 A)
 //   grids                 double4 / float4 {fx, fy, fz, e} per grid point,
@@ -39,7 +39,7 @@ struct SiteF64 {
 
 struct Site32 {
     float4 a;         // x, y, z (relative to DevMol::origin), 6 es6 | 6 De
-    float4 b;         // 6 es12 | a log2(e), - | re, qq, - | 2 De a
+    float4 b;         // 6 es12 | a log2(e), qq, - | re, - | 2 De a
 };
 
 struct DevGrid {
@@ -66,6 +66,9 @@ struct DevMol {
     double cell[9], cell_inv[9], offset[3];
     double origin[3];
     int64_t rec_off;                       // first record of this molecule
+    int32_t n_images;                      // >= 1; image 0 of a non-periodic system is a zero shift
+    int32_t pad;
+    double shift[27][3];                   // periodic image shifts, evaluated shift-major
 };
 
 struct DevRecord {
@@ -86,7 +89,8 @@ struct ScanArgs {
     const DevMol* mols;
     const Site64* sites_ref;
     const SiteF64* segf64;
-    const Site32* seg32;
+    const float4* seg32a;     // FP32 sites: {x, y, z, 6 es6 | 6 De}
+    const float4* seg32b;     //             {6 es12 | a log2(e), qq, - | re, - | 2 De a}
     const DevGrid* grids;
     DevRecord* rec;
     // work-stealing state (FAST kernels)

@@ -131,6 +131,8 @@ void LJInteraction::lower(Lowering& out) const {
         s.vdw = MAFM_VDW_LJ;
         s.p0 = es6_; s.p1 = es12_;
         out.sites.push_back(s);
+        out.site_atom.push_back(i2_);
+        out.site_shift.push_back(shift_);
     } else if (i1_ == 0 && i2_ == 1) {
         out.tip_dummy.vdw = MAFM_VDW_LJ;
         out.tip_dummy.p0 = es6_; out.tip_dummy.p1 = es12_; out.tip_dummy.p2 = 0;
@@ -148,6 +150,8 @@ void MorseInteraction::lower(Lowering& out) const {
         s.vdw = MAFM_VDW_MORSE;
         s.p0 = de_; s.p1 = a_; s.p2 = re_;
         out.sites.push_back(s);
+        out.site_atom.push_back(i2_);
+        out.site_shift.push_back(shift_);
     } else if (i1_ == 0 && i2_ == 1) {
         out.tip_dummy.vdw = MAFM_VDW_MORSE;
         out.tip_dummy.p0 = de_; out.tip_dummy.p1 = a_; out.tip_dummy.p2 = re_;
@@ -172,6 +176,39 @@ void CoulombInteraction::lower(Lowering& out) const {
     } else {
         fail(MAFM_ERR_INVALID, "surface-surface interactions need flexible mode, which is not built");
     }
+}
+
+void factorImages(Lowering& low) {
+    const size_t n = low.sites.size();
+    std::vector<Vec3d> shifts;
+    for (const Vec3d& v : low.site_shift) {
+        bool seen = false;
+        for (const Vec3d& u : shifts) seen |= (u == v);
+        if (!seen) shifts.push_back(v);
+    }
+    if (shifts.size() <= 1 || n % shifts.size() != 0) return;
+    const size_t block = n / shifts.size();
+    for (size_t k = 0; k < n; ++k) {
+        if (!(low.site_shift[k] == shifts[k / block]) || low.site_atom[k] != low.site_atom[k % block]) return;
+        if (low.sites[k].has_coulomb) return;
+    }
+    std::vector<mafm_site> base(low.sites.begin(), low.sites.begin() + (long)block);
+    for (size_t k = 0; k < block; ++k) {
+        const Vec3d& p = (*low.positions)[low.site_atom[k]];
+        base[k].x = p.x;
+        base[k].y = p.y;
+        base[k].z = p.z;
+    }
+    low.sites.swap(base);
+    low.site_atom.resize(block);
+    low.site_shift.assign(block, Vec3d());
+    low.image_shifts.clear();
+    for (const Vec3d& v : shifts) {
+        low.image_shifts.push_back(v.x);
+        low.image_shifts.push_back(v.y);
+        low.image_shifts.push_back(v.z);
+    }
+    low.n_images = (int)shifts.size();
 }
 
 void TipHarmonicInteraction::lower(Lowering& out) const {
@@ -632,9 +669,12 @@ void Simulation::buildTipGridInteractions() {     // simulation.cpp:420-501
     Lowering low;
     low.positions = &system.positions_;
     for (const auto& it : interactions_) it->lower(low);
+    factorImages(low);
     mafm_system_desc sd{};
     sd.n_sites = (int32_t)low.sites.size();
     sd.sites = low.sites.data();
+    sd.n_images = low.n_images;
+    sd.image_shifts = low.image_shifts.empty() ? nullptr : low.image_shifts.data();
     sd.force_grid = -1;
     sd.potential_grid = low.potential_grid;
     sd.tip_pbc = system.tipPbc() ? 1 : 0;
@@ -699,9 +739,12 @@ void Simulation::lowerSystem() {
     low_.positions = &system.positions_;
     for (const auto& it : interactions_) it->lower(low_);
     if (!low_.tip_dummy_vdw_set) fail(MAFM_ERR_STATE, "no tip-dummy VDW term in the interaction list");
+    factorImages(low_);
     desc_ = mafm_system_desc{};
     desc_.n_sites = (int32_t)low_.sites.size();
     desc_.sites = low_.sites.data();
+    desc_.n_images = low_.n_images;
+    desc_.image_shifts = low_.image_shifts.empty() ? nullptr : low_.image_shifts.data();
     desc_.force_grid = low_.force_grid;
     desc_.potential_grid = low_.potential_grid;
     desc_.tip_pbc = (system.tipPbc() && !options_.rigidgrid) ? 1 : 0;   // setTipPbc(false), simulation.cpp:98-99

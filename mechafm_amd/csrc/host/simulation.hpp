@@ -111,12 +111,21 @@ inline double mixeps(double e1, double e2) { return std::sqrt(e1 * e2); }
 // Device-table builder that Interaction::lower() writes into.
 struct Lowering {
     std::vector<mafm_site> sites;
+    std::vector<int> site_atom;        // surface atom index of each site
+    std::vector<Vec3d> site_shift;     // periodic image shift of each site
+    std::vector<double> image_shifts;  // factored shifts (n_images x 3) after factorImages()
+    int n_images = 0;
     mafm_tip_dummy tip_dummy{};
     bool tip_dummy_vdw_set = false;
     int force_grid = -1;
     int potential_grid = -1;
     const std::vector<Vec3d>* positions = nullptr;
 };
+
+// Rewrite a site list of the form "every atom for shift 1, every atom for
+// shift 2, ..." (vdw_pbc, simulation.cpp:298-312) as one block of atoms plus
+// the list of shifts, so the device streams each atom once per image.
+void factorImages(Lowering& low);
 
 // Interaction (interactions.hpp:68-76): immutable after build; lowers itself
 // into the device tables.  atom indices: 0 dummy, 1 tip, >= 2 surface.

@@ -26,8 +26,6 @@
 //     square root per site, summed in FP32 blocks of 8 folded into FP64
 //     (ACC_BLOCK), Kahan-compensated (ACC_KAHAN) or plain (ACC_NONE);
 //     F64 = FP64 with Newton-refined hardware reciprocals.
-#include <type_traits>
-
 #include "device_common.hpp"
 
 namespace mafm {
@@ -70,6 +68,32 @@ struct Sites {
     __device__ inline S operator[](int t) const {
         if (G == 1) return load_const(g + t);
         return l[t];
+    }
+    __device__ inline void advance(int n) {
+        g += n;
+        l += n;
+    }
+};
+
+// FP32 sites as two arrays so a pair reads only what it needs: A = {x, y, z,
+// 6 es6 | 6 De} (16 B) and B = {6 es12 | a log2(e), qq, - | re, - | 2 De a};
+// an LJ pair reads 20 B, LJ + Coulomb 24 B, Morse 32 B.
+template <int G>
+struct Sites32 {
+    const float4* ga;
+    const float4* gb;
+    const float4* la;
+    const float4* lb;
+    __device__ inline float4 A(int t) const { return G == 1 ? load_const(ga + t) : la[t]; }
+    __device__ inline float B1(int t) const {
+        return G == 1 ? load_const(reinterpret_cast<const float*>(gb + t)) : reinterpret_cast<const float*>(lb + t)[0];
+    }
+    __device__ inline float2 B2(int t) const {
+        return G == 1 ? load_const(reinterpret_cast<const float2*>(gb + t)) : reinterpret_cast<const float2*>(lb + t)[0];
+    }
+    __device__ inline float4 B4(int t) const { return G == 1 ? load_const(gb + t) : lb[t]; }
+    __device__ inline void advance(int n) {
+        ga += n; gb += n; la += n; lb += n;
     }
 };
 
@@ -116,26 +140,31 @@ struct Acc32 {
 // multiply: a.w = 6 es6 | 6 De, b.x = 6 es12 | a log2(e), b.y = - | re,
 // b.z = qq, b.w = - | 2 De a.  Returns the force on the tip and 6x the energy.
 //   SEG 0 LJ + Coulomb, 1 LJ, 2 Morse + Coulomb, 3 Morse
-template <int SEG>
-__device__ inline void pair32(const Site32& st, float tx, float ty, float tz, float& fx, float& fy, float& fz,
-                              float& e6) {
-    const float dx = tx - st.a.x, dy = ty - st.a.y, dz = tz - st.a.z;
+template <int SEG, int G>
+__device__ inline void pair32(const Sites32<G>& s, int t, float tx, float ty, float tz, float& fx, float& fy,
+                              float& fz, float& e6) {
+    const float4 a = s.A(t);
+    const float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     float fs;
     if (SEG <= 1) {
-        float ir = 0.f, ir2;
+        float ir = 0.f, ir2, es12, qq = 0.f;
         if (SEG == 0) {
+            const float2 b = s.B2(t);
+            es12 = b.x;
+            qq = b.y;
             ir = __builtin_amdgcn_rsqf(r2);
             ir2 = ir * ir;
         } else {
+            es12 = s.B1(t);
             ir2 = __builtin_amdgcn_rcpf(r2);
         }
         const float ir6 = ir2 * ir2 * ir2;
-        const float tb = st.a.w * ir6;          // 6 es6 / r^6
-        const float ta = st.b.x * ir6 * ir6;    // 6 es12 / r^12
+        const float tb = a.w * ir6;             // 6 es6 / r^6
+        const float ta = es12 * ir6 * ir6;      // 6 es12 / r^12
         const float el = ta - tb;               // 6 E_LJ; force factor (12 a - 6 b)/r^2 = (ta + el)/r^2
         if (SEG == 0) {
-            const float ec = st.b.z * ir;       // E_Coulomb; force factor E_C / r^2
+            const float ec = qq * ir;           // E_Coulomb; force factor E_C / r^2
             fs = (ta + el + ec) * ir2;
             e6 = fmaf(6.f, ec, el);
         } else {
@@ -143,14 +172,15 @@ __device__ inline void pair32(const Site32& st, float tx, float ty, float tz, fl
             e6 = el;
         }
     } else {
+        const float4 b = s.B4(t);
         const float ir = __builtin_amdgcn_rsqf(r2);
         const float r = r2 * ir;
-        const float x = exp2f(st.b.x * (st.b.y - r));
+        const float x = exp2f(b.x * (b.z - r));
         const float xm = x - 1.f;
-        fs = st.b.w * x * xm * ir;              // 2 De a x (x - 1) / r
-        e6 = st.a.w * xm * xm;                  // 6 De (x - 1)^2
+        fs = b.w * x * xm * ir;                 // 2 De a x (x - 1) / r
+        e6 = a.w * xm * xm;                     // 6 De (x - 1)^2
         if (SEG == 2) {
-            const float ec = st.b.z * ir;
+            const float ec = b.y * ir;
             fs = fmaf(ec, ir * ir, fs);
             e6 = fmaf(6.f, ec, e6);
         }
@@ -163,7 +193,7 @@ __device__ inline void pair32(const Site32& st, float tx, float ty, float tz, fl
 // Sites t = gl, gl + G, ... of one segment: full blocks of 8 per lane without
 // a bound test, then the remainder.
 template <int SEG, int G, int MODE>
-__device__ inline void seg_sum32(const Sites<G, Site32>& s, int n, int gl, float tx, float ty, float tz,
+__device__ inline void seg_sum32(const Sites32<G>& s, int n, int gl, float tx, float ty, float tz,
                                  Acc32<MODE>& acc) {
     const int full = (n / (8 * G)) * (8 * G);
     for (int t0 = 0; t0 < full; t0 += 8 * G) {
@@ -171,7 +201,7 @@ __device__ inline void seg_sum32(const Sites<G, Site32>& s, int n, int gl, float
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             float fx, fy, fz, e;
-            pair32<SEG>(s[t0 + u * G + gl], tx, ty, tz, fx, fy, fz, e);
+            pair32<SEG, G>(s, t0 + u * G + gl, tx, ty, tz, fx, fy, fz, e);
             acc.add(0, fx); acc.add(1, fy); acc.add(2, fz); acc.add(3, e);
         }
         acc.end();
@@ -179,7 +209,7 @@ __device__ inline void seg_sum32(const Sites<G, Site32>& s, int n, int gl, float
     acc.begin();
     for (int t = full + gl; t < n; t += G) {
         float fx, fy, fz, e;
-        pair32<SEG>(s[t], tx, ty, tz, fx, fy, fz, e);
+        pair32<SEG, G>(s, t, tx, ty, tz, fx, fy, fz, e);
         acc.add(0, fx); acc.add(1, fy); acc.add(2, fz); acc.add(3, e);
     }
     acc.end();
@@ -189,24 +219,29 @@ __device__ inline void seg_sum32(const Sites<G, Site32>& s, int n, int gl, float
 // this lane's share (sites gl, gl+G, ... of each segment).  Coordinates are
 // relative to the molecule's origin so FP32 keeps ~1e-6 A resolution.
 template <int G, int MODE>
-__device__ void site_sums32(cptr<DevMol> M, Sites<G, Site32> s, double px, double py, double pz, int gl,
+__device__ void site_sums32(cptr<DevMol> M, const Sites32<G> s0, double px, double py, double pz, int gl,
                             double out[4]) {
-    const float tx = (float)(px - M->origin[0]);
-    const float ty = (float)(py - M->origin[1]);
-    const float tz = (float)(pz - M->origin[2]);
     Acc32<MODE> acc;
     acc.init();
-    int n = M->n_seg[0];
-    seg_sum32<0, G, MODE>(s, n, gl, tx, ty, tz, acc);
-    s.g += n; s.l += n;
-    n = M->n_seg[1];
-    seg_sum32<1, G, MODE>(s, n, gl, tx, ty, tz, acc);
-    s.g += n; s.l += n;
-    n = M->n_seg[2];
-    if (n) seg_sum32<2, G, MODE>(s, n, gl, tx, ty, tz, acc);
-    s.g += n; s.l += n;
-    n = M->n_seg[3];
-    if (n) seg_sum32<3, G, MODE>(s, n, gl, tx, ty, tz, acc);
+    const int n_img = M->n_images;
+    for (int im = 0; im < n_img; ++im) {
+        // periodic image: shift the tip instead of the sites (FP64, then FP32 relative to the origin)
+        const float tx = (float)(px - M->shift[im][0] - M->origin[0]);
+        const float ty = (float)(py - M->shift[im][1] - M->origin[1]);
+        const float tz = (float)(pz - M->shift[im][2] - M->origin[2]);
+        Sites32<G> s = s0;
+        int n = M->n_seg[0];
+        seg_sum32<0, G, MODE>(s, n, gl, tx, ty, tz, acc);
+        s.advance(n);
+        n = M->n_seg[1];
+        seg_sum32<1, G, MODE>(s, n, gl, tx, ty, tz, acc);
+        s.advance(n);
+        n = M->n_seg[2];
+        if (n) seg_sum32<2, G, MODE>(s, n, gl, tx, ty, tz, acc);
+        s.advance(n);
+        n = M->n_seg[3];
+        if (n) seg_sum32<3, G, MODE>(s, n, gl, tx, ty, tz, acc);
+    }
     out[0] = acc.get(0);
     out[1] = acc.get(1);
     out[2] = acc.get(2);
@@ -278,20 +313,25 @@ __device__ inline void seg_sum64(const Sites<G, SiteF64>& s, int n, int gl, doub
 }
 
 template <int G>
-__device__ void site_sums64(cptr<DevMol> M, Sites<G, SiteF64> s, double tx, double ty, double tz, int gl,
+__device__ void site_sums64(cptr<DevMol> M, const Sites<G, SiteF64> s0, double px, double py, double pz, int gl,
                             double out[4]) {
     double acc[4] = {0, 0, 0, 0};
-    int n = M->n_seg[0];
-    seg_sum64<0, G>(s, n, gl, tx, ty, tz, acc);
-    s.g += n; s.l += n;
-    n = M->n_seg[1];
-    seg_sum64<1, G>(s, n, gl, tx, ty, tz, acc);
-    s.g += n; s.l += n;
-    n = M->n_seg[2];
-    if (n) seg_sum64<2, G>(s, n, gl, tx, ty, tz, acc);
-    s.g += n; s.l += n;
-    n = M->n_seg[3];
-    if (n) seg_sum64<3, G>(s, n, gl, tx, ty, tz, acc);
+    const int n_img = M->n_images;
+    for (int im = 0; im < n_img; ++im) {
+        const double tx = px - M->shift[im][0], ty = py - M->shift[im][1], tz = pz - M->shift[im][2];
+        Sites<G, SiteF64> s = s0;
+        int n = M->n_seg[0];
+        seg_sum64<0, G>(s, n, gl, tx, ty, tz, acc);
+        s.advance(n);
+        n = M->n_seg[1];
+        seg_sum64<1, G>(s, n, gl, tx, ty, tz, acc);
+        s.advance(n);
+        n = M->n_seg[2];
+        if (n) seg_sum64<2, G>(s, n, gl, tx, ty, tz, acc);
+        s.advance(n);
+        n = M->n_seg[3];
+        if (n) seg_sum64<3, G>(s, n, gl, tx, ty, tz, acc);
+    }
     out[0] = acc[0];
     out[1] = acc[1];
     out[2] = acc[2];
@@ -429,19 +469,26 @@ __global__ void __launch_bounds__(64) k_scan_fast(ScanArgs a, int n_waves) {
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
     int m = __builtin_amdgcn_readfirstlane(pick_molecule(a, blockIdx.x, n_waves));
-    extern __shared__ char lds_raw[];
-    using S = typename std::conditional<F32, Site32, SiteF64>::type;
-    const S* seg_all = F32 ? (const S*)a.seg32 : (const S*)a.segf64;
-    S* lds_sites = (S*)lds_raw;
+    extern __shared__ float4 lds_raw[];
+    int lds_n = 0;   // site count of the molecule staged in LDS
     // G > 1: copy molecule m's site segments into this wave's LDS
+    // (FP32: A array then B array; FP64: SiteF64 records)
     auto stage = [&](int mol) {
         if (G > 1) {
             cptr<DevMol> Ms = as_const(a.mols) + mol;
             const int n = Ms->n_seg[0] + Ms->n_seg[1] + Ms->n_seg[2] + Ms->n_seg[3];
-            const S* src = seg_all + Ms->seg_off;
             __syncthreads();
-            for (int t = lane; t < n; t += 64) lds_sites[t] = src[t];
+            if (F32) {
+                for (int t = lane; t < n; t += 64) {
+                    lds_raw[t] = a.seg32a[Ms->seg_off + t];
+                    lds_raw[n + t] = a.seg32b[Ms->seg_off + t];
+                }
+            } else {
+                SiteF64* dst = reinterpret_cast<SiteF64*>(lds_raw);
+                for (int t = lane; t < n; t += 64) dst[t] = a.segf64[Ms->seg_off + t];
+            }
             __syncthreads();
+            lds_n = n;
         }
     };
     stage(m);
@@ -528,10 +575,10 @@ __global__ void __launch_bounds__(64) k_scan_fast(ScanArgs a, int n_waves) {
             grid_lookup<F32>(a.grids[M->force_grid], Px, Py, Pz, r, flags);
         } else {
             if constexpr (F32) {
-                Sites<G, Site32> src{a.seg32 + M->seg_off, (const Site32*)lds_sites};
+                Sites32<G> src{a.seg32a + M->seg_off, a.seg32b + M->seg_off, lds_raw, lds_raw + lds_n};
                 site_sums32<G, COMP>(M, src, Px, Py, Pz, gl, r);
             } else {
-                Sites<G, SiteF64> src{a.segf64 + M->seg_off, (const SiteF64*)lds_sites};
+                Sites<G, SiteF64> src{a.segf64 + M->seg_off, reinterpret_cast<const SiteF64*>(lds_raw)};
                 site_sums64<G>(M, src, Px, Py, Pz, gl, r);
             }
             if (G > 1) {
@@ -705,7 +752,7 @@ hipError_t launch_g(const ScanArgs& a, int G, int n_waves, size_t lds, hipStream
 hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_waves, int lds_sites,
                             hipStream_t st) {
     if (!f32) return launch_g<false, 0>(a, G, n_waves, (size_t)lds_sites * sizeof(SiteF64), st);
-    const size_t lds = (size_t)lds_sites * sizeof(Site32);
+    const size_t lds = (size_t)lds_sites * 2 * sizeof(float4);
     if (comp == ACC_KAHAN) return launch_g<true, ACC_KAHAN>(a, G, n_waves, lds, st);
     if (comp == ACC_NONE) return launch_g<true, ACC_NONE>(a, G, n_waves, lds, st);
     return launch_g<true, ACC_BLOCK>(a, G, n_waves, lds, st);

@@ -144,9 +144,12 @@ __device__ void eval_tip(Ctx& c, V3 dummy, V3 tip, bool ts_only, V3& F, double& 
         e += ge;
     } else {
         const Site64* s = c.a->sites_ref + m.ref_off;
+        for (int im = 0; im < m.n_images; ++im)
         for (int n = 0; n < m.n_sites; ++n) {
             Site64 st = s[n];
-            V3 r = {tip.x - (st.x + 0.0), tip.y - (st.y + 0.0), tip.z - (st.z + 0.0)};
+            // positions[i2] + pbc_shift, subtracted from the tip (interactions.cpp:16)
+            V3 r = {tip.x - (st.x + m.shift[im][0]), tip.y - (st.y + m.shift[im][1]),
+                    tip.z - (st.z + m.shift[im][2])};
             V3 fv; double ev;
             vdw(st.vdw, st.p0, st.p1, st.p2, r, fv, ev);
             f.x += fv.x; f.y += fv.y; f.z += fv.z;
