@@ -69,11 +69,12 @@ def run_batch(abi, sims, prec, **kw):
     return abi.batch_run(sims, prec, **kw)
 
 
-def autotune(abi, sims, prec, candidates, compensation=0):
+def autotune(abi, sims, prec, candidates, compensation=0, waves_per_group=0):
     best = None
     for G, wps in candidates:
         try:
-            st = run_batch(abi, sims, prec, lanes_per_pixel=G, waves_per_simd=wps, compensation=compensation)
+            st = run_batch(abi, sims, prec, lanes_per_pixel=G, waves_per_simd=wps, compensation=compensation,
+                           waves_per_group=waves_per_group)
         except abi.MafmError as e:
             print(f"# tune G={G} waves/SIMD={wps}: skipped ({e})", file=sys.stderr, flush=True)
             continue
@@ -126,6 +127,7 @@ def main():
     ap.add_argument("--precision", default="fast32", choices=["fast32", "fast64", "plain64"])
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--group", type=int, default=0, help="waves per workgroup (0 = automatic)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
@@ -168,8 +170,9 @@ def main():
         G, wps = args.lanes, args.waves
         if args.precision != "plain64" and not args.no_tune and (G == 0 or wps == 0):
             cands = [(g, w) for g in ([G] if G else [1, 2, 4]) for w in ([wps] if wps else [1, 2])]
-            G, wps, _ = autotune(abi, sims, args.precision, cands, args.compensation)
-        kw = dict(lanes_per_pixel=G, waves_per_simd=wps, compensation=args.compensation)
+            G, wps, _ = autotune(abi, sims, args.precision, cands, args.compensation, args.group)
+        kw = dict(lanes_per_pixel=G, waves_per_simd=wps, compensation=args.compensation,
+                  waves_per_group=args.group)
         for _ in range(args.warmup):
             run_batch(abi, sims, args.precision, **kw)
         kms = []
@@ -192,7 +195,8 @@ def main():
                                           "64x64 px x 40 z each, FIRE+midpoint, one batched launch"}.get(
                 args.config, args.config), "systems": len(sims), "positions": positions,
                 "precision": args.precision, "lanes_per_pixel": st["lanes_per_pixel"],
-                "waves_per_simd": st["waves_per_simd"], "total_steps": st["total_steps"],
+                "waves_per_simd": st["waves_per_simd"], "workgroup_threads": st["block_threads"],
+                "total_steps": st["total_steps"],
                 "compensation": {1: "kahan", 2: "none"}.get(args.compensation, "block8-fp64"),
                 "group_utilisation": st["active_group_trips"] / max(1, st["group_trips"]),
                 "unconverged": st["unconverged"]},

@@ -135,7 +135,7 @@ typedef struct {
     int32_t waves_per_simd;    /* persistent grid size */
     int32_t compensation;      /* FAST32 sums: 0 = 3 = FP32 blocks of 8 sites folded into FP64
                                   (default), 1 = Kahan-compensated FP32, 2 = plain FP32 */
-    int32_t reserved;
+    int32_t waves_per_group;   /* 1, 2, 4 waves per workgroup sharing one LDS site copy */
 } mafm_scan_desc;
 
 /* Per scan position (OutputData, globals.hpp:27-32, minus the derived r and

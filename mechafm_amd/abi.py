@@ -55,7 +55,7 @@ class ScanDesc(ctypes.Structure):
                 ("minterm", ctypes.c_int32), ("etol", ctypes.c_double), ("ftol", ctypes.c_double),
                 ("dt", ctypes.c_double), ("maxsteps", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("lanes_per_pixel", ctypes.c_int32), ("waves_per_simd", ctypes.c_int32),
-                ("compensation", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("compensation", ctypes.c_int32), ("waves_per_group", ctypes.c_int32)]
 
 
 RECORD_DTYPE = np.dtype([("x", "f8"), ("y", "f8"), ("z", "f8"), ("fx", "f8"), ("fy", "f8"), ("fz", "f8"),
@@ -107,7 +107,8 @@ class SimInfo(ctypes.Structure):
 
 class RunOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("lanes_per_pixel", ctypes.c_int32),
-                ("waves_per_simd", ctypes.c_int32), ("compensation", ctypes.c_int32)]
+                ("waves_per_simd", ctypes.c_int32), ("compensation", ctypes.c_int32),
+                ("waves_per_group", ctypes.c_int32)]
 
 
 # every exported entry point and its signature: (restype, argtypes)
@@ -325,15 +326,15 @@ class Simulation:
         return [d.sites[k] for k in range(d.n_sites)]
 
     def scan_desc(self, precision: str = "fast64", lanes_per_pixel: int = 0, waves_per_simd: int = 0,
-                  compensation: int = 0) -> ScanDesc:
-        o = RunOpts(PREC[precision], lanes_per_pixel, waves_per_simd, compensation)
+                  compensation: int = 0, waves_per_group: int = 0) -> ScanDesc:
+        o = RunOpts(PREC[precision], lanes_per_pixel, waves_per_simd, compensation, waves_per_group)
         d = ScanDesc()
         self.check(lib().mafm_sim_scan_desc(self.h, ctypes.byref(o), ctypes.byref(d)))
         return d
 
     def run(self, precision: str = "fast64", lanes_per_pixel: int = 0, waves_per_simd: int = 0,
-            compensation: int = 0) -> dict:
-        o = RunOpts(PREC[precision], lanes_per_pixel, waves_per_simd, compensation)
+            compensation: int = 0, waves_per_group: int = 0) -> dict:
+        o = RunOpts(PREC[precision], lanes_per_pixel, waves_per_simd, compensation, waves_per_group)
         st = ScanStats()
         self.check(lib().mafm_sim_run(self.h, ctypes.byref(o), ctypes.byref(st)))
         return st.as_dict()
@@ -350,9 +351,9 @@ class Simulation:
 
 
 def batch_run(sims: Sequence[Simulation], precision: str = "fast32", lanes_per_pixel: int = 0,
-              waves_per_simd: int = 0, compensation: int = 0) -> dict:
+              waves_per_simd: int = 0, compensation: int = 0, waves_per_group: int = 0) -> dict:
     arr = (ctypes.c_void_p * len(sims))(*[s.h.value for s in sims])
-    o = RunOpts(PREC[precision], lanes_per_pixel, waves_per_simd, compensation)
+    o = RunOpts(PREC[precision], lanes_per_pixel, waves_per_simd, compensation, waves_per_group)
     st = ScanStats()
     rc = lib().mafm_batch_run(len(sims), arr, ctypes.byref(o), ctypes.byref(st))
     if rc != MAFM_OK:

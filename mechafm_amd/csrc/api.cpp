@@ -19,7 +19,7 @@ hipError_t launch_scan_plain(const ScanArgs& a, hipStream_t st);
 hipError_t launch_eval_plain(const ScanArgs& a, int mol, long n, const double* pos, double* out, hipStream_t st);
 hipError_t launch_grid_build(const ScanArgs& a, int mol, const int* n, const double* s, const double* o,
                              double4* out, hipStream_t st);
-hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_waves, int lds_sites,
+hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_blocks, int W, int lds_sites,
                             hipStream_t st);
 hipError_t launch_to_f32(const double4* in, float4* out, long n, hipStream_t st);
 bool fft_length_supported(int n);
@@ -105,6 +105,7 @@ struct mafm_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
     int cus = 0;
+    size_t lds_per_cu = 0;   // LDS bytes per compute unit
     std::vector<HostGrid> grids;
     DevBuf<DevGrid> d_grids;
     std::vector<DevMol> mols;
@@ -239,8 +240,13 @@ static mafm_status build_tables(mafm_ctx* c, int n, const mafm_system_desc* sys,
         if (s.n_images < 0 || s.n_images > 27 || (s.n_images > 0 && !s.image_shifts))
             return fail(c, MAFM_ERR_INVALID, "system %d: n_images must be 0..27 with shifts", m);
         d.n_images = s.n_images > 0 ? s.n_images : 1;
-        for (int k = 0; k < d.n_images; ++k)
-            for (int q = 0; q < 3; ++q) d.shift[k][q] = s.n_images > 0 ? s.image_shifts[3 * k + q] : 0.0;
+        for (int k = 0; k < d.n_images; ++k) {
+            for (int q = 0; q < 3; ++q) {
+                d.shift[k][q] = s.n_images > 0 ? s.image_shifts[3 * k + q] : 0.0;
+                d.shift32[k][q] = (float)d.shift[k][q];
+            }
+            d.shift32[k][3] = 0.f;
+        }
         // FP32 coordinate origin: centre of the sites' bounding box
         double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
         for (int k = 0; k < s.n_sites; ++k) {
@@ -313,8 +319,12 @@ mafm_status mafm_create(int device, mafm_ctx** out) {
         return MAFM_ERR_HIP;
     }
     hipDeviceProp_t p;
-    if (hipGetDeviceProperties(&p, device) == hipSuccess) c->cus = p.multiProcessorCount;
+    if (hipGetDeviceProperties(&p, device) == hipSuccess) {
+        c->cus = p.multiProcessorCount;
+        c->lds_per_cu = p.maxSharedMemoryPerMultiProcessor;
+    }
     if (c->cus <= 0) c->cus = 256;
+    if (c->lds_per_cu < 64 * 1024) c->lds_per_cu = 64 * 1024;
     *out = c;
     return MAFM_OK;
 }
@@ -520,7 +530,7 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
 
     const long cols_per_mol = (long)d->nx * d->ny;
     const long total_cols = cols_per_mol * n_mols;
-    int G = 1, wps = 0, n_waves = 0, lds_sites = 0;
+    int G = 1, wps = 0, n_waves = 0, lds_sites = 0, W = 1, n_blocks = 0;
     if (d->precision != MAFM_PREC_PLAIN64) {
         wps = d->waves_per_simd > 0 ? d->waves_per_simd : 2;
         const long lanes = (long)c->cus * 4 * wps * 64;
@@ -542,10 +552,23 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
             G = 1;
         }
         lds_sites = (int)max_sites;
+        // waves per workgroup: G > 1 workgroups share one LDS copy of the sites;
+        // grow W until the LDS a CU holds admits wps waves per SIMD
+        if (d->waves_per_group > 0) {
+            W = d->waves_per_group;
+            if (W != 1 && W != 2 && W != 4)
+                return fail(c, MAFM_ERR_INVALID, "waves_per_group must be 1, 2 or 4");
+        } else if (G > 1) {
+            const size_t lds_bytes = std::max<size_t>((size_t)max_sites * site_bytes, 1);
+            const long blocks_per_cu = (long)(c->lds_per_cu / lds_bytes);
+            while (W < 4 && blocks_per_cu * W < 4L * wps) W <<= 1;
+        }
         const long groups_per_wave = 64 / G;
         const long need_waves = (total_cols + groups_per_wave - 1) / groups_per_wave;
         n_waves = (int)std::min<long>((long)c->cus * 4 * wps, need_waves);
         n_waves = std::max(n_waves, 1);
+        n_blocks = (n_waves + W - 1) / W;
+        n_waves = n_blocks * W;
         // per-molecule column counters and cumulative work for the initial wave placement
         std::vector<int32_t> zero(n_mols, 0);
         std::vector<float> cum(n_mols + 1, 0.f);
@@ -570,7 +593,7 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
     } else {
         // 0 / 3: block-compensated (default), 1: Kahan, 2: plain FP32
         const int comp = d->compensation == 1 ? 1 : (d->compensation == 2 ? 0 : 2);
-        HIPCHK(c, launch_scan_fast(a, d->precision == MAFM_PREC_FAST32, comp, G, n_waves, lds_sites, c->stream));
+        HIPCHK(c, launch_scan_fast(a, d->precision == MAFM_PREC_FAST32, comp, G, n_blocks, W, lds_sites, c->stream));
     }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     int64_t counters[8];
@@ -597,8 +620,8 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
         stats->group_trips = counters[4];
         stats->active_group_trips = counters[5];
         stats->waves_per_simd = wps;
-        stats->grid_blocks = d->precision == MAFM_PREC_PLAIN64 ? (int)((total_cols + 255) / 256) : n_waves;
-        stats->block_threads = d->precision == MAFM_PREC_PLAIN64 ? 256 : 64;
+        stats->grid_blocks = d->precision == MAFM_PREC_PLAIN64 ? (int)((total_cols + 255) / 256) : n_blocks;
+        stats->block_threads = d->precision == MAFM_PREC_PLAIN64 ? 256 : 64 * W;
     }
     if (counters[3] != 0)
         return fail(c, MAFM_ERR_OUT_OF_GRID, "%lld positions needed a force-grid point past the grid",

@@ -69,6 +69,7 @@ struct DevMol {
     int32_t n_images;                      // >= 1; image 0 of a non-periodic system is a zero shift
     int32_t pad;
     double shift[27][3];                   // periodic image shifts, evaluated shift-major
+    float shift32[27][4];                  // the same shifts in FP32 (x, y, z, 0)
 };
 
 struct DevRecord {

@@ -773,6 +773,7 @@ mafm_scan_desc Simulation::scanDesc(const mafm_run_opts& r) const {
     d.lanes_per_pixel = r.lanes_per_pixel;
     d.waves_per_simd = r.waves_per_simd;
     d.compensation = r.compensation;
+    d.waves_per_group = r.waves_per_group;
     return d;
 }
 

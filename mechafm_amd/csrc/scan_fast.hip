@@ -135,36 +135,47 @@ struct Acc32 {
     }
 };
 
+// One site of segment SEG in registers: a = {x, y, z, 6 es6 | 6 De}, b = {6 es12 |
+// a log2(e), qq, - | re, - | 2 De a}; only the B words the segment uses are read.
+template <int SEG, int G>
+__device__ inline void load_site(const Sites32<G>& s, int t, float4& a, float4& b) {
+    a = s.A(t);
+    if (SEG == 1) {
+        b.x = s.B1(t);
+    } else if (SEG == 0) {
+        const float2 v = s.B2(t);
+        b.x = v.x;
+        b.y = v.y;
+    } else {
+        b = s.B4(t);
+    }
+}
+
 // One site term in FP32 (LJ: interactions.cpp:15-27, Morse :29-40, Coulomb
 // :42-51).  Site constants are pre-scaled so the LJ force needs no extra
-// multiply: a.w = 6 es6 | 6 De, b.x = 6 es12 | a log2(e), b.y = - | re,
-// b.z = qq, b.w = - | 2 De a.  Returns the force on the tip and 6x the energy.
-//   SEG 0 LJ + Coulomb, 1 LJ, 2 Morse + Coulomb, 3 Morse
-template <int SEG, int G>
-__device__ inline void pair32(const Sites32<G>& s, int t, float tx, float ty, float tz, float& fx, float& fy,
+// multiply: a.w = 6 es6 | 6 De, b.x = 6 es12 | a log2(e), b.y = qq,
+// b.z = - | re, b.w = - | 2 De a.  Returns the force on the tip and 6x the
+// energy.   SEG 0 LJ + Coulomb, 1 LJ, 2 Morse + Coulomb, 3 Morse
+template <int SEG>
+__device__ inline void pair32(const float4 a, const float4 b, float tx, float ty, float tz, float& fx, float& fy,
                               float& fz, float& e6) {
-    const float4 a = s.A(t);
     const float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     float fs;
     if (SEG <= 1) {
-        float ir = 0.f, ir2, es12, qq = 0.f;
+        float ir = 0.f, ir2;
         if (SEG == 0) {
-            const float2 b = s.B2(t);
-            es12 = b.x;
-            qq = b.y;
             ir = __builtin_amdgcn_rsqf(r2);
             ir2 = ir * ir;
         } else {
-            es12 = s.B1(t);
             ir2 = __builtin_amdgcn_rcpf(r2);
         }
         const float ir6 = ir2 * ir2 * ir2;
         const float tb = a.w * ir6;             // 6 es6 / r^6
-        const float ta = es12 * ir6 * ir6;      // 6 es12 / r^12
+        const float ta = b.x * ir6 * ir6;       // 6 es12 / r^12
         const float el = ta - tb;               // 6 E_LJ; force factor (12 a - 6 b)/r^2 = (ta + el)/r^2
         if (SEG == 0) {
-            const float ec = qq * ir;           // E_Coulomb; force factor E_C / r^2
+            const float ec = b.y * ir;          // E_Coulomb; force factor E_C / r^2
             fs = (ta + el + ec) * ir2;
             e6 = fmaf(6.f, ec, el);
         } else {
@@ -172,7 +183,6 @@ __device__ inline void pair32(const Sites32<G>& s, int t, float tx, float ty, fl
             e6 = el;
         }
     } else {
-        const float4 b = s.B4(t);
         const float ir = __builtin_amdgcn_rsqf(r2);
         const float r = r2 * ir;
         const float x = exp2f(b.x * (b.z - r));
@@ -190,58 +200,92 @@ __device__ inline void pair32(const Sites32<G>& s, int t, float tx, float ty, fl
     fz = fs * dz;
 }
 
-// Sites t = gl, gl + G, ... of one segment: full blocks of 8 per lane without
-// a bound test, then the remainder.
-template <int SEG, int G, int MODE>
-__device__ inline void seg_sum32(const Sites32<G>& s, int n, int gl, float tx, float ty, float tz,
+// Sites t = gl, gl + G, ... of one segment against every periodic image of
+// the tip: full blocks of 8 sites per lane (loaded once, reused for all
+// images, no bound test), then the remainder one site at a time.  The image
+// shifts are wave-uniform (scalar loads); the tip of image im is
+// t - shift[im] in FP32 relative to the molecule's origin.  IMG = false is
+// the single zero-shift image of a non-periodic system.
+template <int SEG, int G, int MODE, bool IMG>
+__device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<G>& s, int n, int gl, float tx, float ty, float tz,
                                  Acc32<MODE>& acc) {
+    const int n_img = IMG ? M->n_images : 1;
     const int full = (n / (8 * G)) * (8 * G);
     for (int t0 = 0; t0 < full; t0 += 8 * G) {
-        acc.begin();
+        float4 sa[8], sb[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < 8; ++u) load_site<SEG, G>(s, t0 + u * G + gl, sa[u], sb[u]);
+        for (int im = 0; im < n_img; ++im) {
+            float ix = tx, iy = ty, iz = tz;
+            if (IMG) {
+                ix -= M->shift32[im][0];
+                iy -= M->shift32[im][1];
+                iz -= M->shift32[im][2];
+            }
+            acc.begin();
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                float fx, fy, fz, e;
+                pair32<SEG>(sa[u], sb[u], ix, iy, iz, fx, fy, fz, e);
+                acc.add(0, fx); acc.add(1, fy); acc.add(2, fz); acc.add(3, e);
+            }
+            acc.end();
+        }
+    }
+    if (!IMG) acc.begin();
+    for (int t = full + gl; t < n; t += G) {
+        float4 sa, sb;
+        load_site<SEG, G>(s, t, sa, sb);
+        if (IMG) acc.begin();
+        for (int im = 0; im < n_img; ++im) {
+            float ix = tx, iy = ty, iz = tz;
+            if (IMG) {
+                ix -= M->shift32[im][0];
+                iy -= M->shift32[im][1];
+                iz -= M->shift32[im][2];
+            }
             float fx, fy, fz, e;
-            pair32<SEG, G>(s, t0 + u * G + gl, tx, ty, tz, fx, fy, fz, e);
+            pair32<SEG>(sa, sb, ix, iy, iz, fx, fy, fz, e);
             acc.add(0, fx); acc.add(1, fy); acc.add(2, fz); acc.add(3, e);
         }
-        acc.end();
+        if (IMG) acc.end();
     }
-    acc.begin();
-    for (int t = full + gl; t < n; t += G) {
-        float fx, fy, fz, e;
-        pair32<SEG, G>(s, t, tx, ty, tz, fx, fy, fz, e);
-        acc.add(0, fx); acc.add(1, fy); acc.add(2, fz); acc.add(3, e);
-    }
-    acc.end();
+    if (!IMG) acc.end();
+}
+
+template <int G, int MODE, bool IMG>
+__device__ inline void seg_sums32(cptr<DevMol> M, const Sites32<G> s0, float tx, float ty, float tz, int gl,
+                                  Acc32<MODE>& acc) {
+    Sites32<G> s = s0;
+    int n = M->n_seg[0];
+    seg_sum32<0, G, MODE, IMG>(M, s, n, gl, tx, ty, tz, acc);
+    s.advance(n);
+    n = M->n_seg[1];
+    seg_sum32<1, G, MODE, IMG>(M, s, n, gl, tx, ty, tz, acc);
+    s.advance(n);
+    n = M->n_seg[2];
+    if (n) seg_sum32<2, G, MODE, IMG>(M, s, n, gl, tx, ty, tz, acc);
+    s.advance(n);
+    n = M->n_seg[3];
+    if (n) seg_sum32<3, G, MODE, IMG>(M, s, n, gl, tx, ty, tz, acc);
 }
 
 // Sum of all tip-surface site terms of molecule M at tip position (px,py,pz):
-// this lane's share (sites gl, gl+G, ... of each segment).  Coordinates are
-// relative to the molecule's origin so FP32 keeps ~1e-6 A resolution.
+// this lane's share (sites gl, gl+G, ... of each segment, every image).
+// Coordinates are relative to the molecule's origin so FP32 keeps ~1e-6 A
+// resolution.
 template <int G, int MODE>
 __device__ void site_sums32(cptr<DevMol> M, const Sites32<G> s0, double px, double py, double pz, int gl,
                             double out[4]) {
     Acc32<MODE> acc;
     acc.init();
-    const int n_img = M->n_images;
-    for (int im = 0; im < n_img; ++im) {
-        // periodic image: shift the tip instead of the sites (FP64, then FP32 relative to the origin)
-        const float tx = (float)(px - M->shift[im][0] - M->origin[0]);
-        const float ty = (float)(py - M->shift[im][1] - M->origin[1]);
-        const float tz = (float)(pz - M->shift[im][2] - M->origin[2]);
-        Sites32<G> s = s0;
-        int n = M->n_seg[0];
-        seg_sum32<0, G, MODE>(s, n, gl, tx, ty, tz, acc);
-        s.advance(n);
-        n = M->n_seg[1];
-        seg_sum32<1, G, MODE>(s, n, gl, tx, ty, tz, acc);
-        s.advance(n);
-        n = M->n_seg[2];
-        if (n) seg_sum32<2, G, MODE>(s, n, gl, tx, ty, tz, acc);
-        s.advance(n);
-        n = M->n_seg[3];
-        if (n) seg_sum32<3, G, MODE>(s, n, gl, tx, ty, tz, acc);
-    }
+    const float tx = (float)(px - M->origin[0]);
+    const float ty = (float)(py - M->origin[1]);
+    const float tz = (float)(pz - M->origin[2]);
+    if (M->n_images > 1)
+        seg_sums32<G, MODE, true>(M, s0, tx, ty, tz, gl, acc);
+    else
+        seg_sums32<G, MODE, false>(M, s0, tx, ty, tz, gl, acc);
     out[0] = acc.get(0);
     out[1] = acc.get(1);
     out[2] = acc.get(2);
@@ -294,21 +338,32 @@ __device__ inline void pair64(const SiteF64& st, double tx, double ty, double tz
 }
 
 template <int SEG, int G>
-__device__ inline void seg_sum64(const Sites<G, SiteF64>& s, int n, int gl, double tx, double ty, double tz,
-                                 double acc[4]) {
+__device__ inline void seg_sum64(cptr<DevMol> M, const Sites<G, SiteF64>& s, int n, int gl, double px, double py,
+                                 double pz, double acc[4]) {
+    const int n_img = M->n_images;
     const int full = (n / (4 * G)) * (4 * G);
     for (int t0 = 0; t0 < full; t0 += 4 * G) {
+        SiteF64 st[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            double fx, fy, fz, e;
-            pair64<SEG>(s[t0 + u * G + gl], tx, ty, tz, fx, fy, fz, e);
-            acc[0] += fx; acc[1] += fy; acc[2] += fz; acc[3] += e;
+        for (int u = 0; u < 4; ++u) st[u] = s[t0 + u * G + gl];
+        for (int im = 0; im < n_img; ++im) {
+            const double tx = px - M->shift[im][0], ty = py - M->shift[im][1], tz = pz - M->shift[im][2];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                double fx, fy, fz, e;
+                pair64<SEG>(st[u], tx, ty, tz, fx, fy, fz, e);
+                acc[0] += fx; acc[1] += fy; acc[2] += fz; acc[3] += e;
+            }
         }
     }
     for (int t = full + gl; t < n; t += G) {
-        double fx, fy, fz, e;
-        pair64<SEG>(s[t], tx, ty, tz, fx, fy, fz, e);
-        acc[0] += fx; acc[1] += fy; acc[2] += fz; acc[3] += e;
+        const SiteF64 st = s[t];
+        for (int im = 0; im < n_img; ++im) {
+            const double tx = px - M->shift[im][0], ty = py - M->shift[im][1], tz = pz - M->shift[im][2];
+            double fx, fy, fz, e;
+            pair64<SEG>(st, tx, ty, tz, fx, fy, fz, e);
+            acc[0] += fx; acc[1] += fy; acc[2] += fz; acc[3] += e;
+        }
     }
 }
 
@@ -316,22 +371,18 @@ template <int G>
 __device__ void site_sums64(cptr<DevMol> M, const Sites<G, SiteF64> s0, double px, double py, double pz, int gl,
                             double out[4]) {
     double acc[4] = {0, 0, 0, 0};
-    const int n_img = M->n_images;
-    for (int im = 0; im < n_img; ++im) {
-        const double tx = px - M->shift[im][0], ty = py - M->shift[im][1], tz = pz - M->shift[im][2];
-        Sites<G, SiteF64> s = s0;
-        int n = M->n_seg[0];
-        seg_sum64<0, G>(s, n, gl, tx, ty, tz, acc);
-        s.advance(n);
-        n = M->n_seg[1];
-        seg_sum64<1, G>(s, n, gl, tx, ty, tz, acc);
-        s.advance(n);
-        n = M->n_seg[2];
-        if (n) seg_sum64<2, G>(s, n, gl, tx, ty, tz, acc);
-        s.advance(n);
-        n = M->n_seg[3];
-        if (n) seg_sum64<3, G>(s, n, gl, tx, ty, tz, acc);
-    }
+    Sites<G, SiteF64> s = s0;
+    int n = M->n_seg[0];
+    seg_sum64<0, G>(M, s, n, gl, px, py, pz, acc);
+    s.advance(n);
+    n = M->n_seg[1];
+    seg_sum64<1, G>(M, s, n, gl, px, py, pz, acc);
+    s.advance(n);
+    n = M->n_seg[2];
+    if (n) seg_sum64<2, G>(M, s, n, gl, px, py, pz, acc);
+    s.advance(n);
+    n = M->n_seg[3];
+    if (n) seg_sum64<3, G>(M, s, n, gl, px, py, pz, acc);
     out[0] = acc[0];
     out[1] = acc[1];
     out[2] = acc[2];
@@ -449,8 +500,8 @@ __device__ inline bool converged(const ScanArgs& a, double fx, double fy, double
     }
 }
 
-__device__ int pick_molecule(const ScanArgs& a, int wave, int n_waves) {
-    const float target = (wave + 0.5f) / n_waves;
+__device__ int pick_molecule(const ScanArgs& a, int block, int n_blocks) {
+    const float target = (block + 0.5f) / n_blocks;
     int lo = 0, hi = a.n_mols - 1;
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
@@ -459,19 +510,36 @@ __device__ int pick_molecule(const ScanArgs& a, int wave, int n_waves) {
     return lo;
 }
 
+// Next molecule with columns left after m, or -1.
+__device__ int next_molecule(const ScanArgs& a, int m, int n_cols) {
+    for (int q = 1; q <= a.n_mols; ++q) {
+        int cand = m + q;
+        if (cand >= a.n_mols) cand -= a.n_mols;
+        if (__atomic_load_n(&a.next_col[cand], __ATOMIC_RELAXED) < n_cols) return cand;
+    }
+    return -1;
+}
+
+// blockDim.x = 64 W (W = 1, 2 or 4 waves).  With G > 1 the W waves of a
+// workgroup share one LDS copy of their molecule's sites (so a large
+// molecule costs its LDS once per W waves, not once per wave); each wave
+// still steals columns on its own and the workgroup moves to the next
+// molecule together once every wave has finished its last column.
 template <int G, bool F32, int COMP>
-__global__ void __launch_bounds__(64) k_scan_fast(ScanArgs a, int n_waves) {
-    const int lane = threadIdx.x;
+__global__ void __launch_bounds__(256) k_scan_fast(ScanArgs a) {
+    const int lane = threadIdx.x & 63;
+    const bool shared_lds = G > 1 && blockDim.x > 64;
     const int gl = lane % G;
     const int gbase = lane - gl;
     const bool leader = gl == 0;
     const int n_cols = a.nx * a.ny;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
-    int m = __builtin_amdgcn_readfirstlane(pick_molecule(a, blockIdx.x, n_waves));
+    int m = __builtin_amdgcn_readfirstlane(pick_molecule(a, blockIdx.x, gridDim.x));
     extern __shared__ float4 lds_raw[];
+    __shared__ int next_shared;
     int lds_n = 0;   // site count of the molecule staged in LDS
-    // G > 1: copy molecule m's site segments into this wave's LDS
+    // G > 1: copy molecule m's site segments into the workgroup's LDS
     // (FP32: A array then B array; FP64: SiteF64 records)
     auto stage = [&](int mol) {
         if (G > 1) {
@@ -479,13 +547,13 @@ __global__ void __launch_bounds__(64) k_scan_fast(ScanArgs a, int n_waves) {
             const int n = Ms->n_seg[0] + Ms->n_seg[1] + Ms->n_seg[2] + Ms->n_seg[3];
             __syncthreads();
             if (F32) {
-                for (int t = lane; t < n; t += 64) {
+                for (int t = threadIdx.x; t < n; t += blockDim.x) {
                     lds_raw[t] = a.seg32a[Ms->seg_off + t];
                     lds_raw[n + t] = a.seg32b[Ms->seg_off + t];
                 }
             } else {
                 SiteF64* dst = reinterpret_cast<SiteF64*>(lds_raw);
-                for (int t = lane; t < n; t += 64) dst[t] = a.segf64[Ms->seg_off + t];
+                for (int t = threadIdx.x; t < n; t += blockDim.x) dst[t] = a.segf64[Ms->seg_off + t];
             }
             __syncthreads();
             lds_n = n;
@@ -551,11 +619,16 @@ __global__ void __launch_bounds__(64) k_scan_fast(ScanArgs a, int n_waves) {
         }
         if (!__any(col >= 0)) {
             // molecule m exhausted for this wave: move to one with columns left
-            int nm = -1;
-            for (int q = 1; q <= a.n_mols; ++q) {
-                int cand = m + q;
-                if (cand >= a.n_mols) cand -= a.n_mols;
-                if (__atomic_load_n(&a.next_col[cand], __ATOMIC_RELAXED) < n_cols) { nm = cand; break; }
+            int nm;
+            if (shared_lds) {
+                // m's counter is spent, so every wave of the workgroup arrives
+                // here after its last column; one lane picks for all of them
+                __syncthreads();
+                if (threadIdx.x == 0) next_shared = next_molecule(a, m, n_cols);
+                __syncthreads();
+                nm = next_shared;
+            } else {
+                nm = next_molecule(a, m, n_cols);
             }
             nm = __builtin_amdgcn_readfirstlane(nm);
             if (nm < 0) break;
@@ -727,35 +800,37 @@ __global__ void __launch_bounds__(64) k_scan_fast(ScanArgs a, int n_waves) {
 }
 
 template <int G, bool F32, int COMP>
-hipError_t launch_one(const ScanArgs& a, int n_waves, size_t lds, hipStream_t st) {
-    hipLaunchKernelGGL((k_scan_fast<G, F32, COMP>), dim3(n_waves), dim3(64), G > 1 ? lds : 0, st, a, n_waves);
+hipError_t launch_one(const ScanArgs& a, int n_blocks, int W, size_t lds, hipStream_t st) {
+    hipLaunchKernelGGL((k_scan_fast<G, F32, COMP>), dim3(n_blocks), dim3(64 * W), G > 1 ? lds : 0, st, a);
     return hipGetLastError();
 }
 
 template <bool F32, int COMP>
-hipError_t launch_g(const ScanArgs& a, int G, int n_waves, size_t lds, hipStream_t st) {
+hipError_t launch_g(const ScanArgs& a, int G, int n_blocks, int W, size_t lds, hipStream_t st) {
     switch (G) {
-        case 1: return launch_one<1, F32, COMP>(a, n_waves, lds, st);
-        case 2: return launch_one<2, F32, COMP>(a, n_waves, lds, st);
-        case 4: return launch_one<4, F32, COMP>(a, n_waves, lds, st);
-        case 8: return launch_one<8, F32, COMP>(a, n_waves, lds, st);
-        case 16: return launch_one<16, F32, COMP>(a, n_waves, lds, st);
-        case 32: return launch_one<32, F32, COMP>(a, n_waves, lds, st);
-        case 64: return launch_one<64, F32, COMP>(a, n_waves, lds, st);
+        case 1: return launch_one<1, F32, COMP>(a, n_blocks, W, lds, st);
+        case 2: return launch_one<2, F32, COMP>(a, n_blocks, W, lds, st);
+        case 4: return launch_one<4, F32, COMP>(a, n_blocks, W, lds, st);
+        case 8: return launch_one<8, F32, COMP>(a, n_blocks, W, lds, st);
+        case 16: return launch_one<16, F32, COMP>(a, n_blocks, W, lds, st);
+        case 32: return launch_one<32, F32, COMP>(a, n_blocks, W, lds, st);
+        case 64: return launch_one<64, F32, COMP>(a, n_blocks, W, lds, st);
         default: return hipErrorInvalidValue;
     }
 }
 
 }  // namespace
 
-// lds_sites: largest per-molecule site count (sizes the LDS copy for G > 1)
-hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_waves, int lds_sites,
+// n_blocks workgroups of W waves (W = 1, 2, 4); lds_sites: largest
+// per-molecule site count (sizes the LDS copy for G > 1)
+hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_blocks, int W, int lds_sites,
                             hipStream_t st) {
-    if (!f32) return launch_g<false, 0>(a, G, n_waves, (size_t)lds_sites * sizeof(SiteF64), st);
+    if (W != 1 && W != 2 && W != 4) return hipErrorInvalidValue;
+    if (!f32) return launch_g<false, 0>(a, G, n_blocks, W, (size_t)lds_sites * sizeof(SiteF64), st);
     const size_t lds = (size_t)lds_sites * 2 * sizeof(float4);
-    if (comp == ACC_KAHAN) return launch_g<true, ACC_KAHAN>(a, G, n_waves, lds, st);
-    if (comp == ACC_NONE) return launch_g<true, ACC_NONE>(a, G, n_waves, lds, st);
-    return launch_g<true, ACC_BLOCK>(a, G, n_waves, lds, st);
+    if (comp == ACC_KAHAN) return launch_g<true, ACC_KAHAN>(a, G, n_blocks, W, lds, st);
+    if (comp == ACC_NONE) return launch_g<true, ACC_NONE>(a, G, n_blocks, W, lds, st);
+    return launch_g<true, ACC_BLOCK>(a, G, n_blocks, W, lds, st);
 }
 
 }  // namespace mafm
