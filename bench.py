@@ -69,12 +69,12 @@ def run_batch(abi, sims, prec, **kw):
     return abi.batch_run(sims, prec, **kw)
 
 
-def autotune(abi, sims, prec, candidates, compensation=0, waves_per_group=0):
+def autotune(abi, sims, prec, candidates, compensation=0, waves_per_group=0, options=0):
     best = None
     for G, wps in candidates:
         try:
             st = run_batch(abi, sims, prec, lanes_per_pixel=G, waves_per_simd=wps, compensation=compensation,
-                           waves_per_group=waves_per_group)
+                           waves_per_group=waves_per_group, options=options)
         except abi.MafmError as e:
             print(f"# tune G={G} waves/SIMD={wps}: skipped ({e})", file=sys.stderr, flush=True)
             continue
@@ -128,6 +128,8 @@ def main():
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--group", type=int, default=0, help="waves per workgroup (0 = automatic)")
+    ap.add_argument("--reproducible", action="store_true",
+                    help="fixed summation order per position (no wide evaluation of a wave's last groups)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
@@ -168,11 +170,12 @@ def main():
               file=sys.stderr, flush=True)
 
         G, wps = args.lanes, args.waves
+        options = abi.SCAN_REPRODUCIBLE if args.reproducible else 0
         if args.precision != "plain64" and not args.no_tune and (G == 0 or wps == 0):
             cands = [(g, w) for g in ([G] if G else [1, 2, 4]) for w in ([wps] if wps else [1, 2])]
-            G, wps, _ = autotune(abi, sims, args.precision, cands, args.compensation, args.group)
+            G, wps, _ = autotune(abi, sims, args.precision, cands, args.compensation, args.group, options)
         kw = dict(lanes_per_pixel=G, waves_per_simd=wps, compensation=args.compensation,
-                  waves_per_group=args.group)
+                  waves_per_group=args.group, options=options)
         for _ in range(args.warmup):
             run_batch(abi, sims, args.precision, **kw)
         kms = []
@@ -199,6 +202,7 @@ def main():
                 "total_steps": st["total_steps"],
                 "compensation": {1: "kahan", 2: "none"}.get(args.compensation, "block8-fp64"),
                 "group_utilisation": st["active_group_trips"] / max(1, st["group_trips"]),
+                "wide_trip_fraction": st["wide_trips"] * 64 / st["lanes_per_pixel"] / max(1, st["group_trips"]),
                 "unconverged": st["unconverged"]},
             "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": FP32_VALU_PEAK / 1e12,
                          "unit": "TFLOP/s", "frac": achieved / FP32_VALU_PEAK, "traffic": None,

@@ -122,6 +122,15 @@ enum { MAFM_TERM_E = 0, MAFM_TERM_F = 1, MAFM_TERM_EF = 2 };
  * accumulation; minimiser state and convergence tests stay FP64. */
 enum { MAFM_PREC_PLAIN64 = 0, MAFM_PREC_FAST64 = 1, MAFM_PREC_FAST32 = 2 };
 
+/* mafm_scan_desc::options bits (FAST kernels).
+ * REPRODUCIBLE: every position's site sums follow one fixed order, so two
+ * runs with the same tuning give identical records.  Without it, a wave
+ * whose pixel groups have mostly finished evaluates its remaining groups
+ * one after another with all 64 lanes (shorter tail, same arithmetic, but
+ * the summation order of a position then depends on timing: results agree
+ * to FP32 rounding, not bit for bit). */
+enum { MAFM_SCAN_REPRODUCIBLE = 1 };
+
 typedef struct {
     int32_t nx, ny, nz;
     double dx, dy, dz;
@@ -136,6 +145,8 @@ typedef struct {
     int32_t compensation;      /* FAST32 sums: 0 = 3 = FP32 blocks of 8 sites folded into FP64
                                   (default), 1 = Kahan-compensated FP32, 2 = plain FP32 */
     int32_t waves_per_group;   /* 1, 2, 4 waves per workgroup sharing one LDS site copy */
+    uint32_t options;          /* MAFM_SCAN_* bits */
+    int32_t reserved;
 } mafm_scan_desc;
 
 /* Per scan position (OutputData, globals.hpp:27-32, minus the derived r and
@@ -164,6 +175,7 @@ typedef struct {
     int32_t block_threads;
     int64_t group_trips;        /* FAST kernels: pixel-group slots x loop trips */
     int64_t active_group_trips; /* ... of which the group had a column (utilisation) */
+    int64_t wide_trips;         /* wave trips that evaluated their groups with all 64 lanes */
 } mafm_scan_stats;
 
 typedef struct {

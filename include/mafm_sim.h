@@ -48,6 +48,7 @@ typedef struct {
     int32_t waves_per_simd;       /* 0 = automatic */
     int32_t compensation;         /* FAST32 sums (mafm_scan_desc::compensation) */
     int32_t waves_per_group;      /* 0 = automatic (mafm_scan_desc::waves_per_group) */
+    uint32_t options;             /* mafm_scan_desc::options */
 } mafm_run_opts;
 
 mafm_status mafm_sim_open(const char* input_file, const char* output_dir, mafm_ctx* ctx, mafm_sim** out);

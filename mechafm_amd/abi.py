@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libmafm.so")
 MAFM_OK = 0
 STATUS = {0: "OK", 1: "INVALID", 2: "HIP", 3: "OUT_OF_GRID", 4: "STATE", 5: "FFT_IMAG", 6: "NO_DEVICE"}
 PREC = {"plain64": 0, "fast64": 1, "fast32": 2}
+SCAN_REPRODUCIBLE = 1
 VDW_LJ, VDW_MORSE = 0, 1
 
 
@@ -55,7 +56,8 @@ class ScanDesc(ctypes.Structure):
                 ("minterm", ctypes.c_int32), ("etol", ctypes.c_double), ("ftol", ctypes.c_double),
                 ("dt", ctypes.c_double), ("maxsteps", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("lanes_per_pixel", ctypes.c_int32), ("waves_per_simd", ctypes.c_int32),
-                ("compensation", ctypes.c_int32), ("waves_per_group", ctypes.c_int32)]
+                ("compensation", ctypes.c_int32), ("waves_per_group", ctypes.c_int32),
+                ("options", ctypes.c_uint32), ("reserved", ctypes.c_int32)]
 
 
 RECORD_DTYPE = np.dtype([("x", "f8"), ("y", "f8"), ("z", "f8"), ("fx", "f8"), ("fy", "f8"), ("fz", "f8"),
@@ -68,7 +70,8 @@ class ScanStats(ctypes.Structure):
                 ("unconverged", ctypes.c_int64), ("grid_warnings", ctypes.c_int64), ("pair_evals", ctypes.c_int64),
                 ("lanes_per_pixel", ctypes.c_int32), ("waves_per_simd", ctypes.c_int32),
                 ("grid_blocks", ctypes.c_int32), ("block_threads", ctypes.c_int32),
-                ("group_trips", ctypes.c_int64), ("active_group_trips", ctypes.c_int64)]
+                ("group_trips", ctypes.c_int64), ("active_group_trips", ctypes.c_int64),
+                ("wide_trips", ctypes.c_int64)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -108,7 +111,7 @@ class SimInfo(ctypes.Structure):
 class RunOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("lanes_per_pixel", ctypes.c_int32),
                 ("waves_per_simd", ctypes.c_int32), ("compensation", ctypes.c_int32),
-                ("waves_per_group", ctypes.c_int32)]
+                ("waves_per_group", ctypes.c_int32), ("options", ctypes.c_uint32)]
 
 
 # every exported entry point and its signature: (restype, argtypes)
@@ -326,15 +329,15 @@ class Simulation:
         return [d.sites[k] for k in range(d.n_sites)]
 
     def scan_desc(self, precision: str = "fast64", lanes_per_pixel: int = 0, waves_per_simd: int = 0,
-                  compensation: int = 0, waves_per_group: int = 0) -> ScanDesc:
-        o = RunOpts(PREC[precision], lanes_per_pixel, waves_per_simd, compensation, waves_per_group)
+                  compensation: int = 0, waves_per_group: int = 0, options: int = 0) -> ScanDesc:
+        o = RunOpts(PREC[precision], lanes_per_pixel, waves_per_simd, compensation, waves_per_group, options)
         d = ScanDesc()
         self.check(lib().mafm_sim_scan_desc(self.h, ctypes.byref(o), ctypes.byref(d)))
         return d
 
     def run(self, precision: str = "fast64", lanes_per_pixel: int = 0, waves_per_simd: int = 0,
-            compensation: int = 0, waves_per_group: int = 0) -> dict:
-        o = RunOpts(PREC[precision], lanes_per_pixel, waves_per_simd, compensation, waves_per_group)
+            compensation: int = 0, waves_per_group: int = 0, options: int = 0) -> dict:
+        o = RunOpts(PREC[precision], lanes_per_pixel, waves_per_simd, compensation, waves_per_group, options)
         st = ScanStats()
         self.check(lib().mafm_sim_run(self.h, ctypes.byref(o), ctypes.byref(st)))
         return st.as_dict()
@@ -351,9 +354,10 @@ class Simulation:
 
 
 def batch_run(sims: Sequence[Simulation], precision: str = "fast32", lanes_per_pixel: int = 0,
-              waves_per_simd: int = 0, compensation: int = 0, waves_per_group: int = 0) -> dict:
+              waves_per_simd: int = 0, compensation: int = 0, waves_per_group: int = 0,
+              options: int = 0) -> dict:
     arr = (ctypes.c_void_p * len(sims))(*[s.h.value for s in sims])
-    o = RunOpts(PREC[precision], lanes_per_pixel, waves_per_simd, compensation, waves_per_group)
+    o = RunOpts(PREC[precision], lanes_per_pixel, waves_per_simd, compensation, waves_per_group, options)
     st = ScanStats()
     rc = lib().mafm_batch_run(len(sims), arr, ctypes.byref(o), ctypes.byref(st))
     if rc != MAFM_OK:

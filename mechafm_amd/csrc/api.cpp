@@ -526,6 +526,8 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
     a.minimiser = d->minimiser; a.integrator = d->integrator; a.minterm = d->minterm;
     a.maxsteps = d->maxsteps; a.etol = d->etol; a.ftol = d->ftol; a.dt = d->dt;
     a.rec = c->d_rec.p;
+    // last groups of a wave: regroup over 64 lanes once that halves the sites per lane
+    a.wide_tail = (d->options & MAFM_SCAN_REPRODUCIBLE) ? 0 : 2;
     HIPCHK(c, hipMemsetAsync(c->d_counters.p, 0, 8 * sizeof(int64_t), c->stream));
 
     const long cols_per_mol = (long)d->nx * d->ny;
@@ -619,6 +621,7 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
         stats->lanes_per_pixel = G;
         stats->group_trips = counters[4];
         stats->active_group_trips = counters[5];
+        stats->wide_trips = counters[6];
         stats->waves_per_simd = wps;
         stats->grid_blocks = d->precision == MAFM_PREC_PLAIN64 ? (int)((total_cols + 255) / 256) : n_blocks;
         stats->block_threads = d->precision == MAFM_PREC_PLAIN64 ? 256 : 64 * W;

@@ -98,6 +98,8 @@ struct ScanArgs {
     int32_t* next_col;        // per molecule: next column to hand out
     const float* mol_cum;     // cumulative work fraction per molecule (n_mols + 1)
     int64_t* counters;        // [0] total steps [1] unconverged [2] grid warnings [3] out-of-grid
+    int32_t wide_tail;        // > 0: regroup a wave's last groups over 64 lanes when that cuts
+                              // the sites per lane by this factor; 0: never
 };
 
 enum { REC_UNCONVERGED = 1, REC_GRID_CLAMP = 2, REC_OUT_OF_GRID = 4 };

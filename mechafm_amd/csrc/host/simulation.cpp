@@ -774,6 +774,7 @@ mafm_scan_desc Simulation::scanDesc(const mafm_run_opts& r) const {
     d.waves_per_simd = r.waves_per_simd;
     d.compensation = r.compensation;
     d.waves_per_group = r.waves_per_group;
+    d.options = r.options;
     return d;
 }
 

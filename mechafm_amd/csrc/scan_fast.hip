@@ -171,15 +171,16 @@ __device__ inline void pair32(const float4 a, const float4 b, float tx, float ty
             ir2 = __builtin_amdgcn_rcpf(r2);
         }
         const float ir6 = ir2 * ir2 * ir2;
-        const float tb = a.w * ir6;             // 6 es6 / r^6
-        const float ta = b.x * ir6 * ir6;       // 6 es12 / r^12
-        const float el = ta - tb;               // 6 E_LJ; force factor (12 a - 6 b)/r^2 = (ta + el)/r^2
+        const float w = b.x * ir6;              // 6 es12 / r^6
+        const float pl = w - a.w;               // 6 (es12 / r^6 - es6)
+        const float el = pl * ir6;              // 6 E_LJ
+        const float fl = w + pl;                // (w + pl) / r^6 = 12 es12 / r^12 - 6 es6 / r^6
         if (SEG == 0) {
             const float ec = b.y * ir;          // E_Coulomb; force factor E_C / r^2
-            fs = (ta + el + ec) * ir2;
+            fs = fmaf(fl, ir6, ec) * ir2;
             e6 = fmaf(6.f, ec, el);
         } else {
-            fs = (ta + el) * ir2;
+            fs = fl * ir6 * ir2;
             e6 = el;
         }
     } else {
@@ -207,14 +208,14 @@ __device__ inline void pair32(const float4 a, const float4 b, float tx, float ty
 // t - shift[im] in FP32 relative to the molecule's origin.  IMG = false is
 // the single zero-shift image of a non-periodic system.
 template <int SEG, int G, int MODE, bool IMG>
-__device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<G>& s, int n, int gl, float tx, float ty, float tz,
-                                 Acc32<MODE>& acc) {
+__device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<G>& s, int n, int gl, int S, float tx, float ty,
+                                 float tz, Acc32<MODE>& acc) {
     const int n_img = IMG ? M->n_images : 1;
-    const int full = (n / (8 * G)) * (8 * G);
-    for (int t0 = 0; t0 < full; t0 += 8 * G) {
+    const int full = (n / (8 * S)) * (8 * S);
+    for (int t0 = 0; t0 < full; t0 += 8 * S) {
         float4 sa[8], sb[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) load_site<SEG, G>(s, t0 + u * G + gl, sa[u], sb[u]);
+        for (int u = 0; u < 8; ++u) load_site<SEG, G>(s, t0 + u * S + gl, sa[u], sb[u]);
         for (int im = 0; im < n_img; ++im) {
             float ix = tx, iy = ty, iz = tz;
             if (IMG) {
@@ -233,7 +234,7 @@ __device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<G>& s, int n, int
         }
     }
     if (!IMG) acc.begin();
-    for (int t = full + gl; t < n; t += G) {
+    for (int t = full + gl; t < n; t += S) {
         float4 sa, sb;
         load_site<SEG, G>(s, t, sa, sb);
         if (IMG) acc.begin();
@@ -254,28 +255,28 @@ __device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<G>& s, int n, int
 }
 
 template <int G, int MODE, bool IMG>
-__device__ inline void seg_sums32(cptr<DevMol> M, const Sites32<G> s0, float tx, float ty, float tz, int gl,
+__device__ inline void seg_sums32(cptr<DevMol> M, const Sites32<G> s0, float tx, float ty, float tz, int gl, int S,
                                   Acc32<MODE>& acc) {
     Sites32<G> s = s0;
     int n = M->n_seg[0];
-    seg_sum32<0, G, MODE, IMG>(M, s, n, gl, tx, ty, tz, acc);
+    seg_sum32<0, G, MODE, IMG>(M, s, n, gl, S, tx, ty, tz, acc);
     s.advance(n);
     n = M->n_seg[1];
-    seg_sum32<1, G, MODE, IMG>(M, s, n, gl, tx, ty, tz, acc);
+    seg_sum32<1, G, MODE, IMG>(M, s, n, gl, S, tx, ty, tz, acc);
     s.advance(n);
     n = M->n_seg[2];
-    if (n) seg_sum32<2, G, MODE, IMG>(M, s, n, gl, tx, ty, tz, acc);
+    if (n) seg_sum32<2, G, MODE, IMG>(M, s, n, gl, S, tx, ty, tz, acc);
     s.advance(n);
     n = M->n_seg[3];
-    if (n) seg_sum32<3, G, MODE, IMG>(M, s, n, gl, tx, ty, tz, acc);
+    if (n) seg_sum32<3, G, MODE, IMG>(M, s, n, gl, S, tx, ty, tz, acc);
 }
 
 // Sum of all tip-surface site terms of molecule M at tip position (px,py,pz):
-// this lane's share (sites gl, gl+G, ... of each segment, every image).
-// Coordinates are relative to the molecule's origin so FP32 keeps ~1e-6 A
-// resolution.
+// this lane's share, sites gl, gl+S, ... of each segment (every image), S
+// lanes per position.  Coordinates are relative to the molecule's origin so
+// FP32 keeps ~1e-6 A resolution.
 template <int G, int MODE>
-__device__ void site_sums32(cptr<DevMol> M, const Sites32<G> s0, double px, double py, double pz, int gl,
+__device__ void site_sums32(cptr<DevMol> M, const Sites32<G> s0, double px, double py, double pz, int gl, int S,
                             double out[4]) {
     Acc32<MODE> acc;
     acc.init();
@@ -283,9 +284,9 @@ __device__ void site_sums32(cptr<DevMol> M, const Sites32<G> s0, double px, doub
     const float ty = (float)(py - M->origin[1]);
     const float tz = (float)(pz - M->origin[2]);
     if (M->n_images > 1)
-        seg_sums32<G, MODE, true>(M, s0, tx, ty, tz, gl, acc);
+        seg_sums32<G, MODE, true>(M, s0, tx, ty, tz, gl, S, acc);
     else
-        seg_sums32<G, MODE, false>(M, s0, tx, ty, tz, gl, acc);
+        seg_sums32<G, MODE, false>(M, s0, tx, ty, tz, gl, S, acc);
     out[0] = acc.get(0);
     out[1] = acc.get(1);
     out[2] = acc.get(2);
@@ -338,14 +339,14 @@ __device__ inline void pair64(const SiteF64& st, double tx, double ty, double tz
 }
 
 template <int SEG, int G>
-__device__ inline void seg_sum64(cptr<DevMol> M, const Sites<G, SiteF64>& s, int n, int gl, double px, double py,
-                                 double pz, double acc[4]) {
+__device__ inline void seg_sum64(cptr<DevMol> M, const Sites<G, SiteF64>& s, int n, int gl, int S, double px,
+                                 double py, double pz, double acc[4]) {
     const int n_img = M->n_images;
-    const int full = (n / (4 * G)) * (4 * G);
-    for (int t0 = 0; t0 < full; t0 += 4 * G) {
+    const int full = (n / (4 * S)) * (4 * S);
+    for (int t0 = 0; t0 < full; t0 += 4 * S) {
         SiteF64 st[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) st[u] = s[t0 + u * G + gl];
+        for (int u = 0; u < 4; ++u) st[u] = s[t0 + u * S + gl];
         for (int im = 0; im < n_img; ++im) {
             const double tx = px - M->shift[im][0], ty = py - M->shift[im][1], tz = pz - M->shift[im][2];
 #pragma unroll
@@ -356,7 +357,7 @@ __device__ inline void seg_sum64(cptr<DevMol> M, const Sites<G, SiteF64>& s, int
             }
         }
     }
-    for (int t = full + gl; t < n; t += G) {
+    for (int t = full + gl; t < n; t += S) {
         const SiteF64 st = s[t];
         for (int im = 0; im < n_img; ++im) {
             const double tx = px - M->shift[im][0], ty = py - M->shift[im][1], tz = pz - M->shift[im][2];
@@ -369,20 +370,20 @@ __device__ inline void seg_sum64(cptr<DevMol> M, const Sites<G, SiteF64>& s, int
 
 template <int G>
 __device__ void site_sums64(cptr<DevMol> M, const Sites<G, SiteF64> s0, double px, double py, double pz, int gl,
-                            double out[4]) {
+                            int S, double out[4]) {
     double acc[4] = {0, 0, 0, 0};
     Sites<G, SiteF64> s = s0;
     int n = M->n_seg[0];
-    seg_sum64<0, G>(M, s, n, gl, px, py, pz, acc);
+    seg_sum64<0, G>(M, s, n, gl, S, px, py, pz, acc);
     s.advance(n);
     n = M->n_seg[1];
-    seg_sum64<1, G>(M, s, n, gl, px, py, pz, acc);
+    seg_sum64<1, G>(M, s, n, gl, S, px, py, pz, acc);
     s.advance(n);
     n = M->n_seg[2];
-    if (n) seg_sum64<2, G>(M, s, n, gl, px, py, pz, acc);
+    if (n) seg_sum64<2, G>(M, s, n, gl, S, px, py, pz, acc);
     s.advance(n);
     n = M->n_seg[3];
-    if (n) seg_sum64<3, G>(M, s, n, gl, px, py, pz, acc);
+    if (n) seg_sum64<3, G>(M, s, n, gl, S, px, py, pz, acc);
     out[0] = acc[0];
     out[1] = acc[1];
     out[2] = acc[2];
@@ -484,6 +485,20 @@ __device__ void tip_dummy(cptr<DevMol> M, double dx_, double dy_, double dz_, do
     out[0] += fx; out[1] += fy; out[2] += fz; out[3] += e;
 }
 
+// Position of the k-th (0-based) set bit of m; k < popcount(m).
+__device__ inline int nth_set_bit(uint64_t m, int k) {
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const int c = __popcll((m >> pos) & ((1ull << w) - 1));
+        if (k >= c) {
+            k -= c;
+            pos += w;
+        }
+    }
+    return pos;
+}
+
 template <int G>
 __device__ inline double group_sum(double v) {
 #pragma unroll
@@ -526,7 +541,7 @@ __device__ int next_molecule(const ScanArgs& a, int m, int n_cols) {
 // still steals columns on its own and the workgroup moves to the next
 // molecule together once every wave has finished its last column.
 template <int G, bool F32, int COMP>
-__global__ void __launch_bounds__(256) k_scan_fast(ScanArgs a) {
+__device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanArgs& a) {
     const int lane = threadIdx.x & 63;
     const bool shared_lds = G > 1 && blockDim.x > 64;
     const int gl = lane % G;
@@ -560,7 +575,7 @@ __global__ void __launch_bounds__(256) k_scan_fast(ScanArgs a) {
         }
     };
     stage(m);
-    int64_t trips = 0, active = 0;
+    int64_t trips = 0, active = 0, wide_trips = 0;
 
     // group state (identical on the G lanes of a group)
     int col = -1, k = 0, ph = 0, n_tot = 1, nnn = 0, flags = 0, i = 0, j = 0;
@@ -647,15 +662,53 @@ __global__ void __launch_bounds__(256) k_scan_fast(ScanArgs a) {
             r[0] = r[1] = r[2] = r[3] = 0;
             grid_lookup<F32>(a.grids[M->force_grid], Px, Py, Pz, r, flags);
         } else {
-            if constexpr (F32) {
-                Sites32<G> src{a.seg32a + M->seg_off, a.seg32b + M->seg_off, lds_raw, lds_raw + lds_n};
-                site_sums32<G, COMP>(M, src, Px, Py, Pz, gl, r);
+            // Few groups left with work (a molecule or the whole scan running
+            // dry): spread the n_act active positions over all 64 lanes,
+            // Ge = 64 / pow2ceil(n_act) lanes each, when that cuts the sites
+            // per lane by at least wide_tail (a.wide_tail = 0: never).
+            const uint64_t lead_act = __ballot(leader && col >= 0);
+            const int n_act = __popcll(lead_act);
+            int p2 = 1;
+            while (p2 < n_act) p2 <<= 1;
+            const int Ge = 64 / p2;
+            const bool wide = G < 64 && a.wide_tail > 0 && Ge >= a.wide_tail * G;
+            if (wide) {
+                ++wide_trips;
+                const int slot = lane / Ge, sub = lane & (Ge - 1);
+                const bool live = slot < n_act;
+                const int src = live ? nth_set_bit(lead_act, slot) : 0;
+                const double qx = __shfl(Px, src, 64), qy = __shfl(Py, src, 64), qz = __shfl(Pz, src, 64);
+                double part[4] = {0, 0, 0, 0};
+                if (live) {
+                    if constexpr (F32) {
+                        // G == 1 keeps the sites in global memory: wide loads read them there
+                        const float4* wa = G > 1 ? lds_raw : a.seg32a + M->seg_off;
+                        const float4* wb = G > 1 ? lds_raw + lds_n : a.seg32b + M->seg_off;
+                        Sites32<64> src64{wa, wb, wa, wb};
+                        site_sums32<64, COMP>(M, src64, qx, qy, qz, sub, Ge, part);
+                    } else {
+                        const SiteF64* w = G > 1 ? reinterpret_cast<const SiteF64*>(lds_raw) : a.segf64 + M->seg_off;
+                        Sites<64, SiteF64> src64{w, w};
+                        site_sums64<64>(M, src64, qx, qy, qz, sub, Ge, part);
+                    }
+                }
+                for (int w2 = 1; w2 < Ge; w2 <<= 1)
+                    for (int q = 0; q < 4; ++q) part[q] += __shfl_xor(part[q], w2, 64);
+                // group of rank k (k-th active leader) reads slot k
+                const int rank = __popcll(lead_act & ((1ull << gbase) - 1));
+                const int from = (rank * Ge) & 63;
+                for (int q = 0; q < 4; ++q) r[q] = __shfl(part[q], from, 64);
             } else {
-                Sites<G, SiteF64> src{a.segf64 + M->seg_off, reinterpret_cast<const SiteF64*>(lds_raw)};
-                site_sums64<G>(M, src, Px, Py, Pz, gl, r);
-            }
-            if (G > 1) {
-                for (int q = 0; q < 4; ++q) r[q] = group_sum<G>(r[q]);
+                if constexpr (F32) {
+                    Sites32<G> src{a.seg32a + M->seg_off, a.seg32b + M->seg_off, lds_raw, lds_raw + lds_n};
+                    site_sums32<G, COMP>(M, src, Px, Py, Pz, gl, G, r);
+                } else {
+                    Sites<G, SiteF64> src{a.segf64 + M->seg_off, reinterpret_cast<const SiteF64*>(lds_raw)};
+                    site_sums64<G>(M, src, Px, Py, Pz, gl, G, r);
+                }
+                if (G > 1) {
+                    for (int q = 0; q < 4; ++q) r[q] = group_sum<G>(r[q]);
+                }
             }
             if (M->potential_grid >= 0) grid_lookup<F32>(a.grids[M->potential_grid], Px, Py, Pz, r, flags);
         }
@@ -789,6 +842,7 @@ __global__ void __launch_bounds__(256) k_scan_fast(ScanArgs a) {
     }
     if (lane == 0) {
         atomicAdd((unsigned long long*)&a.counters[4], (unsigned long long)(trips * (64 / G)));
+        if (wide_trips) atomicAdd((unsigned long long*)&a.counters[6], (unsigned long long)wide_trips);
     }
     if (leader) {
         if (active) atomicAdd((unsigned long long*)&a.counters[5], (unsigned long long)active);
@@ -799,9 +853,23 @@ __global__ void __launch_bounds__(256) k_scan_fast(ScanArgs a) {
     }
 }
 
+template <int G, int COMP>
+__global__ void __launch_bounds__(256) k_scan_fast32(ScanArgs a) {
+    scan_fast_body<G, true, COMP>(a);
+}
+
+// FP64 pair terms hold more live registers: ask for two waves per SIMD
+template <int G>
+__global__ void __launch_bounds__(256, 2) k_scan_fast64(ScanArgs a) {
+    scan_fast_body<G, false, 0>(a);
+}
+
 template <int G, bool F32, int COMP>
 hipError_t launch_one(const ScanArgs& a, int n_blocks, int W, size_t lds, hipStream_t st) {
-    hipLaunchKernelGGL((k_scan_fast<G, F32, COMP>), dim3(n_blocks), dim3(64 * W), G > 1 ? lds : 0, st, a);
+    if constexpr (F32)
+        hipLaunchKernelGGL((k_scan_fast32<G, COMP>), dim3(n_blocks), dim3(64 * W), G > 1 ? lds : 0, st, a);
+    else
+        hipLaunchKernelGGL((k_scan_fast64<G>), dim3(n_blocks), dim3(64 * W), G > 1 ? lds : 0, st, a);
     return hipGetLastError();
 }
 

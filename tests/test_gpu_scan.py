@@ -96,17 +96,62 @@ def test_eval_tip_surface_matches_reference(gpu_ctx, case, tmp_path):
 
 
 def test_batch_equals_single_runs(gpu_ctx, tmp_path):
-    """Three molecules in one launch give the records of three single runs."""
+    """Three molecules in one launch give the records of three single runs
+    (fixed summation order and lanes, so the comparison is bit for bit)."""
     from mechafm_amd import abi, workloads as W
     wls = W.make_c3(str(tmp_path), mini=True, count=3)
     sims = [abi.Simulation(w.input_path, gpu_ctx) for w in wls]
-    abi.batch_run(sims, "fast64")
+    kw = dict(lanes_per_pixel=2, options=abi.SCAN_REPRODUCIBLE)
+    abi.batch_run(sims, "fast64", **kw)
     batched = [s.records().copy() for s in sims]
     for s, b in zip(sims, batched):
-        s.run("fast64")
+        s.run("fast64", **kw)
         single = s.records()
-        assert np.abs(single["rx"] - b["rx"]).max() < 1e-9
-        assert np.abs(single["fz"] - b["fz"]).max() < 1e-9
+        assert (single == b).all()
+
+
+@pytest.mark.parametrize("precision", ["fast32", "fast64"])
+def test_reproducible_runs_are_identical(gpu_ctx, tmp_path, precision):
+    from mechafm_amd import abi, workloads as W
+    wls = W.make_c3(str(tmp_path), mini=True, count=4)
+    sims = [abi.Simulation(w.input_path, gpu_ctx) for w in wls]
+    runs = []
+    for _ in range(2):
+        st = abi.batch_run(sims, precision, lanes_per_pixel=2, options=abi.SCAN_REPRODUCIBLE)
+        assert st["wide_trips"] == 0
+        runs.append(np.concatenate([s.records().copy() for s in sims]))
+    assert (runs[0] == runs[1]).all()
+
+
+@pytest.mark.parametrize("G", [1, 2, 8])
+def test_wide_tail_matches_fixed_order(gpu_ctx, tmp_path, G):
+    """Regrouping a wave's last pixel groups over 64 lanes changes only the
+    summation order: the fast32 bar against the fixed-order run."""
+    from mechafm_amd import abi, workloads as W
+    wls = W.make_c3(str(tmp_path), mini=True, count=4)
+    sims = [abi.Simulation(w.input_path, gpu_ctx) for w in wls]
+    abi.batch_run(sims, "fast32", lanes_per_pixel=G, options=abi.SCAN_REPRODUCIBLE)
+    fixed = np.concatenate([s.records().copy() for s in sims])
+    st = abi.batch_run(sims, "fast32", lanes_per_pixel=G)
+    wide = np.concatenate([s.records().copy() for s in sims])
+    assert st["wide_trips"] > 0
+    dpos = np.abs(np.stack([wide[c] - fixed[c] for c in ("rx", "ry", "rz")])).max(0)
+    dfz = np.abs(wide["fz"] - fixed["fz"]) / np.maximum(np.abs(fixed["fz"]), 1e-2)
+    ok = (dpos <= 1e-4) & (dfz <= 1e-3)
+    print(f"G={G}: wide trips {st['wide_trips']}, within bar {ok.mean() * 100:.2f}%")
+    assert ok.mean() >= 0.99
+
+
+@pytest.mark.parametrize("W", [1, 2, 4])
+def test_waves_per_group_share_sites(gpu_ctx, tmp_path, W):
+    """W waves of a workgroup share one LDS copy of a periodic system's sites
+    (c4 mini: slab with 9 images plus an external potential)."""
+    sim, tab, stats, shape = _run(gpu_ctx, "c4_mini", tmp_path, "fast32", lanes_per_pixel=4, waves_per_group=W)
+    _, ref = golden_records("c4_mini")
+    dpos, dfz = _errors(tab, ref)
+    ok = (dpos <= 1e-4) & (dfz <= 1e-3)
+    assert stats["block_threads"] == 64 * W
+    assert ok.mean() >= 0.99
 
 
 def test_output_files_match_reference_format(gpu_ctx, tmp_path):
