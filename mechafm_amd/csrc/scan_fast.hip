@@ -114,9 +114,11 @@ struct Acc32 {
         if (MODE == ACC_BLOCK)
             for (int q = 0; q < 4; ++q) s[q] = 0.f;
     }
+    // E = false: the energy component was not accumulated
+    template <bool E = true>
     __device__ inline void end() {
         if (MODE == ACC_BLOCK)
-            for (int q = 0; q < 4; ++q) d[q] += (double)s[q];
+            for (int q = 0; q < (E ? 4 : 3); ++q) d[q] += (double)s[q];
     }
     __device__ inline void add(int q, float x) {
         if (MODE == ACC_KAHAN) {
@@ -206,8 +208,9 @@ __device__ inline void pair32(const float4 a, const float4 b, float tx, float ty
 // images, no bound test), then the remainder one site at a time.  The image
 // shifts are wave-uniform (scalar loads); the tip of image im is
 // t - shift[im] in FP32 relative to the molecule's origin.  IMG = false is
-// the single zero-shift image of a non-periodic system.
-template <int SEG, int G, int MODE, bool IMG>
+// the single zero-shift image of a non-periodic system; E = false skips the
+// energy (the compiler drops its arithmetic).
+template <int SEG, int G, int MODE, bool IMG, bool E>
 __device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<G>& s, int n, int gl, int S, float tx, float ty,
                                  float tz, Acc32<MODE>& acc) {
     const int n_img = IMG ? M->n_images : 1;
@@ -228,9 +231,10 @@ __device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<G>& s, int n, int
             for (int u = 0; u < 8; ++u) {
                 float fx, fy, fz, e;
                 pair32<SEG>(sa[u], sb[u], ix, iy, iz, fx, fy, fz, e);
-                acc.add(0, fx); acc.add(1, fy); acc.add(2, fz); acc.add(3, e);
+                acc.add(0, fx); acc.add(1, fy); acc.add(2, fz);
+                if (E) acc.add(3, e);
             }
-            acc.end();
+            acc.template end<E>();
         }
     }
     if (!IMG) acc.begin();
@@ -247,35 +251,36 @@ __device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<G>& s, int n, int
             }
             float fx, fy, fz, e;
             pair32<SEG>(sa, sb, ix, iy, iz, fx, fy, fz, e);
-            acc.add(0, fx); acc.add(1, fy); acc.add(2, fz); acc.add(3, e);
+            acc.add(0, fx); acc.add(1, fy); acc.add(2, fz);
+            if (E) acc.add(3, e);
         }
-        if (IMG) acc.end();
+        if (IMG) acc.template end<E>();
     }
-    if (!IMG) acc.end();
+    if (!IMG) acc.template end<E>();
 }
 
-template <int G, int MODE, bool IMG>
+template <int G, int MODE, bool IMG, bool E>
 __device__ inline void seg_sums32(cptr<DevMol> M, const Sites32<G> s0, float tx, float ty, float tz, int gl, int S,
                                   Acc32<MODE>& acc) {
     Sites32<G> s = s0;
     int n = M->n_seg[0];
-    seg_sum32<0, G, MODE, IMG>(M, s, n, gl, S, tx, ty, tz, acc);
+    seg_sum32<0, G, MODE, IMG, E>(M, s, n, gl, S, tx, ty, tz, acc);
     s.advance(n);
     n = M->n_seg[1];
-    seg_sum32<1, G, MODE, IMG>(M, s, n, gl, S, tx, ty, tz, acc);
+    seg_sum32<1, G, MODE, IMG, E>(M, s, n, gl, S, tx, ty, tz, acc);
     s.advance(n);
     n = M->n_seg[2];
-    if (n) seg_sum32<2, G, MODE, IMG>(M, s, n, gl, S, tx, ty, tz, acc);
+    if (n) seg_sum32<2, G, MODE, IMG, E>(M, s, n, gl, S, tx, ty, tz, acc);
     s.advance(n);
     n = M->n_seg[3];
-    if (n) seg_sum32<3, G, MODE, IMG>(M, s, n, gl, S, tx, ty, tz, acc);
+    if (n) seg_sum32<3, G, MODE, IMG, E>(M, s, n, gl, S, tx, ty, tz, acc);
 }
 
 // Sum of all tip-surface site terms of molecule M at tip position (px,py,pz):
 // this lane's share, sites gl, gl+S, ... of each segment (every image), S
 // lanes per position.  Coordinates are relative to the molecule's origin so
 // FP32 keeps ~1e-6 A resolution.
-template <int G, int MODE>
+template <int G, int MODE, bool E>
 __device__ void site_sums32(cptr<DevMol> M, const Sites32<G> s0, double px, double py, double pz, int gl, int S,
                             double out[4]) {
     Acc32<MODE> acc;
@@ -284,13 +289,13 @@ __device__ void site_sums32(cptr<DevMol> M, const Sites32<G> s0, double px, doub
     const float ty = (float)(py - M->origin[1]);
     const float tz = (float)(pz - M->origin[2]);
     if (M->n_images > 1)
-        seg_sums32<G, MODE, true>(M, s0, tx, ty, tz, gl, S, acc);
+        seg_sums32<G, MODE, true, E>(M, s0, tx, ty, tz, gl, S, acc);
     else
-        seg_sums32<G, MODE, false>(M, s0, tx, ty, tz, gl, S, acc);
+        seg_sums32<G, MODE, false, E>(M, s0, tx, ty, tz, gl, S, acc);
     out[0] = acc.get(0);
     out[1] = acc.get(1);
     out[2] = acc.get(2);
-    out[3] = acc.get(3) * (1.0 / 6.0);
+    out[3] = E ? acc.get(3) * (1.0 / 6.0) : 0.0;
 }
 
 // The FP64 twin of pair32 (same pre-scaled constants, SiteF64::p3 = 2 De a).
@@ -338,7 +343,7 @@ __device__ inline void pair64(const SiteF64& st, double tx, double ty, double tz
     fz = fs * dz;
 }
 
-template <int SEG, int G>
+template <int SEG, int G, bool E>
 __device__ inline void seg_sum64(cptr<DevMol> M, const Sites<G, SiteF64>& s, int n, int gl, int S, double px,
                                  double py, double pz, double acc[4]) {
     const int n_img = M->n_images;
@@ -353,7 +358,8 @@ __device__ inline void seg_sum64(cptr<DevMol> M, const Sites<G, SiteF64>& s, int
             for (int u = 0; u < 4; ++u) {
                 double fx, fy, fz, e;
                 pair64<SEG>(st[u], tx, ty, tz, fx, fy, fz, e);
-                acc[0] += fx; acc[1] += fy; acc[2] += fz; acc[3] += e;
+                acc[0] += fx; acc[1] += fy; acc[2] += fz;
+                if (E) acc[3] += e;
             }
         }
     }
@@ -363,27 +369,28 @@ __device__ inline void seg_sum64(cptr<DevMol> M, const Sites<G, SiteF64>& s, int
             const double tx = px - M->shift[im][0], ty = py - M->shift[im][1], tz = pz - M->shift[im][2];
             double fx, fy, fz, e;
             pair64<SEG>(st, tx, ty, tz, fx, fy, fz, e);
-            acc[0] += fx; acc[1] += fy; acc[2] += fz; acc[3] += e;
+            acc[0] += fx; acc[1] += fy; acc[2] += fz;
+            if (E) acc[3] += e;
         }
     }
 }
 
-template <int G>
+template <int G, bool E>
 __device__ void site_sums64(cptr<DevMol> M, const Sites<G, SiteF64> s0, double px, double py, double pz, int gl,
                             int S, double out[4]) {
     double acc[4] = {0, 0, 0, 0};
     Sites<G, SiteF64> s = s0;
     int n = M->n_seg[0];
-    seg_sum64<0, G>(M, s, n, gl, S, px, py, pz, acc);
+    seg_sum64<0, G, E>(M, s, n, gl, S, px, py, pz, acc);
     s.advance(n);
     n = M->n_seg[1];
-    seg_sum64<1, G>(M, s, n, gl, S, px, py, pz, acc);
+    seg_sum64<1, G, E>(M, s, n, gl, S, px, py, pz, acc);
     s.advance(n);
     n = M->n_seg[2];
-    if (n) seg_sum64<2, G>(M, s, n, gl, S, px, py, pz, acc);
+    if (n) seg_sum64<2, G, E>(M, s, n, gl, S, px, py, pz, acc);
     s.advance(n);
     n = M->n_seg[3];
-    if (n) seg_sum64<3, G>(M, s, n, gl, S, px, py, pz, acc);
+    if (n) seg_sum64<3, G, E>(M, s, n, gl, S, px, py, pz, acc);
     out[0] = acc[0];
     out[1] = acc[1];
     out[2] = acc[2];
@@ -586,6 +593,7 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
     bool ts = false;
     int64_t steps = 0, unconv = 0, warns = 0, oogs = 0;
     const double dt_max = 10 * a.dt;
+    const bool align = a.minimiser != 0 && a.integrator == 1;
 
     for (;;) {
         // ---- refill: groups without a column steal one from molecule m
@@ -654,8 +662,17 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
         }
 
         // ---- one evaluation of the interaction list at P for every group
+        // FIRE + midpoint: every group of the wave runs half-steps (ph 0) on
+        // even trips, full steps (ph 1) and final evaluations on odd trips; a
+        // group out of step waits one trip (once per height).  The wave then
+        // runs one branch of the state machine per trip, and even trips skip
+        // the energy sums, which a half-step does not use.
+        const bool odd = trips & 1;
         ++trips;
-        active += col >= 0;
+        const bool stall = align && col >= 0 && (odd ? ph == 0 : ph == PH_OUT);
+        const bool live = col >= 0 && !stall;
+        const bool need_e = !align || (odd && (a.minterm != 1 || __any(live && ph == PH_OUT)));
+        active += live;
         cptr<DevMol> M = as_const(a.mols) + m;
         double r[4];
         if (M->force_grid >= 0) {
@@ -666,7 +683,7 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
             // dry): spread the n_act active positions over all 64 lanes,
             // Ge = 64 / pow2ceil(n_act) lanes each, when that cuts the sites
             // per lane by at least wide_tail (a.wide_tail = 0: never).
-            const uint64_t lead_act = __ballot(leader && col >= 0);
+            const uint64_t lead_act = __ballot(leader && live);
             const int n_act = __popcll(lead_act);
             int p2 = 1;
             while (p2 < n_act) p2 <<= 1;
@@ -675,21 +692,23 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
             if (wide) {
                 ++wide_trips;
                 const int slot = lane / Ge, sub = lane & (Ge - 1);
-                const bool live = slot < n_act;
-                const int src = live ? nth_set_bit(lead_act, slot) : 0;
+                const bool slot_live = slot < n_act;
+                const int src = slot_live ? nth_set_bit(lead_act, slot) : 0;
                 const double qx = __shfl(Px, src, 64), qy = __shfl(Py, src, 64), qz = __shfl(Pz, src, 64);
                 double part[4] = {0, 0, 0, 0};
-                if (live) {
+                if (slot_live) {
                     if constexpr (F32) {
                         // G == 1 keeps the sites in global memory: wide loads read them there
                         const float4* wa = G > 1 ? lds_raw : a.seg32a + M->seg_off;
                         const float4* wb = G > 1 ? lds_raw + lds_n : a.seg32b + M->seg_off;
                         Sites32<64> src64{wa, wb, wa, wb};
-                        site_sums32<64, COMP>(M, src64, qx, qy, qz, sub, Ge, part);
+                        if (need_e) site_sums32<64, COMP, true>(M, src64, qx, qy, qz, sub, Ge, part);
+                        else site_sums32<64, COMP, false>(M, src64, qx, qy, qz, sub, Ge, part);
                     } else {
                         const SiteF64* w = G > 1 ? reinterpret_cast<const SiteF64*>(lds_raw) : a.segf64 + M->seg_off;
                         Sites<64, SiteF64> src64{w, w};
-                        site_sums64<64>(M, src64, qx, qy, qz, sub, Ge, part);
+                        if (need_e) site_sums64<64, true>(M, src64, qx, qy, qz, sub, Ge, part);
+                        else site_sums64<64, false>(M, src64, qx, qy, qz, sub, Ge, part);
                     }
                 }
                 for (int w2 = 1; w2 < Ge; w2 <<= 1)
@@ -701,10 +720,12 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
             } else {
                 if constexpr (F32) {
                     Sites32<G> src{a.seg32a + M->seg_off, a.seg32b + M->seg_off, lds_raw, lds_raw + lds_n};
-                    site_sums32<G, COMP>(M, src, Px, Py, Pz, gl, G, r);
+                    if (need_e) site_sums32<G, COMP, true>(M, src, Px, Py, Pz, gl, G, r);
+                    else site_sums32<G, COMP, false>(M, src, Px, Py, Pz, gl, G, r);
                 } else {
                     Sites<G, SiteF64> src{a.segf64 + M->seg_off, reinterpret_cast<const SiteF64*>(lds_raw)};
-                    site_sums64<G>(M, src, Px, Py, Pz, gl, G, r);
+                    if (need_e) site_sums64<G, true>(M, src, Px, Py, Pz, gl, G, r);
+                    else site_sums64<G, false>(M, src, Px, Py, Pz, gl, G, r);
                 }
                 if (G > 1) {
                     for (int q = 0; q < 4; ++q) r[q] = group_sum<G>(r[q]);
@@ -714,7 +735,7 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
         }
         if (!ts) tip_dummy(M, dmx, dmy, dmz, Px, Py, Pz, r);
 
-        if (col < 0) continue;
+        if (!live) continue;
 
         // ---- state machine (FP64)
         bool step_done = false;
@@ -853,12 +874,14 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
     }
 }
 
+// Both variants hold 200+ live registers; the second launch bound asks for
+// two waves per SIMD (left free, the compiler can take all 256 VGPRs and drop
+// to one wave per SIMD).
 template <int G, int COMP>
-__global__ void __launch_bounds__(256) k_scan_fast32(ScanArgs a) {
+__global__ void __launch_bounds__(256, 2) k_scan_fast32(ScanArgs a) {
     scan_fast_body<G, true, COMP>(a);
 }
 
-// FP64 pair terms hold more live registers: ask for two waves per SIMD
 template <int G>
 __global__ void __launch_bounds__(256, 2) k_scan_fast64(ScanArgs a) {
     scan_fast_body<G, false, 0>(a);
