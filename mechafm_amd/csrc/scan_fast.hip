@@ -457,37 +457,43 @@ __device__ void grid_lookup(const DevGrid& g, double px, double py, double pz, d
 }
 
 // Tip-dummy VDW, Coulomb and lateral spring on the tip (simulation.cpp:503-515,
-// interactions.cpp:15-51, 252-266), FP64.
+// interactions.cpp:15-51, 252-266), FP64 with Newton-refined hardware
+// reciprocals in place of divisions and square roots.
 __device__ void tip_dummy(cptr<DevMol> M, double dx_, double dy_, double dz_, double tx, double ty, double tz,
                           double out[4]) {
     const double rx = dx_ - tx, ry = dy_ - ty, rz = dz_ - tz;
     const double r2 = rx * rx + ry * ry + rz * rz;
+    const double ir2 = rcp64(r2);
     double s, e;
     if (M->td_vdw == 0) {
-        double r6 = r2 * r2 * r2;
-        double ta = M->td_p1 / (r6 * r6);
-        double tb = M->td_p0 / r6;
+        const double ir6 = ir2 * ir2 * ir2;
+        const double ta = M->td_p1 * ir6 * ir6;
+        const double tb = M->td_p0 * ir6;
         e = ta - tb;
-        s = (12 * ta - 6 * tb) / r2;
+        s = (12 * ta - 6 * tb) * ir2;
     } else {
-        double r = sqrt(r2);
-        double x = exp(-M->td_p1 * (r - M->td_p2));
+        const double ir = rsq64(r2);
+        const double x = exp(-M->td_p1 * (r2 * ir - M->td_p2));
         e = M->td_p0 * (x * x - 2 * x + 1);
-        s = 2 * M->td_p0 * M->td_p1 * (x * x - x) / r;
+        s = 2 * M->td_p0 * M->td_p1 * (x * x - x) * ir;
     }
     if (M->td_coul) {
-        double r = sqrt(r2);
-        e += M->td_qq / r;
-        s += M->td_qq / (r * r * r);
+        const double ir = rsq64(r2);
+        e += M->td_qq * ir;
+        s += M->td_qq * ir * ir2;
     }
     double fx = -s * rx, fy = -s * ry, fz = -s * rz;
-    const double rl = sqrt(rx * rx + ry * ry);
-    const double dr = rl - M->td_r0;
-    e += M->td_k * dr * dr;
-    if (rl > 1e-10) {
-        const double h = -2 * M->td_k * dr / rl;
+    const double rl2 = rx * rx + ry * ry;
+    if (rl2 > 1e-20) {
+        const double irl = rsq64(rl2);
+        const double dr = rl2 * irl - M->td_r0;
+        e += M->td_k * dr * dr;
+        const double h = -2 * M->td_k * dr * irl;
         fx -= h * rx;
         fy -= h * ry;
+    } else {
+        const double dr = sqrt(rl2) - M->td_r0;
+        e += M->td_k * dr * dr;
     }
     out[0] += fx; out[1] += fy; out[2] += fz; out[3] += e;
 }
@@ -513,12 +519,14 @@ __device__ inline double group_sum(double v) {
     return v;
 }
 
+// checkConvergence (minimiser.cpp:15-37); |F| < ftol tested as |F|^2 < ftol^2
 __device__ inline bool converged(const ScanArgs& a, double fx, double fy, double fz, double ediff) {
-    const double fl = sqrt(fx * fx + fy * fy + fz * fz);
+    const double f2 = fx * fx + fy * fy + fz * fz;
+    const double t2 = a.ftol * a.ftol;
     switch (a.minterm) {
         case 0: return fabs(ediff) < a.etol;
-        case 1: return fl < a.ftol;
-        default: return fabs(ediff) < a.etol && fl < a.ftol;
+        case 1: return f2 < t2;
+        default: return fabs(ediff) < a.etol && f2 < t2;
     }
 }
 
@@ -835,10 +843,14 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
             ph = PH_OUT;
         } else {
             const double p = vx * fx + vy * fy + vz * fz;
-            const double fl = sqrt(fx * fx + fy * fy + fz * fz);
+            const double f2 = fx * fx + fy * fy + fz * fz;
             double hx = 0, hy = 0, hz = 0;
-            if (fl > 1e-10) { hx = fx / fl; hy = fy / fl; hz = fz / fl; }
-            const double vl = sqrt(vx * vx + vy * vy + vz * vz);
+            if (f2 > 1e-20) {                          // |f| > 1e-10 (vectors.hpp:107-114)
+                const double ifl = rsq64(f2);
+                hx = fx * ifl; hy = fy * ifl; hz = fz * ifl;
+            }
+            const double v2 = vx * vx + vy * vy + vz * vz;
+            const double vl = v2 > 0 ? v2 * rsq64(v2) : 0.0;
             vx = (1 - alpha) * vx + alpha * hx * vl;
             vy = (1 - alpha) * vy + alpha * hy * vl;
             vz = (1 - alpha) * vz + alpha * hz * vl;
