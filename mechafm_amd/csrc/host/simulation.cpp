@@ -603,6 +603,31 @@ void Simulation::addCoulombInteraction(int i1, int i2) {   // simulation.cpp:269
     }
 }
 
+// DataGrid::rotateCoordAxes("ZXY") (data_grid.cpp:49-70): new axes (z, x, y)
+// of the old ones: counts, voxel vectors (rows and columns permuted, i.e.
+// S^T B S), origin and the x-major / z-fastest values.
+void rotateCubeZXY(Cube& c) {
+    const int o[3] = {c.n[0], c.n[1], c.n[2]};
+    const int perm[3] = {2, 0, 1};   // new axis a = old axis perm[a]
+    Vec3d vox[3];
+    for (int a = 0; a < 3; ++a) {
+        const Vec3d& v = c.voxel[perm[a]];
+        vox[a] = Vec3d(v.z, v.x, v.y);
+    }
+    const int n[3] = {o[2], o[0], o[1]};
+    std::vector<double> out(c.values.size());
+    for (int ix = 0; ix < n[0]; ++ix)
+        for (int iy = 0; iy < n[1]; ++iy)
+            for (int iz = 0; iz < n[2]; ++iz)
+                out[((size_t)ix * n[1] + iy) * n[2] + iz] = c.values[((size_t)iy * o[1] + iz) * o[2] + ix];
+    c.values.swap(out);
+    for (int a = 0; a < 3; ++a) {
+        c.n[a] = n[a];
+        c.voxel[a] = vox[a];
+    }
+    c.origin = Vec3d(c.origin.z, c.origin.x, c.origin.y);
+}
+
 ForceGrid Simulation::buildElectrostaticPotential() {      // simulation.cpp:323-417
     Cube cube = readCube(options_.e_potential_file, true);
     double scale;
@@ -610,9 +635,13 @@ ForceGrid Simulation::buildElectrostaticPotential() {      // simulation.cpp:323
     else if (options_.units == U_KJ) scale = -HARTREE_TO_KJ;
     else scale = -HARTREE_TO_KCAL;
     for (double& v : cube.values) v *= scale;
-    if (options_.normal != NORMAL_Z)
-        fail(MAFM_ERR_INVALID, "an external potential needs surface_normal z (the reference's grid rotation for "
-                               "x asks for an axis order it does not implement)");
+    // surface_normal y: the grid takes the system's ZXY axis order
+    // (simulation.cpp:358-362, data_grid.cpp:49-70); for x the reference asks
+    // for ZYX, which DataGrid::rotateCoordAxes does not implement (it throws).
+    if (options_.normal == NORMAL_X)
+        fail(MAFM_ERR_INVALID, "an external potential with surface_normal x needs the grid axis order ZYX, which "
+                               "the reference does not implement");
+    if (options_.normal == NORMAL_Y) rotateCubeZXY(cube);
     mafm_potential_desc pd{};
     ForceGrid g;
     for (int a = 0; a < 3; ++a) {

@@ -251,6 +251,7 @@ struct Cube {
     std::vector<double> values;   // x-major, z fastest
 };
 Cube readCube(const std::string& path, bool with_values);
+void rotateCubeZXY(Cube& c);
 
 class Simulation {
   public:

@@ -294,19 +294,25 @@ def write_input(path: str, opts: Dict[str, str]) -> None:
 
 
 def write_cube(path: str, values: np.ndarray, cell: Sequence[float], mol: Molecule,
-               origin=(0.0, 0.0, 0.0)) -> None:
-    """Cube file in Angstrom units (negative voxel counts on all three axes,
-    cube_io.cpp:107-118), x-major / z-fastest values, 6 per line."""
+               origin=(0.0, 0.0, 0.0), voxels: Optional[np.ndarray] = None, bohr: bool = False) -> None:
+    """Cube file, x-major / z-fastest values, 6 per line.  Voxel vectors are
+    ``voxels`` (rows) or the orthorhombic cell / counts.  Angstrom units are
+    flagged by negative voxel counts on all three axes; ``bohr`` writes
+    positive counts and every length in Bohr (cube_io.cpp:107-118)."""
     nx, ny, nz = values.shape
-    vx, vy, vz = cell[0] / nx, cell[1] / ny, cell[2] / nz
-    lines = ["synthetic Hartree potential (Gaussian charges, FFT Poisson)",
-             "units: Angstrom, Hartree"]
-    lines.append(f"{len(mol.types):5d} {origin[0]:12.6f} {origin[1]:12.6f} {origin[2]:12.6f}")
-    lines.append(f"{-nx:5d} {vx:12.6f} {0.0:12.6f} {0.0:12.6f}")
-    lines.append(f"{-ny:5d} {0.0:12.6f} {vy:12.6f} {0.0:12.6f}")
-    lines.append(f"{-nz:5d} {0.0:12.6f} {0.0:12.6f} {vz:12.6f}")
+    vox = np.diag([cell[0] / nx, cell[1] / ny, cell[2] / nz]) if voxels is None else np.asarray(voxels, float)
+    scale = 1.0 / BOHR if bohr else 1.0
+    sign = 1 if bohr else -1
+    o = np.asarray(origin, float) * scale
+    lines = ["synthetic Hartree potential",
+             "units: " + ("Bohr" if bohr else "Angstrom") + ", Hartree"]
+    lines.append(f"{len(mol.types):5d} {o[0]:14.8f} {o[1]:14.8f} {o[2]:14.8f}")
+    for a, n in enumerate((nx, ny, nz)):
+        v = vox[a] * scale
+        lines.append(f"{sign * n:5d} {v[0]:14.8f} {v[1]:14.8f} {v[2]:14.8f}")
     for t, p in zip(mol.types, mol.pos):
-        lines.append(f"{ATOMIC_NUMBER.get(t, 0):5d} {0.0:12.6f} {p[0]:12.6f} {p[1]:12.6f} {p[2]:12.6f}")
+        q = np.asarray(p) * scale
+        lines.append(f"{ATOMIC_NUMBER.get(t, 0):5d} {0.0:12.6f} {q[0]:14.8f} {q[1]:14.8f} {q[2]:14.8f}")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
         flat = values.reshape(nx * ny, nz)
@@ -419,6 +425,74 @@ def make_c5(outdir: str, mini: bool = False, seed: int = 1239, **over) -> Worklo
         opts.update(area="0.3 0.3")
     opts.update({k: str(v) for k, v in over.items()})
     return _emit("c5-mini" if mini else "c5", outdir, mol, opts, True)
+
+
+# File frame of a system whose surface normal is x or y: the reference rotates
+# positions to (y, z, x) for x and (z, x, y) for y (System::rotateCoordAxes,
+# system.cpp:68-89), so a system built with its normal along z is written with
+# these column orders to come back unchanged.
+FILE_COLUMNS = {"x": [2, 0, 1], "y": [1, 2, 0]}
+
+
+def rotated_molecule(mol: Molecule, normal: str) -> Molecule:
+    return Molecule(list(mol.types), mol.pos[:, FILE_COLUMNS[normal]].copy(),
+                    None if mol.charges is None else mol.charges.copy())
+
+
+def make_c1_normal(outdir: str, normal: str = "x", **over) -> Workload:
+    """The c1 flake written with its normal along x or y (surface_normal)."""
+    mol = rotated_molecule(hex_flake(4), normal)
+    opts = _base_options(area="0.5 0.5", surface_normal=normal)
+    opts.update({k: str(v) for k, v in over.items()})
+    return _emit(f"c1-normal-{normal}", outdir, mol, opts, False)
+
+
+def periodic_test_potential(n_grid: Sequence[int], amp: float = 0.004) -> np.ndarray:
+    """A smooth periodic potential in fractional coordinates (a few low
+    Fourier modes, Hartree), usable on any cell shape."""
+    u, v, w = (np.arange(n) / n for n in n_grid)
+    U, V, Wz = np.meshgrid(u, v, w, indexing="ij")
+    tp = 2.0 * math.pi
+    return amp * (np.cos(tp * U) + 0.7 * np.sin(tp * (V + Wz)) + 0.5 * np.cos(tp * (U - V) + 0.3)
+                  + 0.4 * np.cos(tp * 2 * Wz + 1.1))
+
+
+def make_c4_variant(outdir: str, variant: str, seed: int = 1238, **over) -> Workload:
+    """c4-mini variants that reach the remaining branches of the cube and
+    external-potential path:
+
+    * ``bohr``     -- the Hartree cube written in Bohr (positive voxel counts);
+    * ``skew``     -- a non-orthogonal in-plane cell (basis^-1 branches of the
+                      Gaussian placement, the k-vectors and the grid lookup);
+    * ``normal_y`` -- slab and cube written with the surface normal along y
+                      (the ZXY rotation of positions and potential grid)."""
+    rng = np.random.default_rng(seed)
+    cell, n_atoms, n_grid, thick = (6.0, 6.0, 16.0), 45, (24, 24, 64), 4.0
+    mol = random_slab(n_atoms, cell[:2], thick, rng)
+    opts = _base_options(area="0.8 0.8", dx="0.2", dy="0.2", zhigh="14.4", zlow="10.5",
+                         vdw_pbc="on", use_external_potential="on", e_potential_file="hartree.cube")
+    os.makedirs(outdir, exist_ok=True)
+    path = os.path.join(outdir, "hartree.cube")
+    if variant == "bohr":
+        pot = gaussian_hartree_potential(mol, cell, n_grid)
+        write_cube(path, pot, cell, mol, bohr=True)
+    elif variant == "skew":
+        vox = np.array([[6.0, 0.0, 0.0], [1.5, 5.8, 0.0], [0.0, 0.0, 16.0]]) / np.array(n_grid)[:, None]
+        pot = periodic_test_potential(n_grid)
+        write_cube(path, pot, cell, mol, voxels=vox)
+    elif variant == "normal_y":
+        pot = gaussian_hartree_potential(mol, cell, n_grid)
+        cols = FILE_COLUMNS["y"]
+        # file axis a holds system axis cols[a]: values_f[i, j, k] = pot[k, i, j]
+        pot_f = np.transpose(pot, (1, 2, 0)).copy()
+        cell_f = [cell[c] for c in cols]
+        mol = rotated_molecule(mol, "y")
+        write_cube(path, pot_f, cell_f, mol)
+        opts["surface_normal"] = "y"
+    else:
+        raise ValueError(variant)
+    opts.update({k: str(v) for k, v in over.items()})
+    return _emit(f"c4-{variant}", outdir, mol, opts, False)
 
 
 MAKERS = {"c1": make_c1, "c2": make_c2, "c3": make_c3, "c4": make_c4, "c5": make_c5}

@@ -574,8 +574,15 @@ def setup(input_path: str) -> Setup:
         if o.use_external_potential:
             cb = read_cube(o.e_potential_file)
             pot = cb.values * (-HARTREE[o.units])          # :348-355
-            origin = cb.origin + s.offset                  # :365
-            g = electrostatic_force_grid(pot, cb.vox.T.copy(), origin, float(s.charges[1]))
+            vox, corner = cb.vox, cb.origin
+            if o.normal == "x":                            # :358-359: "ZYX" throws (data_grid.cpp:93-95)
+                raise ValueError("rotateCoordAxes(ZYX) is not implemented for data grids")
+            if o.normal == "y":                            # DataGrid::rotateCoordAxes("ZXY") (data_grid.cpp:49-70)
+                pot = np.transpose(pot, (2, 0, 1)).copy()
+                vox = vox[[2, 0, 1]][:, [2, 0, 1]]
+                corner = corner[[2, 0, 1]]
+            origin = corner + s.offset                     # :365
+            g = electrostatic_force_grid(pot, vox.T.copy(), origin, float(s.charges[1]))
             grids.append(g)
             lst.append({"kind": OR_GRID, "grid": len(grids) - 1})
         return lst
