@@ -172,7 +172,7 @@ def main():
         G, wps = args.lanes, args.waves
         options = abi.SCAN_REPRODUCIBLE if args.reproducible else 0
         if args.precision != "plain64" and not args.no_tune and (G == 0 or wps == 0):
-            cands = [(g, w) for g in ([G] if G else [1, 2, 4]) for w in ([wps] if wps else [1, 2])]
+            cands = [(g, w) for g in ([G] if G else [1, 2, 4, 8, 16]) for w in ([wps] if wps else [1, 2])]
             G, wps, _ = autotune(abi, sims, args.precision, cands, args.compensation, args.group, options)
         kw = dict(lanes_per_pixel=G, waves_per_simd=wps, compensation=args.compensation,
                   waves_per_group=args.group, options=options)

@@ -20,7 +20,7 @@ hipError_t launch_eval_plain(const ScanArgs& a, int mol, long n, const double* p
 hipError_t launch_grid_build(const ScanArgs& a, int mol, const int* n, const double* s, const double* o,
                              double4* out, hipStream_t st);
 hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_blocks, int W, int lds_sites,
-                            hipStream_t st);
+                            bool scalar_sites, hipStream_t st);
 hipError_t launch_to_f32(const double4* in, float4* out, long n, hipStream_t st);
 bool fft_length_supported(int n);
 void fft_twiddles(int n, std::vector<double>& out);
@@ -533,6 +533,7 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
     const long cols_per_mol = (long)d->nx * d->ny;
     const long total_cols = cols_per_mol * n_mols;
     int G = 1, wps = 0, n_waves = 0, lds_sites = 0, W = 1, n_blocks = 0;
+    bool scalar_sites = false;
     if (d->precision != MAFM_PREC_PLAIN64) {
         wps = d->waves_per_simd > 0 ? d->waves_per_simd : 2;
         const long lanes = (long)c->cus * 4 * wps * 64;
@@ -546,21 +547,23 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
         }
         if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16 && G != 32 && G != 64)
             return fail(c, MAFM_ERR_INVALID, "lanes_per_pixel must be a power of two <= 64");
-        // G > 1 stages the largest molecule's sites in one wave's LDS
+        // a workgroup stages its molecule's sites in LDS (64 KiB at most); larger
+        // molecules are streamed with scalar loads, one lane per pixel
         const size_t site_bytes = d->precision == MAFM_PREC_FAST32 ? sizeof(Site32) : sizeof(SiteF64);
-        if (G > 1 && (size_t)max_sites * site_bytes > 64 * 1024) {
+        scalar_sites = (size_t)max_sites * site_bytes > 64 * 1024;
+        if (scalar_sites && G > 1) {
             if (d->lanes_per_pixel > 0)
                 return fail(c, MAFM_ERR_INVALID, "lanes_per_pixel > 1 needs the sites of one molecule to fit 64 KiB of LDS");
             G = 1;
         }
         lds_sites = (int)max_sites;
-        // waves per workgroup: G > 1 workgroups share one LDS copy of the sites;
-        // grow W until the LDS a CU holds admits wps waves per SIMD
+        // waves per workgroup share one LDS copy of the sites: grow W until the
+        // LDS a CU holds admits wps waves per SIMD
         if (d->waves_per_group > 0) {
             W = d->waves_per_group;
             if (W != 1 && W != 2 && W != 4)
                 return fail(c, MAFM_ERR_INVALID, "waves_per_group must be 1, 2 or 4");
-        } else if (G > 1) {
+        } else if (!scalar_sites) {
             const size_t lds_bytes = std::max<size_t>((size_t)max_sites * site_bytes, 1);
             const long blocks_per_cu = (long)(c->lds_per_cu / lds_bytes);
             while (W < 4 && blocks_per_cu * W < 4L * wps) W <<= 1;
@@ -595,7 +598,8 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
     } else {
         // 0 / 3: block-compensated (default), 1: Kahan, 2: plain FP32
         const int comp = d->compensation == 1 ? 1 : (d->compensation == 2 ? 0 : 2);
-        HIPCHK(c, launch_scan_fast(a, d->precision == MAFM_PREC_FAST32, comp, G, n_blocks, W, lds_sites, c->stream));
+        HIPCHK(c, launch_scan_fast(a, d->precision == MAFM_PREC_FAST32, comp, G, n_blocks, W, lds_sites, scalar_sites,
+                                   c->stream));
     }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     int64_t counters[8];

@@ -1,16 +1,18 @@
 // Fast scan: persistent, work-stealing, one force evaluation per loop trip.
 //
 // Work decomposition
-//   * A wave (64 lanes, one wave per workgroup) owns one molecule at a time;
-//     the molecule index is wave-uniform, so its DevMol record and, for G == 1,
-//     every site it streams are read with scalar loads into SGPRs and fed to
-//     the VALU as broadcast operands (no LDS, no bank conflicts, no shuffles).
+//   * A workgroup of W waves (W = 1, 2, 4) owns one molecule at a time and
+//     stages its sites once in LDS; the molecule index is wave-uniform, so its
+//     DevMol record is read with scalar loads into SGPRs.  A molecule too
+//     large for LDS is streamed through scalar loads instead (G = 1).
 //   * G lanes form a pixel group; the G lanes split the molecule's sites and
-//     combine their partial sums with a butterfly of lane shuffles.  G = 1 for
-//     large scans, G > 1 lets small scans (few pixel columns) fill the chip.
+//     combine their partial sums with a butterfly of lane shuffles.  Groups
+//     of a wave read the same sites (LDS broadcast).
 //   * Each group steals pixel columns from its molecule's atomic counter
 //     (one atomic per wave per refill, ballot + prefix count).  When a
-//     molecule runs dry the wave moves to the molecule with work left.
+//     molecule runs dry the workgroup moves to the molecule with work left.
+//     A wave whose groups have mostly finished regroups the remaining
+//     positions over all 64 lanes.
 //   * The minimiser / integrator of simulation.cpp:102-122 runs as a per-group
 //     state machine in registers.  Every trip of the main loop performs exactly
 //     one evaluation of the interaction list per group: an integrator
@@ -47,9 +49,11 @@ __device__ inline double rsq64(double x) {
     return y;
 }
 
-// Site source of an evaluation: G == 1 streams the molecule's table with
-// scalar loads (every lane needs the same site), G > 1 reads the copy staged
-// in the wave's LDS (the G lanes of a pixel need G different sites).
+// Site source of an evaluation: the copy of the molecule's table staged in
+// the workgroup's LDS (the G lanes of a pixel read G different sites, groups
+// read the same ones: broadcast), or, SC = true, for a molecule too large for
+// LDS with one lane per pixel, the table itself through scalar loads (every
+// lane needs the same site).
 template <typename T>
 __device__ inline T load_const(const T* p) {
     static_assert(sizeof(T) % 4 == 0, "dword-sized records only");
@@ -61,12 +65,12 @@ __device__ inline T load_const(const T* p) {
     return out;
 }
 
-template <int G, typename S>
+template <bool SC, typename S>
 struct Sites {
-    const S* g;        // global table of the molecule
-    const S* l;        // LDS copy (G > 1)
+    const S* g;        // global table of the molecule (SC)
+    const S* l;        // LDS copy
     __device__ inline S operator[](int t) const {
-        if (G == 1) return load_const(g + t);
+        if (SC) return load_const(g + t);
         return l[t];
     }
     __device__ inline void advance(int n) {
@@ -78,20 +82,20 @@ struct Sites {
 // FP32 sites as two arrays so a pair reads only what it needs: A = {x, y, z,
 // 6 es6 | 6 De} (16 B) and B = {6 es12 | a log2(e), qq, - | re, - | 2 De a};
 // an LJ pair reads 20 B, LJ + Coulomb 24 B, Morse 32 B.
-template <int G>
+template <bool SC>
 struct Sites32 {
     const float4* ga;
     const float4* gb;
     const float4* la;
     const float4* lb;
-    __device__ inline float4 A(int t) const { return G == 1 ? load_const(ga + t) : la[t]; }
+    __device__ inline float4 A(int t) const { return SC ? load_const(ga + t) : la[t]; }
     __device__ inline float B1(int t) const {
-        return G == 1 ? load_const(reinterpret_cast<const float*>(gb + t)) : reinterpret_cast<const float*>(lb + t)[0];
+        return SC ? load_const(reinterpret_cast<const float*>(gb + t)) : reinterpret_cast<const float*>(lb + t)[0];
     }
     __device__ inline float2 B2(int t) const {
-        return G == 1 ? load_const(reinterpret_cast<const float2*>(gb + t)) : reinterpret_cast<const float2*>(lb + t)[0];
+        return SC ? load_const(reinterpret_cast<const float2*>(gb + t)) : reinterpret_cast<const float2*>(lb + t)[0];
     }
-    __device__ inline float4 B4(int t) const { return G == 1 ? load_const(gb + t) : lb[t]; }
+    __device__ inline float4 B4(int t) const { return SC ? load_const(gb + t) : lb[t]; }
     __device__ inline void advance(int n) {
         ga += n; gb += n; la += n; lb += n;
     }
@@ -139,8 +143,8 @@ struct Acc32 {
 
 // One site of segment SEG in registers: a = {x, y, z, 6 es6 | 6 De}, b = {6 es12 |
 // a log2(e), qq, - | re, - | 2 De a}; only the B words the segment uses are read.
-template <int SEG, int G>
-__device__ inline void load_site(const Sites32<G>& s, int t, float4& a, float4& b) {
+template <int SEG, bool SC>
+__device__ inline void load_site(const Sites32<SC>& s, int t, float4& a, float4& b) {
     a = s.A(t);
     if (SEG == 1) {
         b.x = s.B1(t);
@@ -210,15 +214,15 @@ __device__ inline void pair32(const float4 a, const float4 b, float tx, float ty
 // t - shift[im] in FP32 relative to the molecule's origin.  IMG = false is
 // the single zero-shift image of a non-periodic system; E = false skips the
 // energy (the compiler drops its arithmetic).
-template <int SEG, int G, int MODE, bool IMG, bool E>
-__device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<G>& s, int n, int gl, int S, float tx, float ty,
+template <int SEG, bool SC, int MODE, bool IMG, bool E>
+__device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<SC>& s, int n, int gl, int S, float tx, float ty,
                                  float tz, Acc32<MODE>& acc) {
     const int n_img = IMG ? M->n_images : 1;
     const int full = (n / (8 * S)) * (8 * S);
     for (int t0 = 0; t0 < full; t0 += 8 * S) {
         float4 sa[8], sb[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) load_site<SEG, G>(s, t0 + u * S + gl, sa[u], sb[u]);
+        for (int u = 0; u < 8; ++u) load_site<SEG, SC>(s, t0 + u * S + gl, sa[u], sb[u]);
         for (int im = 0; im < n_img; ++im) {
             float ix = tx, iy = ty, iz = tz;
             if (IMG) {
@@ -240,7 +244,7 @@ __device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<G>& s, int n, int
     if (!IMG) acc.begin();
     for (int t = full + gl; t < n; t += S) {
         float4 sa, sb;
-        load_site<SEG, G>(s, t, sa, sb);
+        load_site<SEG, SC>(s, t, sa, sb);
         if (IMG) acc.begin();
         for (int im = 0; im < n_img; ++im) {
             float ix = tx, iy = ty, iz = tz;
@@ -259,29 +263,29 @@ __device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<G>& s, int n, int
     if (!IMG) acc.template end<E>();
 }
 
-template <int G, int MODE, bool IMG, bool E>
-__device__ inline void seg_sums32(cptr<DevMol> M, const Sites32<G> s0, float tx, float ty, float tz, int gl, int S,
+template <bool SC, int MODE, bool IMG, bool E>
+__device__ inline void seg_sums32(cptr<DevMol> M, const Sites32<SC> s0, float tx, float ty, float tz, int gl, int S,
                                   Acc32<MODE>& acc) {
-    Sites32<G> s = s0;
+    Sites32<SC> s = s0;
     int n = M->n_seg[0];
-    seg_sum32<0, G, MODE, IMG, E>(M, s, n, gl, S, tx, ty, tz, acc);
+    seg_sum32<0, SC, MODE, IMG, E>(M, s, n, gl, S, tx, ty, tz, acc);
     s.advance(n);
     n = M->n_seg[1];
-    seg_sum32<1, G, MODE, IMG, E>(M, s, n, gl, S, tx, ty, tz, acc);
+    seg_sum32<1, SC, MODE, IMG, E>(M, s, n, gl, S, tx, ty, tz, acc);
     s.advance(n);
     n = M->n_seg[2];
-    if (n) seg_sum32<2, G, MODE, IMG, E>(M, s, n, gl, S, tx, ty, tz, acc);
+    if (n) seg_sum32<2, SC, MODE, IMG, E>(M, s, n, gl, S, tx, ty, tz, acc);
     s.advance(n);
     n = M->n_seg[3];
-    if (n) seg_sum32<3, G, MODE, IMG, E>(M, s, n, gl, S, tx, ty, tz, acc);
+    if (n) seg_sum32<3, SC, MODE, IMG, E>(M, s, n, gl, S, tx, ty, tz, acc);
 }
 
 // Sum of all tip-surface site terms of molecule M at tip position (px,py,pz):
 // this lane's share, sites gl, gl+S, ... of each segment (every image), S
 // lanes per position.  Coordinates are relative to the molecule's origin so
 // FP32 keeps ~1e-6 A resolution.
-template <int G, int MODE, bool E>
-__device__ void site_sums32(cptr<DevMol> M, const Sites32<G> s0, double px, double py, double pz, int gl, int S,
+template <bool SC, int MODE, bool E>
+__device__ void site_sums32(cptr<DevMol> M, const Sites32<SC> s0, double px, double py, double pz, int gl, int S,
                             double out[4]) {
     Acc32<MODE> acc;
     acc.init();
@@ -289,9 +293,9 @@ __device__ void site_sums32(cptr<DevMol> M, const Sites32<G> s0, double px, doub
     const float ty = (float)(py - M->origin[1]);
     const float tz = (float)(pz - M->origin[2]);
     if (M->n_images > 1)
-        seg_sums32<G, MODE, true, E>(M, s0, tx, ty, tz, gl, S, acc);
+        seg_sums32<SC, MODE, true, E>(M, s0, tx, ty, tz, gl, S, acc);
     else
-        seg_sums32<G, MODE, false, E>(M, s0, tx, ty, tz, gl, S, acc);
+        seg_sums32<SC, MODE, false, E>(M, s0, tx, ty, tz, gl, S, acc);
     out[0] = acc.get(0);
     out[1] = acc.get(1);
     out[2] = acc.get(2);
@@ -343,8 +347,8 @@ __device__ inline void pair64(const SiteF64& st, double tx, double ty, double tz
     fz = fs * dz;
 }
 
-template <int SEG, int G, bool E>
-__device__ inline void seg_sum64(cptr<DevMol> M, const Sites<G, SiteF64>& s, int n, int gl, int S, double px,
+template <int SEG, bool SC, bool E>
+__device__ inline void seg_sum64(cptr<DevMol> M, const Sites<SC, SiteF64>& s, int n, int gl, int S, double px,
                                  double py, double pz, double acc[4]) {
     const int n_img = M->n_images;
     const int full = (n / (4 * S)) * (4 * S);
@@ -375,22 +379,22 @@ __device__ inline void seg_sum64(cptr<DevMol> M, const Sites<G, SiteF64>& s, int
     }
 }
 
-template <int G, bool E>
-__device__ void site_sums64(cptr<DevMol> M, const Sites<G, SiteF64> s0, double px, double py, double pz, int gl,
+template <bool SC, bool E>
+__device__ void site_sums64(cptr<DevMol> M, const Sites<SC, SiteF64> s0, double px, double py, double pz, int gl,
                             int S, double out[4]) {
     double acc[4] = {0, 0, 0, 0};
-    Sites<G, SiteF64> s = s0;
+    Sites<SC, SiteF64> s = s0;
     int n = M->n_seg[0];
-    seg_sum64<0, G, E>(M, s, n, gl, S, px, py, pz, acc);
+    seg_sum64<0, SC, E>(M, s, n, gl, S, px, py, pz, acc);
     s.advance(n);
     n = M->n_seg[1];
-    seg_sum64<1, G, E>(M, s, n, gl, S, px, py, pz, acc);
+    seg_sum64<1, SC, E>(M, s, n, gl, S, px, py, pz, acc);
     s.advance(n);
     n = M->n_seg[2];
-    if (n) seg_sum64<2, G, E>(M, s, n, gl, S, px, py, pz, acc);
+    if (n) seg_sum64<2, SC, E>(M, s, n, gl, S, px, py, pz, acc);
     s.advance(n);
     n = M->n_seg[3];
-    if (n) seg_sum64<3, G, E>(M, s, n, gl, S, px, py, pz, acc);
+    if (n) seg_sum64<3, SC, E>(M, s, n, gl, S, px, py, pz, acc);
     out[0] = acc[0];
     out[1] = acc[1];
     out[2] = acc[2];
@@ -550,15 +554,16 @@ __device__ int next_molecule(const ScanArgs& a, int m, int n_cols) {
     return -1;
 }
 
-// blockDim.x = 64 W (W = 1, 2 or 4 waves).  With G > 1 the W waves of a
-// workgroup share one LDS copy of their molecule's sites (so a large
-// molecule costs its LDS once per W waves, not once per wave); each wave
-// still steals columns on its own and the workgroup moves to the next
-// molecule together once every wave has finished its last column.
-template <int G, bool F32, int COMP>
+// blockDim.x = 64 W (W = 1, 2 or 4 waves).  The W waves of a workgroup share
+// one LDS copy of their molecule's sites (so a large molecule costs its LDS
+// once per W waves, not once per wave); each wave still steals columns on its
+// own and the workgroup moves to the next molecule together once every wave
+// has finished its last column.
+template <int G, bool F32, int COMP, bool SC>
 __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanArgs& a) {
+    static_assert(!SC || G == 1, "scalar site streaming serves one lane per pixel");
     const int lane = threadIdx.x & 63;
-    const bool shared_lds = G > 1 && blockDim.x > 64;
+    const bool shared_lds = !SC && blockDim.x > 64;
     const int gl = lane % G;
     const int gbase = lane - gl;
     const bool leader = gl == 0;
@@ -569,10 +574,10 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
     extern __shared__ float4 lds_raw[];
     __shared__ int next_shared;
     int lds_n = 0;   // site count of the molecule staged in LDS
-    // G > 1: copy molecule m's site segments into the workgroup's LDS
-    // (FP32: A array then B array; FP64: SiteF64 records)
+    // copy molecule m's site segments into the workgroup's LDS (FP32: A array
+    // then B array; FP64: SiteF64 records)
     auto stage = [&](int mol) {
-        if (G > 1) {
+        if (!SC) {
             cptr<DevMol> Ms = as_const(a.mols) + mol;
             const int n = Ms->n_seg[0] + Ms->n_seg[1] + Ms->n_seg[2] + Ms->n_seg[3];
             __syncthreads();
@@ -706,17 +711,17 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
                 double part[4] = {0, 0, 0, 0};
                 if (slot_live) {
                     if constexpr (F32) {
-                        // G == 1 keeps the sites in global memory: wide loads read them there
-                        const float4* wa = G > 1 ? lds_raw : a.seg32a + M->seg_off;
-                        const float4* wb = G > 1 ? lds_raw + lds_n : a.seg32b + M->seg_off;
-                        Sites32<64> src64{wa, wb, wa, wb};
-                        if (need_e) site_sums32<64, COMP, true>(M, src64, qx, qy, qz, sub, Ge, part);
-                        else site_sums32<64, COMP, false>(M, src64, qx, qy, qz, sub, Ge, part);
+                        // SC keeps the sites in global memory: wide loads read them there
+                        const float4* wa = SC ? a.seg32a + M->seg_off : lds_raw;
+                        const float4* wb = SC ? a.seg32b + M->seg_off : lds_raw + lds_n;
+                        Sites32<false> src64{wa, wb, wa, wb};
+                        if (need_e) site_sums32<false, COMP, true>(M, src64, qx, qy, qz, sub, Ge, part);
+                        else site_sums32<false, COMP, false>(M, src64, qx, qy, qz, sub, Ge, part);
                     } else {
-                        const SiteF64* w = G > 1 ? reinterpret_cast<const SiteF64*>(lds_raw) : a.segf64 + M->seg_off;
-                        Sites<64, SiteF64> src64{w, w};
-                        if (need_e) site_sums64<64, true>(M, src64, qx, qy, qz, sub, Ge, part);
-                        else site_sums64<64, false>(M, src64, qx, qy, qz, sub, Ge, part);
+                        const SiteF64* w = SC ? a.segf64 + M->seg_off : reinterpret_cast<const SiteF64*>(lds_raw);
+                        Sites<false, SiteF64> src64{w, w};
+                        if (need_e) site_sums64<false, true>(M, src64, qx, qy, qz, sub, Ge, part);
+                        else site_sums64<false, false>(M, src64, qx, qy, qz, sub, Ge, part);
                     }
                 }
                 for (int w2 = 1; w2 < Ge; w2 <<= 1)
@@ -727,13 +732,13 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
                 for (int q = 0; q < 4; ++q) r[q] = __shfl(part[q], from, 64);
             } else {
                 if constexpr (F32) {
-                    Sites32<G> src{a.seg32a + M->seg_off, a.seg32b + M->seg_off, lds_raw, lds_raw + lds_n};
-                    if (need_e) site_sums32<G, COMP, true>(M, src, Px, Py, Pz, gl, G, r);
-                    else site_sums32<G, COMP, false>(M, src, Px, Py, Pz, gl, G, r);
+                    Sites32<SC> src{a.seg32a + M->seg_off, a.seg32b + M->seg_off, lds_raw, lds_raw + lds_n};
+                    if (need_e) site_sums32<SC, COMP, true>(M, src, Px, Py, Pz, gl, G, r);
+                    else site_sums32<SC, COMP, false>(M, src, Px, Py, Pz, gl, G, r);
                 } else {
-                    Sites<G, SiteF64> src{a.segf64 + M->seg_off, reinterpret_cast<const SiteF64*>(lds_raw)};
-                    if (need_e) site_sums64<G, true>(M, src, Px, Py, Pz, gl, G, r);
-                    else site_sums64<G, false>(M, src, Px, Py, Pz, gl, G, r);
+                    Sites<SC, SiteF64> src{a.segf64 + M->seg_off, reinterpret_cast<const SiteF64*>(lds_raw)};
+                    if (need_e) site_sums64<SC, true>(M, src, Px, Py, Pz, gl, G, r);
+                    else site_sums64<SC, false>(M, src, Px, Py, Pz, gl, G, r);
                 }
                 if (G > 1) {
                     for (int q = 0; q < 4; ++q) r[q] = group_sum<G>(r[q]);
@@ -889,27 +894,32 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
 // Both variants hold 200+ live registers; the second launch bound asks for
 // two waves per SIMD (left free, the compiler can take all 256 VGPRs and drop
 // to one wave per SIMD).
-template <int G, int COMP>
+template <int G, int COMP, bool SC>
 __global__ void __launch_bounds__(256, 2) k_scan_fast32(ScanArgs a) {
-    scan_fast_body<G, true, COMP>(a);
+    scan_fast_body<G, true, COMP, SC>(a);
 }
 
-template <int G>
+template <int G, bool SC>
 __global__ void __launch_bounds__(256, 2) k_scan_fast64(ScanArgs a) {
-    scan_fast_body<G, false, 0>(a);
+    scan_fast_body<G, false, 0, SC>(a);
 }
 
-template <int G, bool F32, int COMP>
+template <int G, bool F32, int COMP, bool SC = false>
 hipError_t launch_one(const ScanArgs& a, int n_blocks, int W, size_t lds, hipStream_t st) {
+    const size_t bytes = SC ? 0 : lds;
     if constexpr (F32)
-        hipLaunchKernelGGL((k_scan_fast32<G, COMP>), dim3(n_blocks), dim3(64 * W), G > 1 ? lds : 0, st, a);
+        hipLaunchKernelGGL((k_scan_fast32<G, COMP, SC>), dim3(n_blocks), dim3(64 * W), bytes, st, a);
     else
-        hipLaunchKernelGGL((k_scan_fast64<G>), dim3(n_blocks), dim3(64 * W), G > 1 ? lds : 0, st, a);
+        hipLaunchKernelGGL((k_scan_fast64<G, SC>), dim3(n_blocks), dim3(64 * W), bytes, st, a);
     return hipGetLastError();
 }
 
 template <bool F32, int COMP>
-hipError_t launch_g(const ScanArgs& a, int G, int n_blocks, int W, size_t lds, hipStream_t st) {
+hipError_t launch_g(const ScanArgs& a, int G, bool scalar_sites, int n_blocks, int W, size_t lds, hipStream_t st) {
+    if (scalar_sites) {
+        if (G != 1) return hipErrorInvalidValue;
+        return launch_one<1, F32, COMP, true>(a, n_blocks, W, lds, st);
+    }
     switch (G) {
         case 1: return launch_one<1, F32, COMP>(a, n_blocks, W, lds, st);
         case 2: return launch_one<2, F32, COMP>(a, n_blocks, W, lds, st);
@@ -925,15 +935,17 @@ hipError_t launch_g(const ScanArgs& a, int G, int n_blocks, int W, size_t lds, h
 }  // namespace
 
 // n_blocks workgroups of W waves (W = 1, 2, 4); lds_sites: largest
-// per-molecule site count (sizes the LDS copy for G > 1)
+// per-molecule site count (sizes the LDS copy); scalar_sites: stream the
+// sites with scalar loads instead (G = 1, molecules too large for LDS)
 hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_blocks, int W, int lds_sites,
-                            hipStream_t st) {
+                            bool scalar_sites, hipStream_t st) {
     if (W != 1 && W != 2 && W != 4) return hipErrorInvalidValue;
-    if (!f32) return launch_g<false, 0>(a, G, n_blocks, W, (size_t)lds_sites * sizeof(SiteF64), st);
+    if (!f32)
+        return launch_g<false, 0>(a, G, scalar_sites, n_blocks, W, (size_t)lds_sites * sizeof(SiteF64), st);
     const size_t lds = (size_t)lds_sites * 2 * sizeof(float4);
-    if (comp == ACC_KAHAN) return launch_g<true, ACC_KAHAN>(a, G, n_blocks, W, lds, st);
-    if (comp == ACC_NONE) return launch_g<true, ACC_NONE>(a, G, n_blocks, W, lds, st);
-    return launch_g<true, ACC_BLOCK>(a, G, n_blocks, W, lds, st);
+    if (comp == ACC_KAHAN) return launch_g<true, ACC_KAHAN>(a, G, scalar_sites, n_blocks, W, lds, st);
+    if (comp == ACC_NONE) return launch_g<true, ACC_NONE>(a, G, scalar_sites, n_blocks, W, lds, st);
+    return launch_g<true, ACC_BLOCK>(a, G, scalar_sites, n_blocks, W, lds, st);
 }
 
 }  // namespace mafm

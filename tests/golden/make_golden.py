@@ -50,7 +50,7 @@ CASES = {
     "c4_mini": (W.make_c4, dict(mini=True), 512),
     "c5_mini": (W.make_c5, dict(mini=True), 0),
     "c5_param_ev": (W.make_c5, dict(mini=True, units="eV", tip_dummy_coulomb="on", coulomb="on"), 0),
-    "c5_kj": (W.make_c5, dict(mini=True, units="kJ/mol", area="0.2 0.2"), 0),
+    "c5_kj": (W.make_c5, dict(mini=True, units="kJ/mol", area="0.6 0.6"), 0),
     "c1_normal_x": (lambda d, **k: W.make_c1_normal(d, "x", **k), {}, 0),
     "c1_normal_y": (lambda d, **k: W.make_c1_normal(d, "y", **k), {}, 0),
     "c4_bohr": (lambda d, **k: W.make_c4_variant(d, "bohr", **k), {}, 512),

@@ -175,3 +175,26 @@ def test_output_files_match_reference_format(gpu_ctx, tmp_path):
         want = [fmt % (int(r[2]), int(r[0]), int(r[1]), r[4], r[5], r[6], r[7], r[8], r[9], r[10], r[11], r[12],
                        r[13], r[14], r[15], int(r[3])) for r in rows]
         assert sorted(got) == sorted(want)
+
+
+@pytest.mark.parametrize("precision,n_atoms", [("fast64", 1100), ("fast32", 2100)])
+def test_large_molecule_streams_sites(gpu_ctx, tmp_path, precision, n_atoms):
+    """A molecule whose sites exceed 64 KiB of LDS (1024 FP64 / 2048 FP32
+    sites) is streamed with scalar loads, one lane per pixel."""
+    from mechafm_amd import abi, workloads as W
+    rng = np.random.default_rng(7)
+    mol = W.random_slab(n_atoms, (40.0, 40.0), 4.0, rng)
+    opts = W._base_options(coulomb="on", area="0.3 0.3", zhigh="10.0", zlow="9.0")
+    wl = W._emit("large", str(tmp_path), mol, opts, True)
+    sim = abi.Simulation(wl.input_path, gpu_ctx)
+    sim.run("plain64")
+    ref = sim.records().copy()
+    st = sim.run(precision)
+    got = sim.records()
+    assert st["lanes_per_pixel"] == 1 and st["block_threads"] == 64
+    dpos = np.abs(np.stack([got[c] - ref[c] for c in ("rx", "ry", "rz")])).max(0)
+    dfz = np.abs(got["fz"] - ref["fz"]) / np.maximum(np.abs(ref["fz"]), 1e-2)
+    ok = (dpos <= 1e-4) & (dfz <= 1e-3)
+    assert ok.mean() >= (1.0 if precision == "fast64" else 0.99)
+    with pytest.raises(abi.MafmError):
+        sim.run(precision, lanes_per_pixel=2)
