@@ -31,6 +31,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_VALU_PEAK = 157.3e12      # MI355X FP32 vector, FLOP/s
+HBM_PEAK = 8.0e12              # MI355X HBM3E, B/s
+LOOKUP_BYTES = 8 * 16          # one trilinear lookup: 8 corners of {fx, fy, fz, e} in FP32
 # algorithmic FLOPs per pair term and per evaluation / iteration overheads
 PAIR_FLOPS = {(0, 0): 26, (0, 1): 31, (1, 0): 28, (1, 1): 33}   # (vdw, coulomb)
 TIP_DUMMY_FLOPS = 50
@@ -61,6 +63,13 @@ def algorithmic_flops(sims, recs, integrator=1):
         steps = float(r["n_steps"].astype(np.int64).sum())
         tot += steps * (e * (c + TIP_DUMMY_FLOPS) + ITER_FLOPS) + len(r) * c
     return tot
+
+
+def grid_lookup_bytes(sims, recs, integrator=1):
+    """Algorithmic bytes of the force-grid lookups of a rigidgrid scan: one
+    lookup per evaluation (E per minimiser iteration + the final one)."""
+    e = EVALS_PER_ITER[integrator]
+    return float(sum((r["n_steps"].astype(np.int64).sum() * e + len(r)) * LOOKUP_BYTES for r in recs))
 
 
 def run_batch(abi, sims, prec, **kw):
@@ -208,6 +217,12 @@ def main():
                          "unit": "TFLOP/s", "frac": achieved / FP32_VALU_PEAK, "traffic": None,
                          "kernel_ms": k_avg * 1e3, "flops_per_launch": flops},
         }
+        if info["rigidgrid"]:
+            # force-grid scan: no site terms, the work is the trilinear gathers
+            gb = grid_lookup_bytes(sims, recs)
+            out["roofline"] = {"bound": "hbm", "achieved": gb / k_avg / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                               "frac": gb / k_avg / HBM_PEAK, "traffic": None, "kernel_ms": k_avg * 1e3,
+                               "bytes_per_launch": gb}
         traffic = os.path.join(ROOT, "profiles", f"traffic_scan_{args.config}_{args.precision}.json")
         if os.path.exists(traffic):
             out["roofline"]["traffic"] = json.load(open(traffic)).get("hbm_bytes_per_launch")

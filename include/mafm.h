@@ -30,6 +30,10 @@
  *                          (interactions.cpp:53-234)
  *   mafm_fft3d             fft_data_grid / ffti_data_grid (fft.hpp:25, 34)
  *   mafm_adjoint           no counterpart (gradients of an image loss)
+ *   mafm_deltaf_weights /  the frequency-shift post-processing of
+ *   mafm_delta_f           simple-vafm-image.py:76-204 (Giessibl weighting of
+ *                          each pixel's Fz(z) over the oscillation window),
+ *                          applied to the scan's records on the device
  */
 #ifndef MAFM_H
 #define MAFM_H
@@ -244,6 +248,36 @@ mafm_status mafm_scan(mafm_ctx* ctx, const mafm_scan_desc* desc, mafm_scan_stats
 mafm_status mafm_records(mafm_ctx* ctx, size_t first, size_t count, mafm_record* out);
 mafm_status mafm_eval_tip_surface(mafm_ctx* ctx, int system, size_t n, const double* pos, double* fe);
 mafm_status mafm_adjoint(mafm_ctx* ctx, int system, const mafm_adjoint_desc* desc, mafm_adjoint_out* out);
+
+/* ---- frequency shift (simple-vafm-image.py:76-204) ----
+ * Forces are taken in kcal/mol/A, as the script assumes (its spring constant
+ * is converted with 1 N/m = 1.438978 kcal/mol/A^2). Zero fields take the
+ * script's values. */
+typedef struct {
+    double amplitude;      /* oscillation amplitude, A (0.50, :81) */
+    double k_cantilever;   /* cantilever spring constant, N/m (1800, :82) */
+    double f0;             /* resonance frequency, Hz (25e3, :83) */
+} mafm_deltaf_desc;
+
+typedef struct {
+    int32_t n_weights;     /* heights per image (window length) */
+    int32_t n_images;      /* nz - n_weights: images p = 0 .. n_images-1, ascending height */
+    double dz;             /* height step as the script derives it from the file names */
+    double amplitude;      /* amplitude after the script's half-step adjustment (:87-88) */
+    double prefactor;      /* f0 / (2k) * 2 / (pi A^2) with the unadjusted A (:86) */
+    double weights[128];   /* W[t], t = 0 .. n_weights-1, lowest height first (:89-92) */
+} mafm_deltaf_weights_out;
+
+/* The window and constants of a scan with nz heights from zhigh down in
+ * steps of dz (host only; ctx may be NULL and only receives the message). */
+mafm_status mafm_deltaf_weights(mafm_ctx* ctx, const mafm_deltaf_desc* desc, int32_t nz, double zhigh, double dz,
+                                mafm_deltaf_weights_out* out);
+/* Delta f images (Hz) of `system` from the records of the last mafm_scan:
+ * df[(p * nx + i) * ny + j] = -prefactor * sum_t Fz(i, j, height p + t) W[t] * dz
+ * for p < n_images (mafm_deltaf_weights gives n_images), and z_low[p] = the
+ * lowest height of image p's window (the height the script names it by).
+ * z_low may be NULL. */
+mafm_status mafm_delta_f(mafm_ctx* ctx, int system, const mafm_deltaf_desc* desc, double* df, double* z_low);
 
 #ifdef __cplusplus
 }

@@ -63,6 +63,10 @@ mafm_status mafm_sim_run(mafm_sim* sim, const mafm_run_opts* opts, mafm_scan_sta
 mafm_status mafm_batch_run(int n, mafm_sim* const* sims, const mafm_run_opts* opts, mafm_scan_stats* stats);
 mafm_status mafm_sim_records(mafm_sim* sim, size_t first, size_t count, mafm_record* out);
 mafm_status mafm_sim_write_output(mafm_sim* sim, int threads);
+/* frequency-shift images of the last run (simple-vafm-image.py:76-204) as
+ * deltaf-%06.3f.dat files in the output folder; desc NULL = the script's
+ * amplitude, cantilever and frequency */
+mafm_status mafm_sim_write_deltaf(mafm_sim* sim, const mafm_deltaf_desc* desc);
 const char* mafm_sim_last_error(const mafm_sim* sim);
 void mafm_sim_close(mafm_sim* sim);
 

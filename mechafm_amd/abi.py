@@ -117,6 +117,15 @@ class RunOpts(ctypes.Structure):
 # every exported entry point and its signature: (restype, argtypes)
 _P = ctypes.POINTER
 _vp = ctypes.c_void_p
+class DeltaFDesc(ctypes.Structure):
+    _fields_ = [("amplitude", ctypes.c_double), ("k_cantilever", ctypes.c_double), ("f0", ctypes.c_double)]
+
+
+class DeltaFWeights(ctypes.Structure):
+    _fields_ = [("n_weights", ctypes.c_int32), ("n_images", ctypes.c_int32), ("dz", ctypes.c_double),
+                ("amplitude", ctypes.c_double), ("prefactor", ctypes.c_double), ("weights", ctypes.c_double * 128)]
+
+
 SIGNATURES = {
     "mafm_create": (ctypes.c_int, [ctypes.c_int, _P(_vp)]),
     "mafm_destroy": (None, [_vp]),
@@ -134,6 +143,9 @@ SIGNATURES = {
     "mafm_eval_tip_surface": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_size_t, _P(ctypes.c_double),
                                              _P(ctypes.c_double)]),
     "mafm_adjoint": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
+    "mafm_deltaf_weights": (ctypes.c_int, [_vp, _P(DeltaFDesc), ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                           _P(DeltaFWeights)]),
+    "mafm_delta_f": (ctypes.c_int, [_vp, ctypes.c_int, _P(DeltaFDesc), _P(ctypes.c_double), _P(ctypes.c_double)]),
     "mafm_sim_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _vp, _P(_vp)]),
     "mafm_sim_initialize": (ctypes.c_int, [_vp]),
     "mafm_sim_info_get": (ctypes.c_int, [_vp, _P(SimInfo)]),
@@ -143,6 +155,7 @@ SIGNATURES = {
     "mafm_batch_run": (ctypes.c_int, [ctypes.c_int, _P(_vp), _P(RunOpts), _P(ScanStats)]),
     "mafm_sim_records": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _vp]),
     "mafm_sim_write_output": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "mafm_sim_write_deltaf": (ctypes.c_int, [_vp, _P(DeltaFDesc)]),
     "mafm_sim_last_error": (ctypes.c_char_p, [_vp]),
     "mafm_sim_close": (None, [_vp]),
 }
@@ -277,6 +290,17 @@ class Context:
         self.check(lib().mafm_adjoint(self.h, system, ctypes.byref(d), ctypes.byref(o)))
         return tuple(out)
 
+    def delta_f(self, system: int, nx: int, ny: int, nz: int, zhigh: float, dz: float, amplitude: float = 0.0,
+                k_cantilever: float = 0.0, f0: float = 0.0):
+        """Frequency-shift images of `system` from the last scan's records
+        (mafm_delta_f): (df [n_images, nx, ny] in Hz, z_low [n_images])."""
+        d = DeltaFDesc(amplitude, k_cantilever, f0)
+        w = deltaf_weights(nz, zhigh, dz, amplitude, k_cantilever, f0)
+        df = np.zeros((w["n_images"], nx, ny))
+        zl = np.zeros(w["n_images"])
+        self.check(lib().mafm_delta_f(self.h, system, ctypes.byref(d), _dp(df), _dp(zl)))
+        return df, zl
+
     def eval_tip_surface(self, system: int, pos: np.ndarray) -> np.ndarray:
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         out = np.zeros((len(pos), 4))
@@ -351,6 +375,23 @@ class Simulation:
 
     def write_output(self, threads: int = 8) -> None:
         self.check(lib().mafm_sim_write_output(self.h, threads))
+
+    def write_deltaf(self, amplitude: float = 0.0, k_cantilever: float = 0.0, f0: float = 0.0) -> None:
+        d = DeltaFDesc(amplitude, k_cantilever, f0)
+        self.check(lib().mafm_sim_write_deltaf(self.h, ctypes.byref(d)))
+
+
+def deltaf_weights(nz: int, zhigh: float, dz: float, amplitude: float = 0.0, k_cantilever: float = 0.0,
+                   f0: float = 0.0) -> dict:
+    """Window and constants of the frequency-shift images (mafm_deltaf_weights;
+    host only, no device needed)."""
+    d = DeltaFDesc(amplitude, k_cantilever, f0)
+    o = DeltaFWeights()
+    rc = lib().mafm_deltaf_weights(None, ctypes.byref(d), nz, zhigh, dz, ctypes.byref(o))
+    if rc != MAFM_OK:
+        raise MafmError(rc, "mafm_deltaf_weights rejected the window")
+    return {"n_weights": o.n_weights, "n_images": o.n_images, "dz": o.dz, "amplitude": o.amplitude,
+            "prefactor": o.prefactor, "weights": np.array(o.weights[:o.n_weights])}
 
 
 def batch_run(sims: Sequence[Simulation], precision: str = "fast32", lanes_per_pixel: int = 0,

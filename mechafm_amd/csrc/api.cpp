@@ -22,6 +22,10 @@ hipError_t launch_grid_build(const ScanArgs& a, int mol, const int* n, const dou
 hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_blocks, int W, int lds_sites,
                             bool scalar_sites, hipStream_t st);
 hipError_t launch_to_f32(const double4* in, float4* out, long n, hipStream_t st);
+bool deltaf_weights(const mafm_deltaf_desc& d, int nz, double zhigh, double dz_input, mafm_deltaf_weights_out& o,
+                    std::vector<double>* z_low, char* err, size_t err_len);
+hipError_t launch_delta_f(const DevRecord* rec, const mafm_deltaf_weights_out& w, int nx, int ny, int nz,
+                          double* out, hipStream_t st);
 bool fft_length_supported(int n);
 void fft_twiddles(int n, std::vector<double>& out);
 hipError_t fft3d_device(double2* data, const int* n, const double2* const* tw, bool inv, hipStream_t st);
@@ -121,6 +125,8 @@ struct mafm_ctx {
     std::vector<int64_t> mol_cost;
     size_t n_records = 0;
     bool have_systems = false;
+    mafm_scan_desc last_scan{};   // geometry of the records in d_rec
+    bool have_scan = false;
     std::map<int, DevBuf<double2>> twiddles;   // FFT twiddle tables by length
     DevBuf<int> d_flag;
 };
@@ -520,6 +526,8 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
     HIPCHK(c, hipMemcpyAsync(c->d_mols.p, c->mols.data(), n_mols * sizeof(DevMol), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, c->d_rec.reserve(n_rec));
     c->n_records = n_rec;
+    c->last_scan = *d;
+    c->have_scan = true;
     ScanArgs a = base_args(c);
     a.nx = d->nx; a.ny = d->ny; a.nz = d->nz;
     a.dx = d->dx; a.dy = d->dy; a.dz = d->dz; a.zhigh = d->zhigh;
@@ -633,6 +641,40 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
     if (counters[3] != 0)
         return fail(c, MAFM_ERR_OUT_OF_GRID, "%lld positions needed a force-grid point past the grid",
                     (long long)counters[3]);
+    return MAFM_OK;
+}
+
+mafm_status mafm_deltaf_weights(mafm_ctx* c, const mafm_deltaf_desc* d, int32_t nz, double zhigh, double dz,
+                                mafm_deltaf_weights_out* out) {
+    if (!d || !out) return fail(c, MAFM_ERR_INVALID, "null argument");
+    char err[256];
+    if (!deltaf_weights(*d, nz, zhigh, dz, *out, nullptr, err, sizeof err)) return fail(c, MAFM_ERR_INVALID, "%s", err);
+    return MAFM_OK;
+}
+
+mafm_status mafm_delta_f(mafm_ctx* c, int system, const mafm_deltaf_desc* d, double* df, double* z_low) {
+    if (!c || !d || !df) return MAFM_ERR_INVALID;
+    if (!c->have_scan) return fail(c, MAFM_ERR_STATE, "frequency shift before a scan");
+    if (system < 0 || system >= (int)c->mols.size()) return fail(c, MAFM_ERR_INVALID, "system %d", system);
+    const mafm_scan_desc& s = c->last_scan;
+    mafm_deltaf_weights_out w;
+    std::vector<double> zl;
+    char err[256];
+    if (!deltaf_weights(*d, s.nz, s.zhigh, s.dz, w, &zl, err, sizeof err)) return fail(c, MAFM_ERR_INVALID, "%s", err);
+    if (w.n_images == 0) return fail(c, MAFM_ERR_INVALID, "%d heights do not fill a %d-height window", s.nz, w.n_weights);
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t n = (size_t)w.n_images * s.nx * s.ny;
+    DevBuf<double> out;
+    struct Release {
+        DevBuf<double>& b;
+        ~Release() { b.release(); }
+    } release_out{out};
+    HIPCHK(c, out.reserve(n));
+    const DevRecord* rec = c->d_rec.p + c->mols[system].rec_off;
+    HIPCHK(c, launch_delta_f(rec, w, s.nx, s.ny, s.nz, out.p, c->stream));
+    HIPCHK(c, hipMemcpyAsync(df, out.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (z_low) memcpy(z_low, zl.data(), zl.size() * sizeof(double));
     return MAFM_OK;
 }
 

@@ -1,5 +1,5 @@
 // mechafm: the reference's command line (mechafm.cpp:209-233),
-//   mechafm INPUT [OUTDIR] [--precision plain64|fast64|fast32] [--device N]
+//   mechafm INPUT [OUTDIR] [--precision plain64|fast64|fast32] [--device N] [--deltaf]
 // parse -> initialize (interactions, force grid / external potential on the
 // device) -> one device scan -> per-height output files.
 #include <chrono>
@@ -21,6 +21,7 @@ int main(int argc, char** argv) {
     mafm_run_opts ro{};
     ro.precision = MAFM_PREC_FAST64;
     int device = 0;
+    bool deltaf = false;
     for (int i = 1; i < argc; ++i) {
         if (!strcmp(argv[i], "--precision") && i + 1 < argc) {
             std::string p = argv[++i];
@@ -30,6 +31,8 @@ int main(int argc, char** argv) {
             else return die("precision", "expected plain64, fast64 or fast32");
         } else if (!strcmp(argv[i], "--device") && i + 1 < argc) {
             device = atoi(argv[++i]);
+        } else if (!strcmp(argv[i], "--deltaf")) {
+            deltaf = true;
         } else if (input.empty()) {
             input = argv[i];
         } else if (outdir.empty()) {
@@ -60,6 +63,7 @@ int main(int argc, char** argv) {
     mafm_scan_stats ss{};
     st = mafm_sim_run(sim, &ro, &ss);
     if (st == MAFM_OK) st = mafm_sim_write_output(sim, 8);
+    if (st == MAFM_OK && deltaf) st = mafm_sim_write_deltaf(sim, nullptr);
     auto t1 = std::chrono::steady_clock::now();
     if (st != MAFM_OK) {
         int rc = die("scan", mafm_sim_last_error(sim));

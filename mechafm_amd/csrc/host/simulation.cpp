@@ -813,6 +813,7 @@ void Simulation::run(const mafm_run_opts& r, mafm_scan_stats* stats) {   // simu
     mafm_scan_desc d = scanDesc(r);
     check(ctx_, mafm_scan(ctx_, &d, &last_stats_), "scan");
     record_base_ = 0;
+    system_index_ = 0;
     if (stats) *stats = last_stats_;
 }
 
@@ -884,6 +885,28 @@ void Simulation::writeOutput(int threads) {
             fprintf(f, "    The scan kernel time is %.2f seconds\n", last_stats_.kernel_ms / 1000.0);
             fclose(f);
         }
+    }
+}
+
+void Simulation::writeDeltaF(const mafm_deltaf_desc& d) {
+    const InputOptions& o = options_;
+    const int nx = n_points_[0], ny = n_points_[1], nz = n_points_[2];
+    mafm_deltaf_weights_out w;
+    check(ctx_, mafm_deltaf_weights(ctx_, &d, nz, o.zhigh, o.dz, &w), "frequency-shift window");
+    if (w.n_images == 0) fail(MAFM_ERR_INVALID, "%d heights do not fill a %d-height window", nz, w.n_weights);
+    std::vector<double> df((size_t)w.n_images * nx * ny), zlow(w.n_images);
+    check(ctx_, mafm_delta_f(ctx_, system_index_, &d, df.data(), zlow.data()), "frequency shift");
+    for (int p = 0; p < w.n_images; ++p) {
+        char name[512];
+        snprintf(name, sizeof name, "%sdeltaf-%06.3f.dat", o.outputfolder.c_str(), zlow[p]);
+        FILE* f = fopen(name, "w");
+        if (!f) fail(MAFM_ERR_INVALID, "cannot open output %s", name);
+        fprintf(f, "%% i j x y delta_f[Hz] (A = %.3f A, window of %d heights, dz = %.4f A)\n", w.amplitude,
+                w.n_weights, w.dz);
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j)
+                fprintf(f, "%d %d %6.3f %6.3f %.6e\n", i, j, i * o.dx, j * o.dy, df[((size_t)p * nx + i) * ny + j]);
+        fclose(f);
     }
 }
 
@@ -1013,10 +1036,17 @@ mafm_status mafm_batch_run(int n, mafm_sim* const* sims, const mafm_run_opts* o,
         for (int m = 0; m < n; ++m) {
             sims[m]->sim.records_.clear();
             sims[m]->sim.record_base_ = per * m;
+            sims[m]->sim.system_index_ = m;
             sims[m]->sim.last_stats_ = ss;
         }
         if (stats) *stats = ss;
     });
+}
+
+mafm_status mafm_sim_write_deltaf(mafm_sim* s, const mafm_deltaf_desc* d) {
+    if (!s) return MAFM_ERR_INVALID;
+    const mafm_deltaf_desc def{};
+    return guarded(s, [&]() { s->sim.writeDeltaF(d ? *d : def); });
 }
 
 mafm_status mafm_sim_records(mafm_sim* s, size_t first, size_t count, mafm_record* out) {

@@ -270,6 +270,9 @@ class Simulation {
     void run(const mafm_run_opts& opts, mafm_scan_stats* stats);
     void fetchRecords();
     void writeOutput(int threads);
+    // Delta f images (simple-vafm-image.py:76-204) from this job's records,
+    // one deltaf-%06.3f.dat per image named by its lowest height
+    void writeDeltaF(const mafm_deltaf_desc& d);
 
     InputOptions options_;
     InteractionParameters interaction_parameters_;
@@ -282,8 +285,10 @@ class Simulation {
     int force_grid_n_[3] = {0, 0, 0};
     mafm_ctx* ctx_;
     bool initialized_ = false;
-    // set by batch runs: this simulation's records start here in the context
+    // set by batch runs: this simulation's records start here in the context,
+    // and it is system `system_index_` of the launch
     size_t record_base_ = 0;
+    int system_index_ = 0;
 
   private:
     void calculateTipDummyDistance();
