@@ -62,6 +62,8 @@ mafm_status mafm_sim_scan_desc(mafm_sim* sim, const mafm_run_opts* opts, mafm_sc
 mafm_status mafm_sim_run(mafm_sim* sim, const mafm_run_opts* opts, mafm_scan_stats* stats);
 mafm_status mafm_batch_run(int n, mafm_sim* const* sims, const mafm_run_opts* opts, mafm_scan_stats* stats);
 mafm_status mafm_sim_records(mafm_sim* sim, size_t first, size_t count, mafm_record* out);
+/* per-height files scan-%06.3f.dat[.gz] with the reference's line format
+ * (simulation.cpp:145-199); threads <= 0 uses every hardware thread */
 mafm_status mafm_sim_write_output(mafm_sim* sim, int threads);
 /* frequency-shift images of the last run (simple-vafm-image.py:76-204) as
  * deltaf-%06.3f.dat files in the output folder; desc NULL = the script's

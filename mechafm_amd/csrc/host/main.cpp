@@ -62,7 +62,7 @@ int main(int argc, char** argv) {
            (long long)info.nx * info.ny * info.nz);
     mafm_scan_stats ss{};
     st = mafm_sim_run(sim, &ro, &ss);
-    if (st == MAFM_OK) st = mafm_sim_write_output(sim, 8);
+    if (st == MAFM_OK) st = mafm_sim_write_output(sim, 0);
     if (st == MAFM_OK && deltaf) st = mafm_sim_write_deltaf(sim, nullptr);
     auto t1 = std::chrono::steady_clock::now();
     if (st != MAFM_OK) {

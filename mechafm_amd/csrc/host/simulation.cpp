@@ -3,6 +3,9 @@
 #include "simulation.hpp"
 
 #include <sys/stat.h>
+#include <zlib.h>
+
+#include <atomic>
 
 #include <algorithm>
 #include <cctype>
@@ -14,6 +17,8 @@
 #include <fstream>
 #include <sstream>
 #include <thread>
+
+#include "format.hpp"
 
 namespace mafm {
 
@@ -826,47 +831,97 @@ void Simulation::fetchRecords() {
 // openUniverse / writeOutput / closeUniverse (mechafm.cpp:59-101,
 // simulation.cpp:145-199): one text file per height, fixed printf formats,
 // records in (i, j) order; each file is formatted on its own thread.
+// gzip stream of `data` into `path` (deflate level 6, the level of the
+// reference's "gzip -6" pipe, mechafm.cpp:63-66); false on an I/O error.
+static bool write_gzip(const std::string& path, const std::string& data) {
+    FILE* f = fopen(path.c_str(), "wb");
+    if (!f) return false;
+    z_stream zs{};
+    if (deflateInit2(&zs, 6, Z_DEFLATED, 15 + 16, 8, Z_DEFAULT_STRATEGY) != Z_OK) {
+        fclose(f);
+        return false;
+    }
+    std::vector<unsigned char> out(1 << 20);
+    zs.next_in = (Bytef*)data.data();
+    zs.avail_in = (uInt)data.size();
+    bool ok = true;
+    int rc;
+    do {
+        zs.next_out = out.data();
+        zs.avail_out = (uInt)out.size();
+        rc = deflate(&zs, Z_FINISH);
+        const size_t have = out.size() - zs.avail_out;
+        if (have && fwrite(out.data(), 1, have, f) != have) ok = false;
+    } while (ok && rc == Z_OK);
+    deflateEnd(&zs);
+    if (rc != Z_STREAM_END) ok = false;
+    if (fclose(f) != 0) ok = false;
+    return ok;
+}
+
 void Simulation::writeOutput(int threads) {
     if (records_.empty()) fetchRecords();
     const InputOptions& o = options_;
     const int nx = n_points_[0], ny = n_points_[1], nz = n_points_[2];
     std::vector<std::string> errors(nz);
+    // one file per height, records formatted as the reference prints them
+    // (simulation.cpp:180-195): "%d %d %d %6.3f %6.3f %6.3f %8.4f %8.4f %8.4f
+    // %6.3f %6.3f %6.3f %6.3f %8.4f %8.4f %d"
     auto write_one = [&](int k) {
         char name[512];
         double z = o.zhigh - k * o.dz;
         snprintf(name, sizeof name, "%sscan-%06.3f.dat", o.outputfolder.c_str(), z);
         std::string body;
-        body.reserve((size_t)nx * ny * 112);
-        char line[256];
+        body.resize((size_t)nx * ny * 160 + 64);
+        char* p = &body[0];
         for (int i = 0; i < nx; ++i)
             for (int j = 0; j < ny; ++j) {
                 const mafm_record& r = records_[((size_t)i * ny + j) * nz + k];
-                double rl = std::sqrt(std::pow(r.rx, 2) + std::pow(r.ry, 2) + std::pow(r.rz, 2));
-                double ang = std::atan2(std::sqrt(std::pow(r.rx, 2) + std::pow(r.ry, 2)), r.rz) * (180.0 / PI);
-                int len = snprintf(line, sizeof line,
-                                   "%d %d %d %6.3f %6.3f %6.3f %8.4f %8.4f %8.4f %6.3f %6.3f %6.3f %6.3f %8.4f %8.4f %d\n",
-                                   k, i, j, r.x, r.y, r.z, r.fx, r.fy, r.fz, r.rx, r.ry, r.rz, rl, ang, r.e, r.n_steps);
-                body.append(line, (size_t)len);
+                const double rl = std::sqrt(std::pow(r.rx, 2) + std::pow(r.ry, 2) + std::pow(r.rz, 2));
+                const double ang = std::atan2(std::sqrt(std::pow(r.rx, 2) + std::pow(r.ry, 2)), r.rz) * (180.0 / PI);
+                p = fmt_int(p, k); *p++ = ' ';
+                p = fmt_int(p, i); *p++ = ' ';
+                p = fmt_int(p, j); *p++ = ' ';
+                p = fmt_fixed(p, r.x, 3, 6); *p++ = ' ';
+                p = fmt_fixed(p, r.y, 3, 6); *p++ = ' ';
+                p = fmt_fixed(p, r.z, 3, 6); *p++ = ' ';
+                p = fmt_fixed(p, r.fx, 4, 8); *p++ = ' ';
+                p = fmt_fixed(p, r.fy, 4, 8); *p++ = ' ';
+                p = fmt_fixed(p, r.fz, 4, 8); *p++ = ' ';
+                p = fmt_fixed(p, r.rx, 3, 6); *p++ = ' ';
+                p = fmt_fixed(p, r.ry, 3, 6); *p++ = ' ';
+                p = fmt_fixed(p, r.rz, 3, 6); *p++ = ' ';
+                p = fmt_fixed(p, rl, 3, 6); *p++ = ' ';
+                p = fmt_fixed(p, ang, 4, 8); *p++ = ' ';
+                p = fmt_fixed(p, r.e, 4, 8); *p++ = ' ';
+                p = fmt_int(p, r.n_steps); *p++ = '\n';
+                if ((size_t)(p - &body[0]) + 4096 > body.size()) {   // non-finite or huge values: grow
+                    const size_t used = (size_t)(p - &body[0]);
+                    body.resize(body.size() * 2);
+                    p = &body[0] + used;
+                }
             }
-        FILE* f;
+        body.resize((size_t)(p - &body[0]));
         if (o.gzip) {
-            std::string cmd = std::string("gzip -6 > '") + name + ".gz'";
-            f = popen(cmd.c_str(), "w");
-        } else {
-            f = fopen(name, "w");
+            const std::string gz = std::string(name) + ".gz";
+            if (!write_gzip(gz, body)) errors[k] = "cannot write " + gz;
+            return;
         }
+        FILE* f = fopen(name, "wb");
         if (!f) {
             errors[k] = std::string("cannot open output ") + name;
             return;
         }
         if (fwrite(body.data(), 1, body.size(), f) != body.size()) errors[k] = std::string("short write to ") + name;
-        if (o.gzip) pclose(f); else fclose(f);
+        if (fclose(f) != 0) errors[k] = std::string("cannot close ") + name;
     };
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
     threads = std::max(1, std::min(threads, nz));
+    std::atomic<int> next{0};
     std::vector<std::thread> pool;
     for (int t = 0; t < threads; ++t)
-        pool.emplace_back([&, t]() {
-            for (int k = t; k < nz; k += threads) write_one(k);
+        pool.emplace_back([&]() {
+            for (int k = next++; k < nz; k = next++) write_one(k);
         });
     for (auto& th : pool) th.join();
     for (const auto& e : errors)

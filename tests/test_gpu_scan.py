@@ -154,11 +154,15 @@ def test_waves_per_group_share_sites(gpu_ctx, tmp_path, W):
     assert ok.mean() >= 0.99
 
 
-def test_output_files_match_reference_format(gpu_ctx, tmp_path):
+@pytest.mark.parametrize("gz", [False, True])
+def test_output_files_match_reference_format(gpu_ctx, tmp_path, gz):
     """Per-height files: reference printf formats (simulation.cpp:180-195),
-    one file per z named scan-%06.3f.dat (mechafm.cpp:59-75)."""
+    one file per z named scan-%06.3f.dat[.gz] (mechafm.cpp:59-75)."""
+    import gzip
     from mechafm_amd import abi
     inp = materialize("c1_minterm_e", str(tmp_path / "in"))
+    text = open(inp).read()
+    open(inp, "w").write(text.replace("gzip                     off", "gzip                     on") if gz else text)
     out = str(tmp_path / "out")
     sim = abi.Simulation(inp, gpu_ctx, outdir=out)
     sim.run("plain64")
@@ -166,11 +170,13 @@ def test_output_files_match_reference_format(gpu_ctx, tmp_path):
     _, ref = golden_records("c1_minterm_e")
     nz = int(ref[:, 2].max()) + 1
     files = sorted(f for f in os.listdir(out) if f.startswith("scan-"))
-    assert len(files) == nz and files[-1] == "scan-10.000.dat"
+    suffix = ".dat.gz" if gz else ".dat"
+    assert len(files) == nz and files[-1] == "scan-10.000" + suffix
     fmt = "%d %d %d %6.3f %6.3f %6.3f %8.4f %8.4f %8.4f %6.3f %6.3f %6.3f %6.3f %8.4f %8.4f %d"
     for k in (0, nz // 2, nz - 1):
         z = 10.0 - k * 0.1
-        got = open(os.path.join(out, "scan-%06.3f.dat" % z)).read().splitlines()
+        path = os.path.join(out, "scan-%06.3f" % z + suffix)
+        got = (gzip.open(path, "rt") if gz else open(path)).read().splitlines()
         rows = ref[ref[:, 2] == k]
         want = [fmt % (int(r[2]), int(r[0]), int(r[1]), r[4], r[5], r[6], r[7], r[8], r[9], r[10], r[11], r[12],
                        r[13], r[14], r[15], int(r[3])) for r in rows]
