@@ -127,6 +127,110 @@ def cpu_baseline(name, tmp, budget_s=25.0):
                       f"x and y (dx = dy = 0.8 A), all heights; {elapsed:.1f} s wall on {threads} threads"}
 
 
+def adjoint_bench(tmp, args):
+    """Config 5: gradients of an image loss through the implicit-function
+    adjoint, timed, and checked against central finite differences.
+
+    The adjoint differentiates the relaxed (minimum) tip position, so the
+    scans of this check relax the tip to |F| < 1e-8 (FP64 pair terms) instead
+    of the workload's 0.01, and positions where a parameter step moves the
+    tip to another minimum (the response is not smooth: the central
+    differences with steps h and h/2 disagree) or that do not converge are
+    left out of the loss for both the adjoint and the finite differences;
+    their fraction is reported."""
+    import torch
+    from mechafm_amd import abi
+    from mechafm_amd import workloads as W
+    from mechafm_amd.adjoint import ScanModel
+    ctx = abi.Context(0)
+    wl = W.make_c5(tmp, minterm="f", ftol="1e-8", maxsteps="200000")
+    m = ScanModel(wl.input_path, ctx, precision="fast64", adjoint_precision="fast32")
+    rng = np.random.default_rng(5)
+
+    def image(**kw):
+        with torch.no_grad():
+            a = dict(eps=m.eps, sig=m.sig, q=m.q, pos=m.pos)
+            a.update(kw)
+            img = m.image(**a).numpy().copy()
+        return img, m.last_records["n_steps"].copy()
+
+    target, _ = image(eps=m.eps * torch.from_numpy(1 + 0.05 * rng.choice([-1, 1], len(m.eps))),
+                      sig=m.sig * torch.from_numpy(1 + 0.05 * rng.choice([-1, 1], len(m.sig))),
+                      q=m.q * torch.from_numpy(1 + 0.05 * rng.choice([-1, 1], len(m.q))))
+    t0 = time.perf_counter()
+    base, steps0 = image()
+    t_scan = time.perf_counter() - t0
+    ok = steps0 < 200000
+
+    # parameters checked: eps and sigma of every type, the charge and the three
+    # coordinates with the largest gradient of the full loss
+    leaves = [t.clone().requires_grad_(True) for t in (m.eps, m.sig, m.q, m.pos)]
+    ((m.image(*leaves) - torch.from_numpy(target)) ** 2).sum().backward()
+    params = []
+    for k in range(len(m.eps)):
+        params.append((f"eps[{m.type_names[k]}]", "eps", k, 1e-4 * float(m.eps[k])))
+        params.append((f"sig[{m.type_names[k]}]", "sig", k, 1e-4 * float(m.sig[k])))
+    i = int(torch.argmax(leaves[2].grad.abs()))
+    params.append((f"q[{i}]", "q", i, 1e-4 * abs(float(m.q[i]))))
+    j = int(torch.argmax(leaves[3].grad.abs().max(1).values))
+    params += [(f"pos[{j},{c}]", "pos", (j, c), 1e-5) for c in range(3)]
+
+    fz = {}
+    for name, key, idx, h in params:
+        for sgn, hh in ((1, h), (-1, h), (1, h / 2), (-1, h / 2)):
+            t = getattr(m, key).clone()
+            t[idx] += sgn * hh
+            fz[(name, sgn, hh)], st = image(**{key: t})
+            ok &= st < 200000
+        d1 = (fz[(name, 1, h)] - fz[(name, -1, h)]) / (2 * h)
+        d2 = (fz[(name, 1, h / 2)] - fz[(name, -1, h / 2)]) / h
+        ok &= np.abs(d1 - d2) <= 1e-3 * np.abs(d1) + 1e-4 * np.abs(d1).max()
+    mask = torch.from_numpy(ok.astype(np.float64))
+    tgt = torch.from_numpy(target)
+
+    def loss_of(img):
+        return float((((img - target) ** 2) * ok).sum())
+
+    # timed: forward scan + adjoint of the masked loss (FP32 adjoint pair terms)
+    runs = []
+    for r in range(args.warmup + max(1, args.steps)):
+        lv = [t.clone().requires_grad_(True) for t in (m.eps, m.sig, m.q, m.pos)]
+        t0 = time.perf_counter()
+        img = m.image(*lv)
+        loss = (((img - tgt) ** 2) * mask).sum()
+        t1 = time.perf_counter()
+        loss.backward()
+        t2 = time.perf_counter()
+        if r >= args.warmup:
+            runs.append((t1 - t0, t2 - t1))
+        g32 = [t.grad.clone() for t in lv]
+    m.adjoint_precision = "fast64"
+    lv = [t.clone().requires_grad_(True) for t in (m.eps, m.sig, m.q, m.pos)]
+    (((m.image(*lv) - tgt) ** 2) * mask).sum().backward()
+    g64 = [t.grad.clone() for t in lv]
+    rel32 = max(float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) for a, b in zip(g32, g64))
+    grad = dict(zip(("eps", "sig", "q", "pos"), g64))
+    checks = []
+    for name, key, idx, h in params:
+        f = (loss_of(fz[(name, 1, h)]) - loss_of(fz[(name, -1, h)])) / (2 * h)
+        checks.append({"param": name, "adjoint": float(grad[key][idx]), "finite_difference": f})
+    err = max(abs(c["adjoint"] - c["finite_difference"]) / max(abs(c["finite_difference"]), 1e-12) for c in checks)
+    pos = m.shape[0] * m.shape[1] * m.shape[2]
+    fwd = float(np.mean([r[0] for r in runs]))
+    bwd = float(np.mean([r[1] for r in runs]))
+    out = {"metric": "adjoint gradient of an image loss vs central finite differences (max relative error)",
+           "value": err, "unit": "relative error", "n_gpus": 1, "steps": max(1, args.steps), "warmup": args.warmup,
+           "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "150-atom seeded molecule, LJ + Coulomb, 128x128 px x 30 z, loss = sum (Fz - Fz*)^2 "
+                                  "against a target with eps/sigma/q perturbed by 5 %, tip relaxed to |F| < 1e-8",
+                      "positions": pos, "sites": len(m.types), "types": m.type_names},
+           "positions_in_loss": int(ok.sum()), "positions_left_out": int(pos - ok.sum()),
+           "forward_ms": fwd * 1e3, "backward_ms": bwd * 1e3, "scan_ms": t_scan * 1e3,
+           "backward_positions_per_s": pos / bwd,
+           "fp32_vs_fp64_gradient_rel": rel32, "checks": checks}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +250,8 @@ def main():
                     help="only run a device-to-device copy of this many bytes (calibrates HBM byte counters)")
     ap.add_argument("--compensation", type=int, default=0,
                     help="FP32 sums: 0 blocks of 8 folded into FP64, 1 Kahan, 2 plain")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="the adjoint workload (150 atoms, 128x128 px x 30 z): gradient timing and finite differences")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (pixel columns would shard "
@@ -165,6 +271,12 @@ def main():
 
     from mechafm_amd import abi
     tmp = tempfile.mkdtemp(prefix="mafm_bench_")
+    if args.adjoint:
+        try:
+            adjoint_bench(tmp, args)
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+        return
     try:
         t0 = time.perf_counter()
         wls = workloads(args.config, tmp)
