@@ -78,21 +78,6 @@ def run_batch(abi, sims, prec, **kw):
     return abi.batch_run(sims, prec, **kw)
 
 
-def autotune(abi, sims, prec, candidates, compensation=0, waves_per_group=0, options=0):
-    best = None
-    for G, wps in candidates:
-        try:
-            st = run_batch(abi, sims, prec, lanes_per_pixel=G, waves_per_simd=wps, compensation=compensation,
-                           waves_per_group=waves_per_group, options=options)
-        except abi.MafmError as e:
-            print(f"# tune G={G} waves/SIMD={wps}: skipped ({e})", file=sys.stderr, flush=True)
-            continue
-        print(f"# tune G={G} waves/SIMD={wps}: {st['kernel_ms']:.1f} ms", file=sys.stderr, flush=True)
-        if best is None or st["kernel_ms"] < best[2]:
-            best = (G, wps, st["kernel_ms"])
-    return best
-
-
 def cpu_baseline(name, tmp, budget_s=25.0):
     """The reference program (oracle/_ref/mechafm-omp, built from the
     reference sources) on a sample of the same workload: every molecule, the
@@ -293,8 +278,11 @@ def main():
         G, wps = args.lanes, args.waves
         options = abi.SCAN_REPRODUCIBLE if args.reproducible else 0
         if args.precision != "plain64" and not args.no_tune and (G == 0 or wps == 0):
-            cands = [(g, w) for g in ([G] if G else [1, 2, 4, 8, 16]) for w in ([wps] if wps else [1, 2])]
-            G, wps, _ = autotune(abi, sims, args.precision, cands, args.compensation, args.group, options)
+            from mechafm_amd import tune
+            best = tune.tune(sims, args.precision, compensation=args.compensation, options=options,
+                             lanes=[G] if G else tune.DEFAULT_LANES, use_cache=False, device=dev,
+                             log=lambda m: print("# tune " + m, file=sys.stderr, flush=True))
+            G, wps = best["lanes_per_pixel"], best["waves_per_simd"]
         kw = dict(lanes_per_pixel=G, waves_per_simd=wps, compensation=args.compensation,
                   waves_per_group=args.group, options=options)
         for _ in range(args.warmup):

@@ -204,3 +204,21 @@ def test_large_molecule_streams_sites(gpu_ctx, tmp_path, precision, n_atoms):
     assert ok.mean() >= (1.0 if precision == "fast64" else 0.99)
     with pytest.raises(abi.MafmError):
         sim.run(precision, lanes_per_pixel=2)
+
+
+def test_autotune_on_device(gpu_ctx, tmp_path):
+    """The package tuner measures launch shapes on the workload and caches
+    the fastest; the cached shape is reused without launches."""
+    from mechafm_amd import abi, tune, workloads as W
+    wls = W.make_c3(str(tmp_path / "in"), mini=True, count=4)
+    sims = [abi.Simulation(w.input_path, gpu_ctx) for w in wls]
+    cache = str(tmp_path / "tune.json")
+    best = tune.tune(sims, "fast32", cache=cache, device=gpu_ctx.device_info())
+    assert best["lanes_per_pixel"] in tune.DEFAULT_LANES and best["waves_per_simd"] in (1, 2)
+    assert not best["cached"] and len(best["tried"]) == len(tune.DEFAULT_LANES) + 1
+    again = tune.tune(sims, "fast32", cache=cache, device=gpu_ctx.device_info(),
+                      run=lambda *a, **k: pytest.fail("cached shape should not launch"))
+    assert again["cached"] and again["lanes_per_pixel"] == best["lanes_per_pixel"]
+    st = abi.batch_run(sims, "fast32", lanes_per_pixel=best["lanes_per_pixel"],
+                       waves_per_simd=best["waves_per_simd"])
+    assert st["unconverged"] == 0
