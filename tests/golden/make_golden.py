@@ -51,6 +51,9 @@ CASES = {
     "c5_mini": (W.make_c5, dict(mini=True), 0),
     "c5_param_ev": (W.make_c5, dict(mini=True, units="eV", tip_dummy_coulomb="on", coulomb="on"), 0),
     "c5_kj": (W.make_c5, dict(mini=True, units="kJ/mol", area="0.6 0.6"), 0),
+    "c1_pixel": (W.make_c1, dict(mini=True, area="0.0 0.0", center="0.3 0.2"), 0),
+    "c1_height": (W.make_c1, dict(mini=True, area="0.5 0.5", zlow="10.0"), 0),
+    "c1_maxsteps1": (W.make_c1, dict(mini=True, area="0.3 0.3", maxsteps="1"), 0),
     "c1_normal_x": (lambda d, **k: W.make_c1_normal(d, "x", **k), {}, 0),
     "c1_normal_y": (lambda d, **k: W.make_c1_normal(d, "y", **k), {}, 0),
     "c4_bohr": (lambda d, **k: W.make_c4_variant(d, "bohr", **k), {}, 512),
