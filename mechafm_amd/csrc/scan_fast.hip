@@ -192,7 +192,7 @@ __device__ inline void pair32(const float4 a, const float4 b, float tx, float ty
     } else {
         const float ir = __builtin_amdgcn_rsqf(r2);
         const float r = r2 * ir;
-        const float x = exp2f(b.x * (b.z - r));
+        const float x = __builtin_amdgcn_exp2f(b.x * (b.z - r));   // hardware exp2; results below 2^-126 flush to 0
         const float xm = x - 1.f;
         fs = b.w * x * xm * ir;                 // 2 De a x (x - 1) / r
         e6 = a.w * xm * xm;                     // 6 De (x - 1)^2
