@@ -518,6 +518,9 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
     if (d->integrator < 0 || d->integrator > 2) return fail(c, MAFM_ERR_INVALID, "integrator");
     if (d->minterm < 0 || d->minterm > 2) return fail(c, MAFM_ERR_INVALID, "minterm");
     if (d->precision < 0 || d->precision > 2) return fail(c, MAFM_ERR_INVALID, "precision");
+    if ((int64_t)d->nx * d->ny > INT32_MAX) return fail(c, MAFM_ERR_INVALID, "more than 2^31 pixel columns");
+    if (!std::isfinite(d->dz) || d->dz == 0 || !std::isfinite(d->dt))
+        return fail(c, MAFM_ERR_INVALID, "dz must be finite and non-zero, dt finite");
     HIPCHK(c, hipSetDevice(c->device));
     const int n_mols = (int)c->mols.size();
     const size_t per_mol = (size_t)d->nx * d->ny * d->nz;
