@@ -546,7 +546,11 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
     int G = 1, wps = 0, n_waves = 0, lds_sites = 0, W = 1, n_blocks = 0;
     bool scalar_sites = false;
     if (d->precision != MAFM_PREC_PLAIN64) {
-        wps = d->waves_per_simd > 0 ? d->waves_per_simd : 2;
+        // default grid: three waves per SIMD for the FIRE + midpoint FP32
+        // kernel (built for that occupancy), two for the others
+        const bool mid32 = d->precision == MAFM_PREC_FAST32 && d->minimiser == MAFM_MIN_FIRE &&
+                           d->integrator == MAFM_INT_MIDPOINT;
+        wps = d->waves_per_simd > 0 ? d->waves_per_simd : (mid32 ? 3 : 2);
         const long lanes = (long)c->cus * 4 * wps * 64;
         int64_t max_sites = 1;
         for (const DevMol& m : c->mols) max_sites = std::max<int64_t>(max_sites, m.force_grid >= 0 ? 1 : m.n_sites);

@@ -28,6 +28,8 @@
 //     square root per site, summed in FP32 blocks of 8 folded into FP64
 //     (ACC_BLOCK), Kahan-compensated (ACC_KAHAN) or plain (ACC_NONE);
 //     F64 = FP64 with Newton-refined hardware reciprocals.
+#pragma once
+
 #include "device_common.hpp"
 
 namespace mafm {
@@ -207,9 +209,10 @@ __device__ inline void pair32(const float4 a, const float4 b, float tx, float ty
     fz = fs * dz;
 }
 
-// Sites t = gl, gl + G, ... of one segment against every periodic image of
-// the tip: full blocks of 8 sites per lane (loaded once, reused for all
-// images, no bound test), then the remainder one site at a time.  The image
+// Sites t = gl, gl + S, ... of one segment against every periodic image of
+// the tip: full blocks of 8 (LJ) or 4 (Morse) sites per lane (loaded once,
+// reused for all images, no bound test), then the remainder one site at a
+// time.  The image
 // shifts are wave-uniform (scalar loads); the tip of image im is
 // t - shift[im] in FP32 relative to the molecule's origin.  IMG = false is
 // the single zero-shift image of a non-periodic system; E = false skips the
@@ -218,11 +221,12 @@ template <int SEG, bool SC, int MODE, bool IMG, bool E>
 __device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<SC>& s, int n, int gl, int S, float tx, float ty,
                                  float tz, Acc32<MODE>& acc) {
     const int n_img = IMG ? M->n_images : 1;
-    const int full = (n / (8 * S)) * (8 * S);
-    for (int t0 = 0; t0 < full; t0 += 8 * S) {
-        float4 sa[8], sb[8];
+    constexpr int B = SEG <= 1 ? 8 : 4;    // Morse sites need 8 floats: smaller blocks keep registers down
+    const int full = (n / (B * S)) * (B * S);
+    for (int t0 = 0; t0 < full; t0 += B * S) {
+        float4 sa[B], sb[B];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) load_site<SEG, SC>(s, t0 + u * S + gl, sa[u], sb[u]);
+        for (int u = 0; u < B; ++u) load_site<SEG, SC>(s, t0 + u * S + gl, sa[u], sb[u]);
         for (int im = 0; im < n_img; ++im) {
             float ix = tx, iy = ty, iz = tz;
             if (IMG) {
@@ -232,7 +236,7 @@ __device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<SC>& s, int n, in
             }
             acc.begin();
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < B; ++u) {
                 float fx, fy, fz, e;
                 pair32<SEG>(sa[u], sb[u], ix, iy, iz, fx, fy, fz, e);
                 acc.add(0, fx); acc.add(1, fy); acc.add(2, fz);
@@ -559,7 +563,9 @@ __device__ int next_molecule(const ScanArgs& a, int m, int n_cols) {
 // once per W waves, not once per wave); each wave still steals columns on its
 // own and the workgroup moves to the next molecule together once every wave
 // has finished its last column.
-template <int G, bool F32, int COMP, bool SC>
+// MID = true: FIRE with the midpoint integrator (the reference's default),
+// fixed at compile time so the RK4 / Euler / SD state is not allocated.
+template <int G, bool F32, int COMP, bool SC, bool MID>
 __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanArgs& a) {
     static_assert(!SC || G == 1, "scalar site streaming serves one lane per pixel");
     const int lane = threadIdx.x & 63;
@@ -606,7 +612,7 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
     bool ts = false;
     int64_t steps = 0, unconv = 0, warns = 0, oogs = 0;
     const double dt_max = 10 * a.dt;
-    const bool align = a.minimiser != 0 && a.integrator == 1;
+    const bool align = MID || (a.minimiser != 0 && a.integrator == 1);
 
     for (;;) {
         // ---- refill: groups without a column steal one from molecule m
@@ -781,7 +787,7 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
             continue;
         }
 
-        if (a.minimiser == 0) {
+        if (!MID && a.minimiser == 0) {
             // SDMinimisation (minimiser.cpp:39-65): evaluate, test, move
             fx = r[0]; fy = r[1]; fz = r[2]; e = r[3];
             if (converged(a, fx, fy, fz, e - prev_e)) {
@@ -797,7 +803,7 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
             continue;
         }
 
-        if (a.integrator == 1) {                       // midpointStep
+        if (MID || a.integrator == 1) {                // midpointStep
             if (ph == 0) {
                 Px = tx + dt / 2 * vx; Py = ty + dt / 2 * vy; Pz = tz + dt / 2 * vz;
                 ax = vx + dt / 2 * r[0]; ay = vy + dt / 2 * r[1]; az = vz + dt / 2 * r[2];
@@ -891,61 +897,62 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
     }
 }
 
-// Both variants hold 200+ live registers; the second launch bound asks for
-// two waves per SIMD (left free, the compiler can take all 256 VGPRs and drop
-// to one wave per SIMD).
-template <int G, int COMP, bool SC>
-__global__ void __launch_bounds__(256, 2) k_scan_fast32(ScanArgs a) {
-    scan_fast_body<G, true, COMP, SC>(a);
+// Register budgets: the FIRE + midpoint FP32 kernel is held to three waves
+// per SIMD (168 VGPRs; a few cold FIRE state values spill, which measured
+// faster than two waves without spills), the others to two waves (left free,
+// the compiler can take all 256 VGPRs and drop to one wave per SIMD).
+template <int G, int COMP, bool SC, bool MID>
+__global__ void __launch_bounds__(256, MID ? 3 : 2) k_scan_fast32(ScanArgs a) {
+    scan_fast_body<G, true, COMP, SC, MID>(a);
 }
 
-template <int G, bool SC>
+template <int G, bool SC, bool MID>
 __global__ void __launch_bounds__(256, 2) k_scan_fast64(ScanArgs a) {
-    scan_fast_body<G, false, 0, SC>(a);
+    scan_fast_body<G, false, 0, SC, MID>(a);
 }
 
-template <int G, bool F32, int COMP, bool SC = false>
+template <int G, bool F32, int COMP, bool MID, bool SC = false>
 hipError_t launch_one(const ScanArgs& a, int n_blocks, int W, size_t lds, hipStream_t st) {
     const size_t bytes = SC ? 0 : lds;
     if constexpr (F32)
-        hipLaunchKernelGGL((k_scan_fast32<G, COMP, SC>), dim3(n_blocks), dim3(64 * W), bytes, st, a);
+        hipLaunchKernelGGL((k_scan_fast32<G, COMP, SC, MID>), dim3(n_blocks), dim3(64 * W), bytes, st, a);
     else
-        hipLaunchKernelGGL((k_scan_fast64<G, SC>), dim3(n_blocks), dim3(64 * W), bytes, st, a);
+        hipLaunchKernelGGL((k_scan_fast64<G, SC, MID>), dim3(n_blocks), dim3(64 * W), bytes, st, a);
     return hipGetLastError();
 }
 
-template <bool F32, int COMP>
+template <bool F32, int COMP, bool MID>
 hipError_t launch_g(const ScanArgs& a, int G, bool scalar_sites, int n_blocks, int W, size_t lds, hipStream_t st) {
     if (scalar_sites) {
         if (G != 1) return hipErrorInvalidValue;
-        return launch_one<1, F32, COMP, true>(a, n_blocks, W, lds, st);
+        return launch_one<1, F32, COMP, MID, true>(a, n_blocks, W, lds, st);
     }
     switch (G) {
-        case 1: return launch_one<1, F32, COMP>(a, n_blocks, W, lds, st);
-        case 2: return launch_one<2, F32, COMP>(a, n_blocks, W, lds, st);
-        case 4: return launch_one<4, F32, COMP>(a, n_blocks, W, lds, st);
-        case 8: return launch_one<8, F32, COMP>(a, n_blocks, W, lds, st);
-        case 16: return launch_one<16, F32, COMP>(a, n_blocks, W, lds, st);
-        case 32: return launch_one<32, F32, COMP>(a, n_blocks, W, lds, st);
-        case 64: return launch_one<64, F32, COMP>(a, n_blocks, W, lds, st);
+        case 1: return launch_one<1, F32, COMP, MID>(a, n_blocks, W, lds, st);
+        case 2: return launch_one<2, F32, COMP, MID>(a, n_blocks, W, lds, st);
+        case 4: return launch_one<4, F32, COMP, MID>(a, n_blocks, W, lds, st);
+        case 8: return launch_one<8, F32, COMP, MID>(a, n_blocks, W, lds, st);
+        case 16: return launch_one<16, F32, COMP, MID>(a, n_blocks, W, lds, st);
+        case 32: return launch_one<32, F32, COMP, MID>(a, n_blocks, W, lds, st);
+        case 64: return launch_one<64, F32, COMP, MID>(a, n_blocks, W, lds, st);
         default: return hipErrorInvalidValue;
     }
 }
 
 }  // namespace
 
-// n_blocks workgroups of W waves (W = 1, 2, 4); lds_sites: largest
-// per-molecule site count (sizes the LDS copy); scalar_sites: stream the
-// sites with scalar loads instead (G = 1, molecules too large for LDS)
-hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_blocks, int W, int lds_sites,
-                            bool scalar_sites, hipStream_t st) {
+// Launch of the fast kernels of one minimiser family (MID: FIRE + midpoint);
+// each family is compiled in a translation unit of its own.
+template <bool MID>
+hipError_t launch_scan_fast_family(const ScanArgs& a, bool f32, int comp, int G, int n_blocks, int W, int lds_sites,
+                                   bool scalar_sites, hipStream_t st) {
     if (W != 1 && W != 2 && W != 4) return hipErrorInvalidValue;
     if (!f32)
-        return launch_g<false, 0>(a, G, scalar_sites, n_blocks, W, (size_t)lds_sites * sizeof(SiteF64), st);
+        return launch_g<false, 0, MID>(a, G, scalar_sites, n_blocks, W, (size_t)lds_sites * sizeof(SiteF64), st);
     const size_t lds = (size_t)lds_sites * 2 * sizeof(float4);
-    if (comp == ACC_KAHAN) return launch_g<true, ACC_KAHAN>(a, G, scalar_sites, n_blocks, W, lds, st);
-    if (comp == ACC_NONE) return launch_g<true, ACC_NONE>(a, G, scalar_sites, n_blocks, W, lds, st);
-    return launch_g<true, ACC_BLOCK>(a, G, scalar_sites, n_blocks, W, lds, st);
+    if (comp == ACC_KAHAN) return launch_g<true, ACC_KAHAN, MID>(a, G, scalar_sites, n_blocks, W, lds, st);
+    if (comp == ACC_NONE) return launch_g<true, ACC_NONE, MID>(a, G, scalar_sites, n_blocks, W, lds, st);
+    return launch_g<true, ACC_BLOCK, MID>(a, G, scalar_sites, n_blocks, W, lds, st);
 }
 
 }  // namespace mafm

@@ -9,8 +9,8 @@ tuner measures candidates on the workload itself (whole launches, kernel
 time from HIP events) and caches the winner under a signature of the
 workload and device, so a repeated workload is tuned once.
 
-Search: every G with 2 waves per SIMD, then 1 wave per SIMD for the best G
-(6 launches by default); waves_per_group stays automatic.
+Search: every G with 2 waves per SIMD, then 1 and 3 waves per SIMD for the
+best G (7 launches by default); waves_per_group stays automatic.
 """
 from __future__ import annotations
 
@@ -128,7 +128,8 @@ def tune(sims: Sequence[abi.Simulation], precision: str = "fast32", compensation
     if not tried:
         raise RuntimeError("no launch shape was accepted for this workload")
     g_best = min(tried, key=lambda t: t[2])[0]
-    measure(g_best, 1)
+    for w in (1, 3):
+        measure(g_best, w)
     g, w, ms = min(tried, key=lambda t: t[2])
     best = {"lanes_per_pixel": g, "waves_per_simd": w, "kernel_ms": ms,
             "tried": [{"lanes_per_pixel": a, "waves_per_simd": b, "kernel_ms": c} for a, b, c in tried]}

@@ -34,19 +34,19 @@ def fake_runner(times, calls):
 
 def test_search_picks_fastest_and_caches(tmp_path):
     sims = [FakeSim(120), FakeSim(300)]
-    times = {(1, 2): 50.0, (2, 2): 40.0, (4, 2): 45.0, (8, 2): 60.0, (16, 2): 90.0, (2, 1): 38.0}
+    times = {(1, 2): 50.0, (2, 2): 40.0, (4, 2): 45.0, (8, 2): 60.0, (16, 2): 90.0, (2, 1): 38.0, (2, 3): 39.0}
     calls = []
     cache = str(tmp_path / "tune.json")
     best = tune.tune(sims, "fast32", cache=cache, run=fake_runner(times, calls))
     assert (best["lanes_per_pixel"], best["waves_per_simd"]) == (2, 1) and not best["cached"]
-    assert calls == [(1, 2), (2, 2), (4, 2), (8, 2), (16, 2), (2, 1)]
+    assert calls == [(1, 2), (2, 2), (4, 2), (8, 2), (16, 2), (2, 1), (2, 3)]
     again = tune.tune(sims, "fast32", cache=cache, run=fake_runner(times, calls))
-    assert again["cached"] and again["lanes_per_pixel"] == 2 and len(calls) == 6
+    assert again["cached"] and again["lanes_per_pixel"] == 2 and len(calls) == 7
     assert len(json.load(open(cache))) == 1
 
 
 def test_rejected_shapes_are_skipped(tmp_path):
-    times = {(1, 2): 70.0, (1, 1): 80.0}       # G > 1 rejected (molecule beyond LDS)
+    times = {(1, 2): 70.0, (1, 1): 80.0, (1, 3): 75.0}       # G > 1 rejected (molecule beyond LDS)
     best = tune.tune([FakeSim(5000)], "fast32", cache=str(tmp_path / "t.json"), run=fake_runner(times, []))
     assert best["lanes_per_pixel"] == 1 and best["waves_per_simd"] == 2
 
