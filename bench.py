@@ -234,7 +234,7 @@ def main():
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (calibrates HBM byte counters)")
     ap.add_argument("--compensation", type=int, default=0,
-                    help="FP32 sums: 0 blocks of 8 folded into FP64, 1 Kahan, 2 plain")
+                    help="FP32 sums: 0 blocks of 4-6 sites folded into FP64, 1 Kahan, 2 plain")
     ap.add_argument("--adjoint", action="store_true",
                     help="the adjoint workload (150 atoms, 128x128 px x 30 z): gradient timing and finite differences")
     args = ap.parse_args()
@@ -309,7 +309,7 @@ def main():
                 "precision": args.precision, "lanes_per_pixel": st["lanes_per_pixel"],
                 "waves_per_simd": st["waves_per_simd"], "workgroup_threads": st["block_threads"],
                 "total_steps": st["total_steps"],
-                "compensation": {1: "kahan", 2: "none"}.get(args.compensation, "block8-fp64"),
+                "compensation": {1: "kahan", 2: "none"}.get(args.compensation, "block-fp64"),
                 "group_utilisation": st["active_group_trips"] / max(1, st["group_trips"]),
                 "wide_trip_fraction": st["wide_trips"] * 64 / st["lanes_per_pixel"] / max(1, st["group_trips"]),
                 "unconverged": st["unconverged"]},

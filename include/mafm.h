@@ -146,9 +146,9 @@ typedef struct {
     /* tuning, 0 = automatic */
     int32_t lanes_per_pixel;   /* 1,2,4,...,64 lanes share one pixel column */
     int32_t waves_per_simd;    /* persistent grid size */
-    int32_t compensation;      /* FAST32 sums: 0 = 3 = FP32 blocks of 8 sites folded into FP64
+    int32_t compensation;      /* FAST32 sums: 0 = 3 = FP32 blocks of 4-6 sites folded into FP64
                                   (default), 1 = Kahan-compensated FP32, 2 = plain FP32 */
-    int32_t waves_per_group;   /* 1, 2, 4 waves per workgroup sharing one LDS site copy */
+    int32_t waves_per_group;   /* 1 .. 8 waves per workgroup sharing one LDS site copy */
     uint32_t options;          /* MAFM_SCAN_* bits */
     int32_t reserved;
 } mafm_scan_desc;

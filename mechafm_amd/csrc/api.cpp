@@ -20,7 +20,7 @@ hipError_t launch_eval_plain(const ScanArgs& a, int mol, long n, const double* p
 hipError_t launch_grid_build(const ScanArgs& a, int mol, const int* n, const double* s, const double* o,
                              double4* out, hipStream_t st);
 hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_blocks, int W, int lds_sites,
-                            bool scalar_sites, hipStream_t st);
+                            bool scalar_sites, bool mid_family, hipStream_t st);
 hipError_t launch_to_f32(const double4* in, float4* out, long n, hipStream_t st);
 bool deltaf_weights(const mafm_deltaf_desc& d, int nz, double zhigh, double dz_input, mafm_deltaf_weights_out& o,
                     std::vector<double>* z_low, char* err, size_t err_len);
@@ -544,12 +544,16 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
     const long cols_per_mol = (long)d->nx * d->ny;
     const long total_cols = cols_per_mol * n_mols;
     int G = 1, wps = 0, n_waves = 0, lds_sites = 0, W = 1, n_blocks = 0;
-    bool scalar_sites = false;
+    bool scalar_sites = false, mid_family = false;
     if (d->precision != MAFM_PREC_PLAIN64) {
-        // default grid: three waves per SIMD for the FIRE + midpoint FP32
-        // kernel (built for that occupancy), two for the others
-        const bool mid32 = d->precision == MAFM_PREC_FAST32 && d->minimiser == MAFM_MIN_FIRE &&
-                           d->integrator == MAFM_INT_MIDPOINT;
+        // kernel family: FIRE + midpoint on non-periodic systems runs the
+        // specialised kernels (FP32: built for three waves per SIMD); periodic
+        // systems, whose image loop needs more registers, and every other
+        // minimiser run the generic ones (two waves per SIMD)
+        bool periodic = false;
+        for (const DevMol& m : c->mols) periodic |= m.n_images > 1;
+        mid_family = d->minimiser == MAFM_MIN_FIRE && d->integrator == MAFM_INT_MIDPOINT && !periodic;
+        const bool mid32 = mid_family && d->precision == MAFM_PREC_FAST32;
         wps = d->waves_per_simd > 0 ? d->waves_per_simd : (mid32 ? 3 : 2);
         const long lanes = (long)c->cus * 4 * wps * 64;
         int64_t max_sites = 1;
@@ -576,12 +580,11 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
         // LDS a CU holds admits wps waves per SIMD
         if (d->waves_per_group > 0) {
             W = d->waves_per_group;
-            if (W != 1 && W != 2 && W != 4)
-                return fail(c, MAFM_ERR_INVALID, "waves_per_group must be 1, 2 or 4");
+            if (W < 1 || W > 8) return fail(c, MAFM_ERR_INVALID, "waves_per_group must be 1 .. 8");
         } else if (!scalar_sites) {
             const size_t lds_bytes = std::max<size_t>((size_t)max_sites * site_bytes, 1);
             const long blocks_per_cu = (long)(c->lds_per_cu / lds_bytes);
-            while (W < 4 && blocks_per_cu * W < 4L * wps) W <<= 1;
+            while (W < 8 && blocks_per_cu * W < 4L * wps) ++W;
         }
         const long groups_per_wave = 64 / G;
         const long need_waves = (total_cols + groups_per_wave - 1) / groups_per_wave;
@@ -614,7 +617,7 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
         // 0 / 3: block-compensated (default), 1: Kahan, 2: plain FP32
         const int comp = d->compensation == 1 ? 1 : (d->compensation == 2 ? 0 : 2);
         HIPCHK(c, launch_scan_fast(a, d->precision == MAFM_PREC_FAST32, comp, G, n_blocks, W, lds_sites, scalar_sites,
-                                   c->stream));
+                                   mid_family, c->stream));
     }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     int64_t counters[8];

@@ -25,7 +25,7 @@
 //     difference), tip-dummy terms and the integrator updates are FP64 and
 //     follow minimiser.cpp:67-129 / integrators.cpp:8-111.
 //   * Pair terms: F32 = FP32 with one hardware reciprocal or reciprocal
-//     square root per site, summed in FP32 blocks of 8 folded into FP64
+//     square root per site, summed in FP32 blocks of 4-6 sites folded into FP64
 //     (ACC_BLOCK), Kahan-compensated (ACC_KAHAN) or plain (ACC_NONE);
 //     F64 = FP64 with Newton-refined hardware reciprocals.
 #pragma once
@@ -104,7 +104,7 @@ struct Sites32 {
 };
 
 // Accumulators of the FP32 pair sums.
-//   ACC_BLOCK  FP32 partial sums over blocks of 8 sites, folded into FP64
+//   ACC_BLOCK  FP32 partial sums over register blocks of sites, folded into FP64
 //   ACC_KAHAN  Kahan-compensated FP32 sum, term by term
 //   ACC_NONE   plain FP32 sum
 enum { ACC_NONE = 0, ACC_KAHAN = 1, ACC_BLOCK = 2 };
@@ -210,7 +210,7 @@ __device__ inline void pair32(const float4 a, const float4 b, float tx, float ty
 }
 
 // Sites t = gl, gl + S, ... of one segment against every periodic image of
-// the tip: full blocks of 8 (LJ) or 4 (Morse) sites per lane (loaded once,
+// the tip: full blocks of 6 (LJ) or 4 (Morse) sites per lane (loaded once,
 // reused for all images, no bound test), then the remainder one site at a
 // time.  The image
 // shifts are wave-uniform (scalar loads); the tip of image im is
@@ -221,7 +221,7 @@ template <int SEG, bool SC, int MODE, bool IMG, bool E>
 __device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<SC>& s, int n, int gl, int S, float tx, float ty,
                                  float tz, Acc32<MODE>& acc) {
     const int n_img = IMG ? M->n_images : 1;
-    constexpr int B = SEG <= 1 ? 8 : 4;    // Morse sites need 8 floats: smaller blocks keep registers down
+    constexpr int B = SEG <= 1 ? 6 : 4;    // register blocks: 6 LJ sites (6 floats each), 4 Morse (8 floats)
     const int full = (n / (B * S)) * (B * S);
     for (int t0 = 0; t0 < full; t0 += B * S) {
         float4 sa[B], sb[B];
@@ -558,7 +558,7 @@ __device__ int next_molecule(const ScanArgs& a, int m, int n_cols) {
     return -1;
 }
 
-// blockDim.x = 64 W (W = 1, 2 or 4 waves).  The W waves of a workgroup share
+// blockDim.x = 64 W (W = 1 .. 8 waves).  The W waves of a workgroup share
 // one LDS copy of their molecule's sites (so a large molecule costs its LDS
 // once per W waves, not once per wave); each wave still steals columns on its
 // own and the workgroup moves to the next molecule together once every wave
@@ -601,7 +601,8 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
         }
     };
     stage(m);
-    int64_t trips = 0, active = 0, wide_trips = 0;
+    int64_t trips = 0, wide_trips = 0;
+    int32_t active = 0;
 
     // group state (identical on the G lanes of a group)
     int col = -1, k = 0, ph = 0, n_tot = 1, nnn = 0, flags = 0, i = 0, j = 0;
@@ -610,7 +611,8 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
     double ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0, cx = 0, cy = 0, cz = 0, ce = 0;
     double Px = 0, Py = 0, Pz = 1.0e3;
     bool ts = false;
-    int64_t steps = 0, unconv = 0, warns = 0, oogs = 0;
+    int64_t steps = 0;
+    int32_t unconv = 0, warns = 0, oogs = 0;
     const double dt_max = 10 * a.dt;
     const bool align = MID || (a.minimiser != 0 && a.integrator == 1);
 
@@ -693,6 +695,16 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
         const bool need_e = !align || (odd && (a.minterm != 1 || __any(live && ph == PH_OUT)));
         active += live;
         cptr<DevMol> M = as_const(a.mols) + m;
+        if constexpr (MID) {
+            // the evaluation point follows from the state (half-step: t + dt/2 v,
+            // the same expression midpointStep uses), so it is not carried in
+            // registers from trip to trip
+            if (ph == 1) {
+                Px = tx + dt / 2 * vx; Py = ty + dt / 2 * vy; Pz = tz + dt / 2 * vz;
+            } else {
+                Px = tx; Py = ty; Pz = tz;
+            }
+        }
         double r[4];
         if (M->force_grid >= 0) {
             r[0] = r[1] = r[2] = r[3] = 0;
@@ -902,12 +914,12 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
 // faster than two waves without spills), the others to two waves (left free,
 // the compiler can take all 256 VGPRs and drop to one wave per SIMD).
 template <int G, int COMP, bool SC, bool MID>
-__global__ void __launch_bounds__(256, MID ? 3 : 2) k_scan_fast32(ScanArgs a) {
+__global__ void __launch_bounds__(512, MID ? 3 : 2) k_scan_fast32(ScanArgs a) {
     scan_fast_body<G, true, COMP, SC, MID>(a);
 }
 
 template <int G, bool SC, bool MID>
-__global__ void __launch_bounds__(256, 2) k_scan_fast64(ScanArgs a) {
+__global__ void __launch_bounds__(512, 2) k_scan_fast64(ScanArgs a) {
     scan_fast_body<G, false, 0, SC, MID>(a);
 }
 
@@ -946,7 +958,7 @@ hipError_t launch_g(const ScanArgs& a, int G, bool scalar_sites, int n_blocks, i
 template <bool MID>
 hipError_t launch_scan_fast_family(const ScanArgs& a, bool f32, int comp, int G, int n_blocks, int W, int lds_sites,
                                    bool scalar_sites, hipStream_t st) {
-    if (W != 1 && W != 2 && W != 4) return hipErrorInvalidValue;
+    if (W < 1 || W > 8) return hipErrorInvalidValue;
     if (!f32)
         return launch_g<false, 0, MID>(a, G, scalar_sites, n_blocks, W, (size_t)lds_sites * sizeof(SiteF64), st);
     const size_t lds = (size_t)lds_sites * 2 * sizeof(float4);

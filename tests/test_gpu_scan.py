@@ -142,7 +142,7 @@ def test_wide_tail_matches_fixed_order(gpu_ctx, tmp_path, G):
     assert ok.mean() >= 0.99
 
 
-@pytest.mark.parametrize("W", [1, 2, 4])
+@pytest.mark.parametrize("W", [1, 2, 4, 6])
 def test_waves_per_group_share_sites(gpu_ctx, tmp_path, W):
     """W waves of a workgroup share one LDS copy of a periodic system's sites
     (c4 mini: slab with 9 images plus an external potential)."""
