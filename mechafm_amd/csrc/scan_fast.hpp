@@ -223,10 +223,12 @@ __device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<SC>& s, int n, in
     const int n_img = IMG ? M->n_images : 1;
     constexpr int B = SEG <= 1 ? 6 : 4;    // register blocks: 6 LJ sites (6 floats each), 4 Morse (8 floats)
     const int full = (n / (B * S)) * (B * S);
+    // one FP32 partial per register block (and all its images), folded into FP64
     for (int t0 = 0; t0 < full; t0 += B * S) {
         float4 sa[B], sb[B];
 #pragma unroll
         for (int u = 0; u < B; ++u) load_site<SEG, SC>(s, t0 + u * S + gl, sa[u], sb[u]);
+        acc.begin();
         for (int im = 0; im < n_img; ++im) {
             float ix = tx, iy = ty, iz = tz;
             if (IMG) {
@@ -234,7 +236,6 @@ __device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<SC>& s, int n, in
                 iy -= M->shift32[im][1];
                 iz -= M->shift32[im][2];
             }
-            acc.begin();
 #pragma unroll
             for (int u = 0; u < B; ++u) {
                 float fx, fy, fz, e;
@@ -242,8 +243,8 @@ __device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<SC>& s, int n, in
                 acc.add(0, fx); acc.add(1, fy); acc.add(2, fz);
                 if (E) acc.add(3, e);
             }
-            acc.template end<E>();
         }
+        acc.template end<E>();
     }
     if (!IMG) acc.begin();
     for (int t = full + gl; t < n; t += S) {
