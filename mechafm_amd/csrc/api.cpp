@@ -85,6 +85,10 @@ template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }   // also on early error returns
     hipError_t reserve(size_t n) {
         if (n <= cap && p) return hipSuccess;
         if (p) (void)hipFree(p);
@@ -675,10 +679,6 @@ mafm_status mafm_delta_f(mafm_ctx* c, int system, const mafm_deltaf_desc* d, dou
     HIPCHK(c, hipSetDevice(c->device));
     const size_t n = (size_t)w.n_images * s.nx * s.ny;
     DevBuf<double> out;
-    struct Release {
-        DevBuf<double>& b;
-        ~Release() { b.release(); }
-    } release_out{out};
     HIPCHK(c, out.reserve(n));
     const DevRecord* rec = c->d_rec.p + c->mols[system].rec_off;
     HIPCHK(c, launch_delta_f(rec, w, s.nx, s.ny, s.nz, out.p, c->stream));
@@ -726,10 +726,15 @@ static mafm_status twiddles_for(mafm_ctx* c, int n, const double2** out) {
     if (it == c->twiddles.end()) {
         std::vector<double> h;
         fft_twiddles(n, h);
-        DevBuf<double2> b;
-        HIPCHK(c, b.reserve(n));
-        HIPCHK(c, hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        it = c->twiddles.emplace(n, b).first;
+        DevBuf<double2>& b = c->twiddles[n];
+        hipError_t e = b.reserve(n);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) {
+            c->twiddles.erase(n);
+            return fail(c, MAFM_ERR_HIP, "twiddle table of length %d: %s", n, hipGetErrorString(e));
+        }
+        it = c->twiddles.find(n);
     }
     *out = it->second.p;
     return MAFM_OK;
