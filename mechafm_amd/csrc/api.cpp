@@ -124,6 +124,7 @@ struct mafm_ctx {
     DevBuf<DevRecord> d_rec;
     DevBuf<int32_t> d_next;
     DevBuf<float> d_cum;
+    DevBuf<int32_t> d_order;
     DevBuf<int64_t> d_counters;
     DevBuf<double> d_tmp;
     std::vector<int64_t> mol_cost;
@@ -356,6 +357,7 @@ void mafm_destroy(mafm_ctx* c) {
     c->d_rec.release();
     c->d_next.release();
     c->d_cum.release();
+    c->d_order.release();
     c->d_counters.release();
     c->d_tmp.release();
     for (auto& kv : c->twiddles) kv.second.release();
@@ -596,23 +598,48 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
         n_waves = std::max(n_waves, 1);
         n_blocks = (n_waves + W - 1) / W;
         n_waves = n_blocks * W;
-        // per-molecule column counters and cumulative work for the initial wave placement
-        std::vector<int32_t> zero(n_mols, 0);
-        std::vector<float> cum(n_mols + 1, 0.f);
+        // Molecules by rank: decreasing work per column (sites x images), so
+        // the longest columns start first.  Initial workgroup shares: when the
+        // batch needs more workgroups than the grid holds, fill ranks in order
+        // with enough workgroups to take every column of the molecule at once
+        // (longest-processing-time first); otherwise share in proportion to
+        // each molecule's total work.
+        std::vector<int32_t> zero(n_mols, 0), order(n_mols);
+        for (int m = 0; m < n_mols; ++m) order[m] = m;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return c->mol_cost[x] > c->mol_cost[y]; });
+        const long cols_per_block = (64 / G) * (long)W;
+        const long need_blocks = (cols_per_mol + cols_per_block - 1) / cols_per_block;
+        std::vector<double> share(n_mols, 0.0);
+        if (need_blocks * n_mols > n_blocks) {
+            long left = n_blocks;
+            for (int k = 0; k < n_mols && left > 0; ++k) {
+                const long b = std::min(need_blocks, left);
+                share[k] = (double)b;
+                left -= b;
+            }
+        } else {
+            for (int k = 0; k < n_mols; ++k) share[k] = (double)c->mol_cost[order[k]];
+        }
         double tot = 0;
-        for (int m = 0; m < n_mols; ++m) tot += (double)c->mol_cost[m];
+        for (double v : share) tot += v;
+        std::vector<float> cum(n_mols + 1, 0.f);
         double run = 0;
-        for (int m = 0; m < n_mols; ++m) {
-            run += (double)c->mol_cost[m];
-            cum[m + 1] = (float)(run / tot);
+        for (int k = 0; k < n_mols; ++k) {
+            run += share[k];
+            cum[k + 1] = (float)(run / tot);
         }
         cum[n_mols] = 1.0f;
         HIPCHK(c, c->d_next.reserve(n_mols));
         HIPCHK(c, c->d_cum.reserve(n_mols + 1));
+        HIPCHK(c, c->d_order.reserve(n_mols));
         HIPCHK(c, hipMemcpyAsync(c->d_next.p, zero.data(), n_mols * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->d_cum.p, cum.data(), (n_mols + 1) * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_order.p, order.data(), n_mols * sizeof(int32_t), hipMemcpyHostToDevice,
+                                 c->stream));
         a.next_col = c->d_next.p;
         a.mol_cum = c->d_cum.p;
+        a.mol_order = c->d_order.p;
     }
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     if (d->precision == MAFM_PREC_PLAIN64) {

@@ -96,7 +96,8 @@ struct ScanArgs {
     DevRecord* rec;
     // work-stealing state (FAST kernels)
     int32_t* next_col;        // per molecule: next column to hand out
-    const float* mol_cum;     // cumulative work fraction per molecule (n_mols + 1)
+    const float* mol_cum;     // initial share of the workgroups, cumulative over ranks (n_mols + 1)
+    const int32_t* mol_order; // molecule of each rank: decreasing work per column
     int64_t* counters;        // [0] total steps [1] unconverged [2] grid warnings [3] out-of-grid
     int32_t wide_tail;        // > 0: regroup a wave's last groups over 64 lanes when that cuts
                               // the sites per lane by this factor; 0: never

@@ -539,7 +539,10 @@ __device__ inline bool converged(const ScanArgs& a, double fx, double fy, double
     }
 }
 
-__device__ int pick_molecule(const ScanArgs& a, int block, int n_blocks) {
+// Molecules are visited by rank (decreasing work per column: the longest
+// columns start first and the tail of the launch is made of short ones).
+// Rank of workgroup `block`'s first molecule from the initial shares.
+__device__ int pick_rank(const ScanArgs& a, int block, int n_blocks) {
     const float target = (block + 0.5f) / n_blocks;
     int lo = 0, hi = a.n_mols - 1;
     while (lo < hi) {
@@ -549,12 +552,12 @@ __device__ int pick_molecule(const ScanArgs& a, int block, int n_blocks) {
     return lo;
 }
 
-// Next molecule with columns left after m, or -1.
-__device__ int next_molecule(const ScanArgs& a, int m, int n_cols) {
+// Next rank after rk whose molecule has columns left, or -1.
+__device__ int next_rank(const ScanArgs& a, int rk, int n_cols) {
     for (int q = 1; q <= a.n_mols; ++q) {
-        int cand = m + q;
+        int cand = rk + q;
         if (cand >= a.n_mols) cand -= a.n_mols;
-        if (__atomic_load_n(&a.next_col[cand], __ATOMIC_RELAXED) < n_cols) return cand;
+        if (__atomic_load_n(&a.next_col[a.mol_order[cand]], __ATOMIC_RELAXED) < n_cols) return cand;
     }
     return -1;
 }
@@ -577,7 +580,8 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
     const int n_cols = a.nx * a.ny;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
-    int m = __builtin_amdgcn_readfirstlane(pick_molecule(a, blockIdx.x, gridDim.x));
+    int rk = __builtin_amdgcn_readfirstlane(pick_rank(a, blockIdx.x, gridDim.x));
+    int m = a.mol_order[rk];
     extern __shared__ float4 lds_raw[];
     __shared__ int next_shared;
     int lds_n = 0;   // site count of the molecule staged in LDS
@@ -664,20 +668,21 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
         }
         if (!__any(col >= 0)) {
             // molecule m exhausted for this wave: move to one with columns left
-            int nm;
+            int nr;
             if (shared_lds) {
                 // m's counter is spent, so every wave of the workgroup arrives
                 // here after its last column; one lane picks for all of them
                 __syncthreads();
-                if (threadIdx.x == 0) next_shared = next_molecule(a, m, n_cols);
+                if (threadIdx.x == 0) next_shared = next_rank(a, rk, n_cols);
                 __syncthreads();
-                nm = next_shared;
+                nr = next_shared;
             } else {
-                nm = next_molecule(a, m, n_cols);
+                nr = next_rank(a, rk, n_cols);
             }
-            nm = __builtin_amdgcn_readfirstlane(nm);
-            if (nm < 0) break;
-            m = nm;
+            nr = __builtin_amdgcn_readfirstlane(nr);
+            if (nr < 0) break;
+            rk = nr;
+            m = a.mol_order[rk];
             col = -1;
             stage(m);
             continue;
