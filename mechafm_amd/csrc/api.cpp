@@ -285,7 +285,7 @@ static mafm_status build_tables(mafm_ctx* c, int n, const mafm_system_desc* sys,
                 const mafm_site& t = s.sites[k];
                 const int coul = (t.has_coulomb && t.qq != 0.0) ? 1 : 0;
                 if (t.vdw != want_vdw || coul != want_coul) continue;
-                // pre-scaled constants (scan_fast.hip, pair32 / pair64)
+                // pre-scaled constants (scan_fast.hpp, pair32 / pair64)
                 const double qq = coul ? t.qq : 0.0;
                 SiteF64 g;
                 g.x = t.x; g.y = t.y; g.z = t.z; g.qq = qq;
@@ -294,12 +294,12 @@ static mafm_status build_tables(mafm_ctx* c, int n, const mafm_system_desc* sys,
                                   (float)(t.z - d.origin[2]), 0.f);
                 if (t.vdw == MAFM_VDW_LJ) {
                     g.p0 = 6 * t.p0; g.p1 = 6 * t.p1; g.p2 = 0; g.p3 = 0;
-                    f.a.w = (float)g.p0;
-                    f.b = make_float4((float)g.p1, (float)qq, 0.f, 0.f);
+                    f.a.w = (float)(6 * t.p0);
+                    f.b = make_float4((float)(12 * t.p1), (float)qq, 0.f, 0.f);
                 } else {
                     g.p0 = 6 * t.p0; g.p1 = t.p1; g.p2 = t.p2; g.p3 = 2 * t.p0 * t.p1;
-                    f.a.w = (float)g.p0;
-                    f.b = make_float4((float)(t.p1 * LOG2E), (float)qq, (float)t.p2, (float)g.p3);
+                    f.a.w = (float)(12 * t.p0);
+                    f.b = make_float4((float)(t.p1 * LOG2E), (float)qq, (float)(t.p1 * LOG2E * t.p2), (float)g.p3);
                 }
                 seg64.push_back(g);
                 seg32.push_back(f);

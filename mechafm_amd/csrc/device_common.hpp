@@ -38,8 +38,8 @@ struct SiteF64 {
 };
 
 struct Site32 {
-    float4 a;         // x, y, z (relative to DevMol::origin), 6 es6 | 6 De
-    float4 b;         // 6 es12 | a log2(e), qq, - | re, - | 2 De a
+    float4 a;         // x, y, z (relative to DevMol::origin), 6 es6 | 12 De
+    float4 b;         // 12 es12 | a log2(e), qq, - | a log2(e) re, - | 2 De a
 };
 
 struct DevGrid {
@@ -90,8 +90,8 @@ struct ScanArgs {
     const DevMol* mols;
     const Site64* sites_ref;
     const SiteF64* segf64;
-    const float4* seg32a;     // FP32 sites: {x, y, z, 6 es6 | 6 De}
-    const float4* seg32b;     //             {6 es12 | a log2(e), qq, - | re, - | 2 De a}
+    const float4* seg32a;     // FP32 sites: {x, y, z, 6 es6 | 12 De}
+    const float4* seg32b;     //             {12 es12 | a log2(e), qq, - | a log2(e) re, - | 2 De a}
     const DevGrid* grids;
     DevRecord* rec;
     // work-stealing state (FAST kernels)

@@ -82,7 +82,8 @@ struct Sites {
 };
 
 // FP32 sites as two arrays so a pair reads only what it needs: A = {x, y, z,
-// 6 es6 | 6 De} (16 B) and B = {6 es12 | a log2(e), qq, - | re, - | 2 De a};
+// 6 es6 | 12 De} (16 B) and B = {12 es12 | a log2(e), qq, - | a log2(e) re,
+// - | 2 De a};
 // an LJ pair reads 20 B, LJ + Coulomb 24 B, Morse 32 B.
 template <bool SC>
 struct Sites32 {
@@ -126,6 +127,11 @@ struct Acc32 {
         if (MODE == ACC_BLOCK)
             for (int q = 0; q < (E ? 4 : 3); ++q) d[q] += (double)s[q];
     }
+    // s[q] += u * v, fused where the mode allows
+    __device__ inline void madd(int q, float u, float v) {
+        if (MODE == ACC_KAHAN) add(q, u * v);
+        else s[q] = fmaf(u, v, s[q]);
+    }
     __device__ inline void add(int q, float x) {
         if (MODE == ACC_KAHAN) {
             float y = x - c[q];
@@ -143,8 +149,9 @@ struct Acc32 {
     }
 };
 
-// One site of segment SEG in registers: a = {x, y, z, 6 es6 | 6 De}, b = {6 es12 |
-// a log2(e), qq, - | re, - | 2 De a}; only the B words the segment uses are read.
+// One site of segment SEG in registers: a = {x, y, z, 6 es6 | 12 De}, b = {12 es12 |
+// a log2(e), qq, - | a log2(e) re, - | 2 De a}; only the B words the segment uses
+// are read.
 template <int SEG, bool SC>
 __device__ inline void load_site(const Sites32<SC>& s, int t, float4& a, float4& b) {
     a = s.A(t);
@@ -160,13 +167,19 @@ __device__ inline void load_site(const Sites32<SC>& s, int t, float4& a, float4&
 }
 
 // One site term in FP32 (LJ: interactions.cpp:15-27, Morse :29-40, Coulomb
-// :42-51).  Site constants are pre-scaled so the LJ force needs no extra
-// multiply: a.w = 6 es6 | 6 De, b.x = 6 es12 | a log2(e), b.y = qq,
-// b.z = - | re, b.w = - | 2 De a.  Returns the force on the tip and 6x the
-// energy.   SEG 0 LJ + Coulomb, 1 LJ, 2 Morse + Coulomb, 3 Morse
-template <int SEG>
-__device__ inline void pair32(const float4 a, const float4 b, float tx, float ty, float tz, float& fx, float& fy,
-                              float& fz, float& e6) {
+// :42-51), accumulated into acc: the force on the tip into components 0-2
+// and, when E, 12x the energy into component 3.  Site constants are
+// pre-scaled so that each quantity is one FMA away from the powers of 1/r:
+//   LJ     a.w = 6 es6, b.x = 12 es12
+//          F/r^2 factor fl = 12 es12 / r^6 - 6 es6 (then / r^8),
+//          12 E = (fl - 6 es6) / r^6
+//   Morse  a.w = 12 De, b.x = a log2(e), b.z = a log2(e) re, b.w = 2 De a
+//          x = exp(-a (r - re)) = exp2(b.z - b.x r),
+//          F/r = 2 De a x (x - 1) / r, 12 E = 12 De (x - 1)^2
+//   Coulomb b.y = qq: E = qq / r, F/r = E / r^2
+// SEG 0 LJ + Coulomb, 1 LJ, 2 Morse + Coulomb, 3 Morse.
+template <int SEG, bool E, int MODE>
+__device__ inline void pair32(const float4 a, const float4 b, float tx, float ty, float tz, Acc32<MODE>& acc) {
     const float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     float fs;
@@ -179,34 +192,33 @@ __device__ inline void pair32(const float4 a, const float4 b, float tx, float ty
             ir2 = __builtin_amdgcn_rcpf(r2);
         }
         const float ir6 = ir2 * ir2 * ir2;
-        const float w = b.x * ir6;              // 6 es12 / r^6
-        const float pl = w - a.w;               // 6 (es12 / r^6 - es6)
-        const float el = pl * ir6;              // 6 E_LJ
-        const float fl = w + pl;                // (w + pl) / r^6 = 12 es12 / r^12 - 6 es6 / r^6
+        const float fl = fmaf(b.x, ir6, -a.w);
+        if (E) acc.madd(3, ir6, fl - a.w);
         if (SEG == 0) {
-            const float ec = b.y * ir;          // E_Coulomb; force factor E_C / r^2
+            const float ec = b.y * ir;
             fs = fmaf(fl, ir6, ec) * ir2;
-            e6 = fmaf(6.f, ec, el);
+            if (E) acc.madd(3, 12.f, ec);
         } else {
             fs = fl * ir6 * ir2;
-            e6 = el;
         }
     } else {
         const float ir = __builtin_amdgcn_rsqf(r2);
         const float r = r2 * ir;
-        const float x = __builtin_amdgcn_exp2f(b.x * (b.z - r));   // hardware exp2; results below 2^-126 flush to 0
-        const float xm = x - 1.f;
-        fs = b.w * x * xm * ir;                 // 2 De a x (x - 1) / r
-        e6 = a.w * xm * xm;                     // 6 De (x - 1)^2
+        const float x = __builtin_amdgcn_exp2f(fmaf(-b.x, r, b.z));   // hardware exp2; below 2^-126 flushes to 0
+        fs = b.w * fmaf(x, x, -x) * ir;
+        if (E) {
+            const float xm = x - 1.f;
+            acc.madd(3, a.w * xm, xm);
+        }
         if (SEG == 2) {
             const float ec = b.y * ir;
             fs = fmaf(ec, ir * ir, fs);
-            e6 = fmaf(6.f, ec, e6);
+            if (E) acc.madd(3, 12.f, ec);
         }
     }
-    fx = fs * dx;
-    fy = fs * dy;
-    fz = fs * dz;
+    acc.madd(0, fs, dx);
+    acc.madd(1, fs, dy);
+    acc.madd(2, fs, dz);
 }
 
 // Sites t = gl, gl + S, ... of one segment against every periodic image of
@@ -238,10 +250,7 @@ __device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<SC>& s, int n, in
             }
 #pragma unroll
             for (int u = 0; u < B; ++u) {
-                float fx, fy, fz, e;
-                pair32<SEG>(sa[u], sb[u], ix, iy, iz, fx, fy, fz, e);
-                acc.add(0, fx); acc.add(1, fy); acc.add(2, fz);
-                if (E) acc.add(3, e);
+                pair32<SEG, E>(sa[u], sb[u], ix, iy, iz, acc);
             }
         }
         acc.template end<E>();
@@ -258,10 +267,7 @@ __device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<SC>& s, int n, in
                 iy -= M->shift32[im][1];
                 iz -= M->shift32[im][2];
             }
-            float fx, fy, fz, e;
-            pair32<SEG>(sa, sb, ix, iy, iz, fx, fy, fz, e);
-            acc.add(0, fx); acc.add(1, fy); acc.add(2, fz);
-            if (E) acc.add(3, e);
+            pair32<SEG, E>(sa, sb, ix, iy, iz, acc);
         }
         if (IMG) acc.template end<E>();
     }
@@ -304,10 +310,11 @@ __device__ void site_sums32(cptr<DevMol> M, const Sites32<SC> s0, double px, dou
     out[0] = acc.get(0);
     out[1] = acc.get(1);
     out[2] = acc.get(2);
-    out[3] = E ? acc.get(3) * (1.0 / 6.0) : 0.0;
+    out[3] = E ? acc.get(3) * (1.0 / 12.0) : 0.0;
 }
 
-// The FP64 twin of pair32 (same pre-scaled constants, SiteF64::p3 = 2 De a).
+// The FP64 site term: SiteF64 constants p0 = 6 es6 | 6 De, p1 = 6 es12 | a,
+// p2 = - | re, p3 = - | 2 De a; returns the force and 6x the energy.
 template <int SEG>
 __device__ inline void pair64(const SiteF64& st, double tx, double ty, double tz, double& fx, double& fy,
                               double& fz, double& e6) {
