@@ -528,11 +528,33 @@ __device__ inline int nth_set_bit(uint64_t m, int k) {
     return pos;
 }
 
+// v of the lane paired with this one by DPP pattern CTRL (a register move
+// across lanes, no LDS round trip); every lane must be active.
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum of v over aligned groups of n lanes (n a power of two, wave-uniform),
+// the same value on every lane of a group: butterflies within a quad (lane
+// ^ 1, lane ^ 2), then mirrors within 8 and 16 lanes (after the quad sums
+// every lane of a quad holds the same value, so a mirror pairs each lane
+// with the other half), then cross-row shuffles.  Called with all lanes
+// active.
+__device__ inline double lanes_sum(double v, int n) {
+    if (n >= 2) v += dpp_f64<0xB1>(v);     // quad_perm [1, 0, 3, 2]
+    if (n >= 4) v += dpp_f64<0x4E>(v);     // quad_perm [2, 3, 0, 1]
+    if (n >= 8) v += dpp_f64<0x141>(v);    // row_half_mirror: i <-> 7 - i
+    if (n >= 16) v += dpp_f64<0x140>(v);   // row_mirror: i <-> 15 - i
+    for (int m = 16; m < n; m <<= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
 template <int G>
 __device__ inline double group_sum(double v) {
-#pragma unroll
-    for (int m = 1; m < G; m <<= 1) v += __shfl_xor(v, m, 64);
-    return v;
+    return lanes_sum(v, G);
 }
 
 // checkConvergence (minimiser.cpp:15-37); |F| < ftol tested as |F|^2 < ftol^2
@@ -755,8 +777,7 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
                         else site_sums64<false, false>(M, src64, qx, qy, qz, sub, Ge, part);
                     }
                 }
-                for (int w2 = 1; w2 < Ge; w2 <<= 1)
-                    for (int q = 0; q < 4; ++q) part[q] += __shfl_xor(part[q], w2, 64);
+                for (int q = 0; q < 4; ++q) part[q] = lanes_sum(part[q], Ge);
                 // group of rank k (k-th active leader) reads slot k
                 const int rank = __popcll(lead_act & ((1ull << gbase) - 1));
                 const int from = (rank * Ge) & 63;
