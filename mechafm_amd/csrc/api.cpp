@@ -185,6 +185,8 @@ static mafm_status new_grid(mafm_ctx* c, const mafm_grid_desc* g, HostGrid& h) {
     memcpy(h.basis, g->basis, sizeof h.basis);
     h.orthogonal = is_diag(h.basis);
     inv3(h.basis, h.inv);
+    if (h.orthogonal)   // exact reciprocals of the spacings (the fast lookups multiply by them)
+        for (int d = 0; d < 3; ++d) h.inv[4 * d] = 1.0 / h.basis[4 * d];
     size_t np = (size_t)h.n[0] * h.n[1] * h.n[2];
     HIPCHK(c, hipMalloc(&h.fe64, np * sizeof(double4)));
     HIPCHK(c, hipMalloc(&h.fe32, np * sizeof(float4)));

@@ -415,14 +415,19 @@ __device__ void site_sums64(cptr<DevMol> M, const Sites<SC, SiteF64> s0, double 
 
 // Trilinear lookup of ForceGrid::interpolate (force_grid.cpp:88-155) with the
 // clamp / wrap rules of getGridPoint / getGridPointIndex / pbcGridPoint
-// (force_grid.cpp:158-275); FP64 weights, FP32 or FP64 samples.
+// (force_grid.cpp:158-275).  Fractional coordinates in FP64; on a diagonal
+// basis they are (p - offset) times the reciprocal spacing where the
+// reference divides (a last-place difference that can only move a point
+// across a cell face, where the interpolant is continuous).  FP32 samples
+// are weighted and summed in FP32 (weights from the FP64 fractions), FP64
+// samples in FP64.
 template <bool F32>
 __device__ void grid_lookup(const DevGrid& g, double px, double py, double pz, double out[4], int& flags) {
     double q[3];
     if (g.orthogonal) {
-        q[0] = (px - g.offset[0]) / g.basis[0];
-        q[1] = (py - g.offset[1]) / g.basis[4];
-        q[2] = (pz - g.offset[2]) / g.basis[8];
+        q[0] = (px - g.offset[0]) * g.inv[0];
+        q[1] = (py - g.offset[1]) * g.inv[4];
+        q[2] = (pz - g.offset[2]) * g.inv[8];
     } else {
         mat_vec(g.inv, px - g.offset[0], py - g.offset[1], pz - g.offset[2], q[0], q[1], q[2]);
     }
@@ -449,27 +454,38 @@ __device__ void grid_lookup(const DevGrid& g, double px, double py, double pz, d
         i1[d] = a1 < 0 ? 0 : (a1 >= g.n[d] ? g.n[d] - 1 : a1);
     }
     const double w[3] = {q[0] - gp[0], q[1] - gp[1], q[2] - gp[2]};
-    double acc[4] = {0, 0, 0, 0};
-    for (int c = 0; c < 8; ++c) {
-        const int ix = (c & 4) ? i1[0] : i0[0];
-        const int iy = (c & 2) ? i1[1] : i0[1];
-        const int iz = (c & 1) ? i1[2] : i0[2];
-        const double wt = ((c & 4) ? w[0] : 1 - w[0]) * ((c & 2) ? w[1] : 1 - w[1]) * ((c & 1) ? w[2] : 1 - w[2]);
-        const long idx = ((long)ix * g.n[1] + iy) * g.n[2] + iz;
-        double v0, v1, v2, v3;
-        if (F32) {
-            float4 v = g.fe32[idx];
-            v0 = v.x; v1 = v.y; v2 = v.z; v3 = v.w;
-        } else {
-            double4 v = g.fe64[idx];
-            v0 = v.x; v1 = v.y; v2 = v.z; v3 = v.w;
+    if (F32) {
+        const float wf[3] = {(float)w[0], (float)w[1], (float)w[2]};
+        float acc[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < 8; ++c) {
+            const int ix = (c & 4) ? i1[0] : i0[0];
+            const int iy = (c & 2) ? i1[1] : i0[1];
+            const int iz = (c & 1) ? i1[2] : i0[2];
+            const float wt = ((c & 4) ? wf[0] : 1.f - wf[0]) * ((c & 2) ? wf[1] : 1.f - wf[1]) *
+                             ((c & 1) ? wf[2] : 1.f - wf[2]);
+            const float4 v = g.fe32[((long)ix * g.n[1] + iy) * g.n[2] + iz];
+            acc[0] = fmaf(wt, v.x, acc[0]);
+            acc[1] = fmaf(wt, v.y, acc[1]);
+            acc[2] = fmaf(wt, v.z, acc[2]);
+            acc[3] = fmaf(wt, v.w, acc[3]);
         }
-        acc[0] = fma(wt, v0, acc[0]);
-        acc[1] = fma(wt, v1, acc[1]);
-        acc[2] = fma(wt, v2, acc[2]);
-        acc[3] = fma(wt, v3, acc[3]);
+        for (int k = 0; k < 4; ++k) out[k] += (double)acc[k];
+    } else {
+        double acc[4] = {0, 0, 0, 0};
+        for (int c = 0; c < 8; ++c) {
+            const int ix = (c & 4) ? i1[0] : i0[0];
+            const int iy = (c & 2) ? i1[1] : i0[1];
+            const int iz = (c & 1) ? i1[2] : i0[2];
+            const double wt = ((c & 4) ? w[0] : 1 - w[0]) * ((c & 2) ? w[1] : 1 - w[1]) * ((c & 1) ? w[2] : 1 - w[2]);
+            const long idx = ((long)ix * g.n[1] + iy) * g.n[2] + iz;
+            const double4 v = g.fe64[idx];
+            acc[0] = fma(wt, v.x, acc[0]);
+            acc[1] = fma(wt, v.y, acc[1]);
+            acc[2] = fma(wt, v.z, acc[2]);
+            acc[3] = fma(wt, v.w, acc[3]);
+        }
+        for (int k = 0; k < 4; ++k) out[k] += acc[k];
     }
-    for (int k = 0; k < 4; ++k) out[k] += acc[k];
 }
 
 // Tip-dummy VDW, Coulomb and lateral spring on the tip (simulation.cpp:503-515,
