@@ -1,5 +1,6 @@
 // C ABI of libmafm: context, device tables, launches.  Host code only; the
-// kernels live in scan_plain.hip, scan_fast.hip and fft.hip.
+// kernels live in scan_plain.hip, scan_fast.hpp (instantiated by scan_fast_mid.hip and
+// scan_fast_gen.hip), fft.hip, adjoint.hip, deltaf.hip and util.hip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

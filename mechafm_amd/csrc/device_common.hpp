@@ -28,7 +28,7 @@ struct Site64 {
     int32_t coul;     // Coulomb term present
 };
 
-// Fast-path FP64 site, constants pre-scaled like Site32 (p0 = 6 es6 | 6 De,
+// Fast-path FP64 site, constants pre-scaled for pair64 (p0 = 6 es6 | 6 De,
 // p1 = 6 es12 | a, p2 = - | re, p3 = - | 2 De a).
 struct SiteF64 {
     double x, y, z;
