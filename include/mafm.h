@@ -226,6 +226,9 @@ typedef struct {
 
 /* ---- entry points ------------------------------------------------------ */
 mafm_status mafm_create(int device, mafm_ctx** out);
+/* Frees the context and its device memory. Simulations opened on it
+ * (mafm_sim_open) keep it alive: while any is open, the destruction is
+ * completed by the last mafm_sim_close. */
 void mafm_destroy(mafm_ctx* ctx);
 const char* mafm_last_error(const mafm_ctx* ctx);
 mafm_status mafm_device_info(mafm_ctx* ctx, mafm_device_props* props);

@@ -70,6 +70,8 @@ mafm_status mafm_sim_write_output(mafm_sim* sim, int threads);
  * amplitude, cantilever and frequency */
 mafm_status mafm_sim_write_deltaf(mafm_sim* sim, const mafm_deltaf_desc* desc);
 const char* mafm_sim_last_error(const mafm_sim* sim);
+/* frees the simulation and the grids it built; completes a pending
+ * mafm_destroy of its context when it is the last simulation open there */
 void mafm_sim_close(mafm_sim* sim);
 
 #ifdef __cplusplus

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/mafm.h"
+#include "ctx_ref.hpp"
 #include "device_common.hpp"
 
 namespace mafm {
@@ -135,6 +136,8 @@ struct mafm_ctx {
     bool have_scan = false;
     std::map<int, DevBuf<double2>> twiddles;   // FFT twiddle tables by length
     DevBuf<int> d_flag;
+    int sims = 0;                  // open simulations on this context (mafm_sim_open)
+    bool destroy_pending = false;  // mafm_destroy called while sims > 0
 };
 
 static mafm_status fail(mafm_ctx* c, mafm_status s, const char* fmt, ...) {
@@ -343,8 +346,7 @@ mafm_status mafm_create(int device, mafm_ctx** out) {
     return MAFM_OK;
 }
 
-void mafm_destroy(mafm_ctx* c) {
-    if (!c) return;
+static void destroy_now(mafm_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (auto& g : c->grids) {
@@ -369,6 +371,15 @@ void mafm_destroy(mafm_ctx* c) {
     (void)hipEventDestroy(c->ev1);
     (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+void mafm_destroy(mafm_ctx* c) {
+    if (!c) return;
+    if (c->sims > 0) {              // finished by the last mafm_sim_close on this context
+        c->destroy_pending = true;
+        return;
+    }
+    destroy_now(c);
 }
 
 const char* mafm_last_error(const mafm_ctx* c) { return c ? c->err.c_str() : "no context"; }
@@ -940,3 +951,15 @@ mafm_status mafm_adjoint(mafm_ctx* c, int system, const mafm_adjoint_desc* d, ma
 }
 
 }  // extern "C"
+
+namespace mafm {
+
+void ctx_retain(mafm_ctx* c) {
+    if (c) ++c->sims;
+}
+
+void ctx_release(mafm_ctx* c) {
+    if (c && --c->sims == 0 && c->destroy_pending) destroy_now(c);
+}
+
+}  // namespace mafm

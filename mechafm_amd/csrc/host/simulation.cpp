@@ -18,6 +18,7 @@
 #include <sstream>
 #include <thread>
 
+#include "../ctx_ref.hpp"
 #include "format.hpp"
 
 namespace mafm {
@@ -971,10 +972,21 @@ void Simulation::writeDeltaF(const mafm_deltaf_desc& d) {
 using mafm::Error;
 using mafm::Simulation;
 
+// A simulation's reference on its context (released after the simulation's
+// own destructor has freed the grids it built there).
+struct CtxRef {
+    mafm_ctx* c;
+    explicit CtxRef(mafm_ctx* x) : c(x) { mafm::ctx_retain(c); }
+    ~CtxRef() { mafm::ctx_release(c); }
+    CtxRef(const CtxRef&) = delete;
+    CtxRef& operator=(const CtxRef&) = delete;
+};
+
 struct mafm_sim {
+    CtxRef ref;   // declared first, destroyed last
     Simulation sim;
     std::string err;
-    explicit mafm_sim(mafm_ctx* c) : sim(c) {}
+    explicit mafm_sim(mafm_ctx* c) : ref(c), sim(c) {}
 };
 
 template <typename F>

@@ -84,6 +84,39 @@ def test_lanes_per_pixel_invariance(gpu_ctx, G, tmp_path):
     assert stats["lanes_per_pixel"] == G
 
 
+@pytest.mark.parametrize("precision", ["plain64", "fast64", "fast32"])
+def test_scan_past_force_grid_is_an_error(gpu_ctx, tmp_path, precision):
+    """A lookup that needs a grid point past a non-periodic force grid stops
+    the reference with an error (force_grid.cpp:236-240): the scan returns
+    MAFM_ERR_OUT_OF_GRID (every gather stays clamped inside the grid), and
+    the context runs the original scan again afterwards."""
+    from mechafm_amd import abi
+    sim, tab, stats, shape = _run(gpu_ctx, "c2_mini", tmp_path, precision,
+                                  options=abi.SCAN_REPRODUCIBLE)
+    d = sim.scan_desc(precision, options=abi.SCAN_REPRODUCIBLE)
+    d.nx += 30                      # x up to 4 A: past the 1 A area and the grid's 1.5 A margin
+    with pytest.raises(abi.MafmError) as err:
+        gpu_ctx.scan(d)
+    assert err.value.status == 3, err.value
+    assert "past the grid" in str(err.value)
+    again = sim.run(precision, options=abi.SCAN_REPRODUCIBLE)
+    assert again["total_steps"] == stats["total_steps"]
+    np.testing.assert_array_equal(abi.records_to_table(sim.records(), shape), tab)
+
+
+def test_context_closed_before_simulation(tmp_path):
+    """mafm_destroy of a context that a simulation still uses is completed by
+    that simulation's mafm_sim_close: the simulation keeps working in between
+    and frees the force grid it built on a live context."""
+    from mechafm_amd import abi
+    ctx = abi.Context(0)
+    sim = abi.Simulation(materialize("c2_mini", str(tmp_path)), ctx)
+    ctx.close()
+    st = sim.run("fast64")
+    assert st["total_steps"] > 0
+    sim.close()
+
+
 @pytest.mark.parametrize("case", [c for c in CASES if golden_eval(c) is not None])
 def test_eval_tip_surface_matches_reference(gpu_ctx, case, tmp_path):
     from mechafm_amd import abi
