@@ -88,4 +88,4 @@ def test_autograd_matches_finite_differences(gpu_ctx, tmp_path):
         checks.append((f"pos[{j},{c}]", float(pos.grad[j, c]), (L(pos=p_p) - L(pos=p_m)) / 2e-5))
     for name, ad, fd in checks:
         print(f"{name}: adjoint {ad:.6e} FD {fd:.6e}")
-        assert ad == pytest.approx(fd, rel=1e-3, abs=1e-9 * float(loss)), name
+        assert ad == pytest.approx(fd, rel=1e-3, abs=1e-9 * float(loss.detach())), name
