@@ -239,8 +239,8 @@ def main():
                     help="the adjoint workload (150 atoms, 128x128 px x 30 z): gradient timing and finite differences")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (pixel columns would shard "
-              "across GPUs with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (pixel columns would shard "
+              "across GPUs with no exchange during the scan). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_bytes:
