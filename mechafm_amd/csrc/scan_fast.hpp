@@ -793,11 +793,16 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
                         else site_sums64<false, false>(M, src64, qx, qy, qz, sub, Ge, part);
                     }
                 }
-                for (int q = 0; q < 4; ++q) part[q] = lanes_sum(part[q], Ge);
+                // the energy (q = 3) only on trips that use it
+#pragma unroll
+                for (int q = 0; q < 3; ++q) part[q] = lanes_sum(part[q], Ge);
+                if (need_e) part[3] = lanes_sum(part[3], Ge);
                 // group of rank k (k-th active leader) reads slot k
                 const int rank = __popcll(lead_act & ((1ull << gbase) - 1));
                 const int from = (rank * Ge) & 63;
-                for (int q = 0; q < 4; ++q) r[q] = __shfl(part[q], from, 64);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) r[q] = __shfl(part[q], from, 64);
+                r[3] = need_e ? __shfl(part[3], from, 64) : 0.0;
             } else {
                 if constexpr (F32) {
                     Sites32<SC> src{a.seg32a + M->seg_off, a.seg32b + M->seg_off, lds_raw, lds_raw + lds_n};
@@ -809,7 +814,9 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
                     else site_sums64<SC, false>(M, src, Px, Py, Pz, gl, G, r);
                 }
                 if (G > 1) {
-                    for (int q = 0; q < 4; ++q) r[q] = group_sum<G>(r[q]);
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) r[q] = group_sum<G>(r[q]);
+                    if (need_e) r[3] = group_sum<G>(r[3]);
                 }
             }
             if (M->potential_grid >= 0) grid_lookup<F32>(a.grids[M->potential_grid], Px, Py, Pz, r, flags);
