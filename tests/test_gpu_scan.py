@@ -143,6 +143,32 @@ def test_batch_equals_single_runs(gpu_ctx, tmp_path):
         assert (single == b).all()
 
 
+@pytest.mark.parametrize("precision", ["plain64", "fast64", "fast32"])
+def test_system_without_sites(gpu_ctx, tmp_path, precision):
+    """A system with no tip-surface sites, batched ahead of a real one: its
+    tip only feels the tip-dummy terms and starts at their LJ minimum
+    straight under the dummy, so every record has zero tip-surface force and
+    energy, no lateral offset and rz = -d; the real system's records equal
+    its single run bit for bit."""
+    from mechafm_amd import abi
+    sim = abi.Simulation(materialize("c1_mini", str(tmp_path)), gpu_ctx)
+    kw = dict(lanes_per_pixel=2, options=abi.SCAN_REPRODUCIBLE)
+    sim.run(precision, **kw)
+    single = sim.records().copy()
+    full = sim.system()
+    empty = abi.SystemDesc.from_buffer_copy(full)
+    empty.n_sites = 0
+    gpu_ctx.set_systems([empty, full])
+    st = gpu_ctx.scan(sim.scan_desc(precision, **kw))
+    n = len(single)
+    rec0, rec1 = gpu_ctx.records(0, n), gpu_ctx.records(n, n)
+    for f in ("fx", "fy", "fz", "e", "rx", "ry"):
+        assert (rec0[f] == 0).all(), f
+    assert np.abs(rec0["rz"] + full.tip_dummy_d).max() < 1e-9
+    assert (rec0["n_steps"] >= 1).all() and st["unconverged"] == 0
+    assert (rec1 == single).all()
+
+
 @pytest.mark.parametrize("precision", ["fast32", "fast64"])
 def test_reproducible_runs_are_identical(gpu_ctx, tmp_path, precision):
     from mechafm_amd import abi, workloads as W
