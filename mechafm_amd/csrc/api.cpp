@@ -257,12 +257,14 @@ static mafm_status build_tables(mafm_ctx* c, int n, const mafm_system_desc* sys,
         if (s.n_images < 0 || s.n_images > 27 || (s.n_images > 0 && !s.image_shifts))
             return fail(c, MAFM_ERR_INVALID, "system %d: n_images must be 0..27 with shifts", m);
         d.n_images = s.n_images > 0 ? s.n_images : 1;
+        d.planar = 1;
         for (int k = 0; k < d.n_images; ++k) {
             for (int q = 0; q < 3; ++q) {
                 d.shift[k][q] = s.n_images > 0 ? s.image_shifts[3 * k + q] : 0.0;
                 d.shift32[k][q] = (float)d.shift[k][q];
             }
             d.shift32[k][3] = 0.f;
+            if (d.shift32[k][2] != 0.f) d.planar = 0;
         }
         // FP32 coordinate origin: centre of the sites' bounding box
         double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};

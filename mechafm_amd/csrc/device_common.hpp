@@ -67,7 +67,7 @@ struct DevMol {
     double origin[3];
     int64_t rec_off;                       // first record of this molecule
     int32_t n_images;                      // >= 1; image 0 of a non-periodic system is a zero shift
-    int32_t pad;
+    int32_t planar;                        // 1: every FP32 image shift has z = 0 (in-plane images)
     double shift[27][3];                   // periodic image shifts, evaluated shift-major
     float shift32[27][4];                  // the same shifts in FP32 (x, y, z, 0)
 };

@@ -178,10 +178,12 @@ __device__ inline void load_site(const Sites32<SC>& s, int t, float4& a, float4&
 //          F/r = 2 De a x (x - 1) / r, 12 E = 12 De (x - 1)^2
 //   Coulomb b.y = qq: E = qq / r, F/r = E / r^2
 // SEG 0 LJ + Coulomb, 1 LJ, 2 Morse + Coulomb, 3 Morse.
+// pair32_d takes the separation d = tip - site and r2 = |d|^2 (computed as
+// fma(dx, dx, fma(dy, dy, dz * dz)), so callers that hoist dz * dz out of an
+// image loop get the same bits).
 template <int SEG, bool E, int MODE>
-__device__ inline void pair32(const float4 a, const float4 b, float tx, float ty, float tz, Acc32<MODE>& acc) {
-    const float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
-    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+__device__ inline void pair32_d(const float4 a, const float4 b, float dx, float dy, float dz, float r2,
+                                Acc32<MODE>& acc) {
     float fs;
     if (SEG <= 1) {
         float ir = 0.f, ir2;
@@ -221,6 +223,12 @@ __device__ inline void pair32(const float4 a, const float4 b, float tx, float ty
     acc.madd(2, fs, dz);
 }
 
+template <int SEG, bool E, int MODE>
+__device__ inline void pair32(const float4 a, const float4 b, float tx, float ty, float tz, Acc32<MODE>& acc) {
+    const float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
+    pair32_d<SEG, E>(a, b, dx, dy, dz, fmaf(dx, dx, fmaf(dy, dy, dz * dz)), acc);
+}
+
 // Sites t = gl, gl + S, ... of one segment against every periodic image of
 // the tip: full blocks of 6 (LJ) or 4 (Morse) sites per lane (loaded once,
 // reused for all images, no bound test), then the remainder one site at a
@@ -241,16 +249,34 @@ __device__ inline void seg_sum32(cptr<DevMol> M, const Sites32<SC>& s, int n, in
 #pragma unroll
         for (int u = 0; u < B; ++u) load_site<SEG, SC>(s, t0 + u * S + gl, sa[u], sb[u]);
         acc.begin();
-        for (int im = 0; im < n_img; ++im) {
-            float ix = tx, iy = ty, iz = tz;
-            if (IMG) {
-                ix -= M->shift32[im][0];
-                iy -= M->shift32[im][1];
-                iz -= M->shift32[im][2];
-            }
+        if (IMG && M->planar) {
+            // in-plane images: dz and dz^2 of a site are the same for all of them
+            float dz[B], dz2[B];
 #pragma unroll
             for (int u = 0; u < B; ++u) {
-                pair32<SEG, E>(sa[u], sb[u], ix, iy, iz, acc);
+                dz[u] = tz - sa[u].z;
+                dz2[u] = dz[u] * dz[u];
+            }
+            for (int im = 0; im < n_img; ++im) {
+                const float ix = tx - M->shift32[im][0], iy = ty - M->shift32[im][1];
+#pragma unroll
+                for (int u = 0; u < B; ++u) {
+                    const float dx = ix - sa[u].x, dy = iy - sa[u].y;
+                    pair32_d<SEG, E>(sa[u], sb[u], dx, dy, dz[u], fmaf(dx, dx, fmaf(dy, dy, dz2[u])), acc);
+                }
+            }
+        } else {
+            for (int im = 0; im < n_img; ++im) {
+                float ix = tx, iy = ty, iz = tz;
+                if (IMG) {
+                    ix -= M->shift32[im][0];
+                    iy -= M->shift32[im][1];
+                    iz -= M->shift32[im][2];
+                }
+#pragma unroll
+                for (int u = 0; u < B; ++u) {
+                    pair32<SEG, E>(sa[u], sb[u], ix, iy, iz, acc);
+                }
             }
         }
         acc.template end<E>();
