@@ -45,6 +45,8 @@ CASES = {
     "c1_maxsteps": (W.make_c1, dict(mini=True, area="0.5 0.5", maxsteps="20"), 0),
     "c1_pbc": (W.make_c1, dict(mini=True, area="0.5 0.5", vdw_pbc="on", cell_a="14.0 0.0 0.0",
                                cell_b="0.0 14.0 0.0", cell_c="0.0 0.0 30.0", center="7.0 7.0"), 0),
+    "c1_pbc_tilt": (W.make_c1, dict(mini=True, area="0.4 0.4", vdw_pbc="on", cell_a="14.0 0.0 0.5",
+                                    cell_b="0.0 14.0 -0.3", cell_c="0.0 0.0 30.0", center="7.0 7.0"), 0),
     "c2_mini": (W.make_c2, dict(mini=True), 512),
     "c2_direct": (W.make_c2, dict(mini=True, rigidgrid="off", area="0.6 0.6"), 512),
     "c2_morse": (W.make_c2, dict(mini=True, rigidgrid="off", coulomb="off", area="0.6 0.6"), 512),
