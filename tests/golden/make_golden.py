@@ -50,6 +50,9 @@ CASES = {
     "c2_mini": (W.make_c2, dict(mini=True), 512),
     "c2_direct": (W.make_c2, dict(mini=True, rigidgrid="off", area="0.6 0.6"), 512),
     "c2_morse": (W.make_c2, dict(mini=True, rigidgrid="off", coulomb="off", area="0.6 0.6"), 512),
+    "c2_morse_pbc": (W.make_c2, dict(mini=True, rigidgrid="off", coulomb="off", area="0.4 0.4", vdw_pbc="on",
+                                     cell_a="16.0 0.0 0.0", cell_b="0.0 16.0 0.0", cell_c="0.0 0.0 30.0",
+                                     center="0.2 0.2"), 0),
     "c3_mol": (lambda d, **k: W.make_c3(d, count=3, **k)[2], dict(mini=True), 0),
     "c4_mini": (W.make_c4, dict(mini=True), 512),
     "c5_mini": (W.make_c5, dict(mini=True), 0),
