@@ -6,12 +6,14 @@ A step is one full scan of the workload (every pixel column, every height,
 FIRE + midpoint to convergence) in one launch of the work-stealing kernel,
 with the molecules' tables already resident in HBM.  Default workload: the
 batched one (64 seeded planar molecules, N ~ U[20, 400], LJ + Coulomb,
-64 x 64 px x 40 z each), FP32 pair terms with Kahan-compensated sums and
-FP64 minimiser state.
+64 x 64 px x 40 z each), FP32 pair terms summed in register blocks folded
+into FP64, FP64 minimiser state.  The FP64 workloads (c1, c2) default to the
+FP64 fast path.
 
 Printed JSON line: metric/value (relaxed tip positions per second, whole
-job), the roofline of the scan kernel (algorithmic FP32 VALU FLOPs per launch
-over its HIP-event duration, against the 157.3 TFLOP/s FP32 vector peak),
+job), the roofline of the scan kernel (algorithmic VALU FLOPs per launch
+over its HIP-event duration, against the 157.3 TFLOP/s FP32 or 78.6 TFLOP/s
+FP64 vector peak; lookup bytes against 8 TB/s for a force-grid scan),
 the plain FP64 GPU scan on the same workload (speed and the fraction of
 positions within 1e-4 A / 1e-3 relative Fz of it), and a CPU baseline: the
 reference program timed on this host on a sample of the same workload.
@@ -31,8 +33,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_VALU_PEAK = 157.3e12      # MI355X FP32 vector, FLOP/s
+FP64_VALU_PEAK = 78.6e12       # MI355X FP64 vector, FLOP/s
 HBM_PEAK = 8.0e12              # MI355X HBM3E, B/s
-LOOKUP_BYTES = 8 * 16          # one trilinear lookup: 8 corners of {fx, fy, fz, e} in FP32
+LOOKUP_BYTES = 8 * 16          # one trilinear lookup: 8 corners of {fx, fy, fz, e} in FP32 (x2 in FP64)
 # algorithmic FLOPs per pair term and per evaluation / iteration overheads
 PAIR_FLOPS = {(0, 0): 26, (0, 1): 31, (1, 0): 28, (1, 1): 33}   # (vdw, coulomb)
 TIP_DUMMY_FLOPS = 50
@@ -65,11 +68,13 @@ def algorithmic_flops(sims, recs, integrator=1):
     return tot
 
 
-def grid_lookup_bytes(sims, recs, integrator=1):
+def grid_lookup_bytes(sims, recs, integrator=1, fp64=False):
     """Algorithmic bytes of the force-grid lookups of a rigidgrid scan: one
-    lookup per evaluation (E per minimiser iteration + the final one)."""
+    lookup per evaluation (E per minimiser iteration + the final one), FP32
+    samples for fast32, FP64 samples otherwise."""
     e = EVALS_PER_ITER[integrator]
-    return float(sum((r["n_steps"].astype(np.int64).sum() * e + len(r)) * LOOKUP_BYTES for r in recs))
+    per = LOOKUP_BYTES * (2 if fp64 else 1)
+    return float(sum((r["n_steps"].astype(np.int64).sum() * e + len(r)) * per for r in recs))
 
 
 def run_batch(abi, sims, prec, **kw):
@@ -222,7 +227,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
-    ap.add_argument("--precision", default="fast32", choices=["fast32", "fast64", "plain64"])
+    ap.add_argument("--precision", default=None, choices=["fast32", "fast64", "plain64"],
+                    help="default: fast64 for the FP64 workloads (c1, c2), fast32 for the others")
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--group", type=int, default=0, help="waves per workgroup (0 = automatic)")
@@ -238,6 +244,8 @@ def main():
     ap.add_argument("--adjoint", action="store_true",
                     help="the adjoint workload (150 atoms, 128x128 px x 30 z): gradient timing and finite differences")
     args = ap.parse_args()
+    if args.precision is None:
+        args.precision = "fast64" if args.config in ("c1", "c2") else "fast32"
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (pixel columns would shard "
               "across GPUs with no exchange during the scan). Run with --gpus 1.", file=sys.stderr)
@@ -298,6 +306,7 @@ def main():
         flops = algorithmic_flops(sims, recs)
         k_avg = float(np.mean(kms)) / 1e3
         achieved = flops / k_avg
+        peak = FP32_VALU_PEAK if args.precision == "fast32" else FP64_VALU_PEAK
         out = {
             "metric": "relaxed tip positions/s at 1e-4 A error; % FP32 VALU peak vs plain FP64",
             "value": value, "unit": "positions/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -313,13 +322,13 @@ def main():
                 "group_utilisation": st["active_group_trips"] / max(1, st["group_trips"]),
                 "wide_trip_fraction": st["wide_trips"] * 64 / st["lanes_per_pixel"] / max(1, st["group_trips"]),
                 "unconverged": st["unconverged"]},
-            "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": FP32_VALU_PEAK / 1e12,
-                         "unit": "TFLOP/s", "frac": achieved / FP32_VALU_PEAK, "traffic": None,
+            "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
+                         "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
                          "kernel_ms": k_avg * 1e3, "flops_per_launch": flops},
         }
         if info["rigidgrid"]:
             # force-grid scan: no site terms, the work is the trilinear gathers
-            gb = grid_lookup_bytes(sims, recs)
+            gb = grid_lookup_bytes(sims, recs, fp64=args.precision != "fast32")
             out["roofline"] = {"bound": "hbm", "achieved": gb / k_avg / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                                "frac": gb / k_avg / HBM_PEAK, "traffic": None, "kernel_ms": k_avg * 1e3,
                                "bytes_per_launch": gb}
