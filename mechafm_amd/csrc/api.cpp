@@ -301,11 +301,11 @@ static mafm_status build_tables(mafm_ctx* c, int n, const mafm_system_desc* sys,
                 f.a = make_float4((float)(t.x - d.origin[0]), (float)(t.y - d.origin[1]),
                                   (float)(t.z - d.origin[2]), 0.f);
                 if (t.vdw == MAFM_VDW_LJ) {
-                    g.p0 = 6 * t.p0; g.p1 = 6 * t.p1; g.p2 = 0; g.p3 = 0;
+                    g.p0 = 6 * t.p0; g.p1 = 12 * t.p1; g.p2 = 0; g.p3 = 0;
                     f.a.w = (float)(6 * t.p0);
                     f.b = make_float4((float)(12 * t.p1), (float)qq, 0.f, 0.f);
                 } else {
-                    g.p0 = 6 * t.p0; g.p1 = t.p1; g.p2 = t.p2; g.p3 = 2 * t.p0 * t.p1;
+                    g.p0 = 12 * t.p0; g.p1 = t.p1; g.p2 = t.p2; g.p3 = 2 * t.p0 * t.p1;
                     f.a.w = (float)(12 * t.p0);
                     f.b = make_float4((float)(t.p1 * LOG2E), (float)qq, (float)(t.p1 * LOG2E * t.p2), (float)g.p3);
                 }

@@ -28,8 +28,8 @@ struct Site64 {
     int32_t coul;     // Coulomb term present
 };
 
-// Fast-path FP64 site, constants pre-scaled for pair64 (p0 = 6 es6 | 6 De,
-// p1 = 6 es12 | a, p2 = - | re, p3 = - | 2 De a).
+// Fast-path FP64 site, constants pre-scaled for pair64 (p0 = 6 es6 | 12 De,
+// p1 = 12 es12 | a, p2 = - | re, p3 = - | 2 De a).
 struct SiteF64 {
     double x, y, z;
     double p0, p1, p2;

@@ -339,11 +339,12 @@ __device__ void site_sums32(cptr<DevMol> M, const Sites32<SC> s0, double px, dou
     out[3] = E ? acc.get(3) * (1.0 / 12.0) : 0.0;
 }
 
-// The FP64 site term: SiteF64 constants p0 = 6 es6 | 6 De, p1 = 6 es12 | a,
-// p2 = - | re, p3 = - | 2 De a; returns the force and 6x the energy.
+// The FP64 site term, the same one-FMA forms as pair32: SiteF64 constants
+// p0 = 6 es6 | 12 De, p1 = 12 es12 | a, p2 = - | re, p3 = - | 2 De a;
+// returns the force and 12x the energy.
 template <int SEG>
 __device__ inline void pair64(const SiteF64& st, double tx, double ty, double tz, double& fx, double& fy,
-                              double& fz, double& e6) {
+                              double& fz, double& e12) {
     const double dx = tx - st.x, dy = ty - st.y, dz = tz - st.z;
     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
     double fs;
@@ -356,16 +357,14 @@ __device__ inline void pair64(const SiteF64& st, double tx, double ty, double tz
             ir2 = rcp64(r2);
         }
         const double ir6 = ir2 * ir2 * ir2;
-        const double tb = st.p0 * ir6;
-        const double ta = st.p1 * ir6 * ir6;
-        const double el = ta - tb;
+        const double fl = fma(st.p1, ir6, -st.p0);   // 12 es12 / r^6 - 6 es6
+        e12 = ir6 * (fl - st.p0);
         if (SEG == 0) {
             const double ec = st.qq * ir;
-            fs = (ta + el + ec) * ir2;
-            e6 = fma(6.0, ec, el);
+            fs = fma(fl, ir6, ec) * ir2;
+            e12 = fma(12.0, ec, e12);
         } else {
-            fs = (ta + el) * ir2;
-            e6 = el;
+            fs = fl * ir6 * ir2;
         }
     } else {
         const double ir = rsq64(r2);
@@ -373,11 +372,11 @@ __device__ inline void pair64(const SiteF64& st, double tx, double ty, double tz
         const double x = exp(st.p1 * (st.p2 - r));
         const double xm = x - 1.0;
         fs = st.p3 * x * xm * ir;
-        e6 = st.p0 * xm * xm;
+        e12 = st.p0 * xm * xm;
         if (SEG == 2) {
             const double ec = st.qq * ir;
             fs = fma(ec, ir * ir, fs);
-            e6 = fma(6.0, ec, e6);
+            e12 = fma(12.0, ec, e12);
         }
     }
     fx = fs * dx;
@@ -436,7 +435,7 @@ __device__ void site_sums64(cptr<DevMol> M, const Sites<SC, SiteF64> s0, double 
     out[0] = acc[0];
     out[1] = acc[1];
     out[2] = acc[2];
-    out[3] = acc[3] * (1.0 / 6.0);
+    out[3] = acc[3] * (1.0 / 12.0);
 }
 
 // Trilinear lookup of ForceGrid::interpolate (force_grid.cpp:88-155) with the
