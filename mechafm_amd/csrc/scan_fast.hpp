@@ -792,8 +792,7 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
             // per lane by at least wide_tail (a.wide_tail = 0: never).
             const uint64_t lead_act = __ballot(leader && live);
             const int n_act = __popcll(lead_act);
-            int p2 = 1;
-            while (p2 < n_act) p2 <<= 1;
+            const int p2 = n_act <= 1 ? 1 : 1 << (32 - __builtin_clz((unsigned)(n_act - 1)));   // pow2ceil
             const int Ge = 64 / p2;
             const bool wide = G < 64 && a.wide_tail > 0 && Ge >= a.wide_tail * G;
             if (wide) {
