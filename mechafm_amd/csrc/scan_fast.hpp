@@ -1,16 +1,17 @@
 // Fast scan: persistent, work-stealing, one force evaluation per loop trip.
 //
 // Work decomposition
-//   * A workgroup of W waves (W = 1, 2, 4) owns one molecule at a time and
+//   * A workgroup of W waves (W = 1 .. 8) owns one molecule at a time and
 //     stages its sites once in LDS; the molecule index is wave-uniform, so its
 //     DevMol record is read with scalar loads into SGPRs.  A molecule too
 //     large for LDS is streamed through scalar loads instead (G = 1).
 //   * G lanes form a pixel group; the G lanes split the molecule's sites and
-//     combine their partial sums with a butterfly of lane shuffles.  Groups
-//     of a wave read the same sites (LDS broadcast).
+//     combine their partial sums with DPP lane moves.  Groups of a wave read
+//     the same sites (LDS broadcast).
 //   * Each group steals pixel columns from its molecule's atomic counter
-//     (one atomic per wave per refill, ballot + prefix count).  When a
-//     molecule runs dry the workgroup moves to the molecule with work left.
+//     (one atomic per wave per refill, ballot + prefix count).  Molecules
+//     are visited longest columns first; when a molecule runs dry the
+//     workgroup moves to the next one with work left.
 //     A wave whose groups have mostly finished regroups the remaining
 //     positions over all 64 lanes.
 //   * The minimiser / integrator of simulation.cpp:102-122 runs as a per-group
