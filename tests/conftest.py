@@ -18,7 +18,8 @@ def pytest_configure(config):
 
 
 def golden_cases():
-    return sorted(d for d in os.listdir(GOLDEN) if os.path.isdir(os.path.join(GOLDEN, d)))
+    """Cases with full-precision reference records (tests/golden/<case>/records.npz)."""
+    return sorted(d for d in os.listdir(GOLDEN) if os.path.exists(os.path.join(GOLDEN, d, "records.npz")))
 
 
 def materialize(case: str, dst: str) -> str:

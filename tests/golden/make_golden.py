@@ -129,11 +129,46 @@ def make_case(name: str) -> None:
         print(f"{name}: {rec.shape[0]} records, steps mean {rec[:, 3].mean():.1f} max {rec[:, 3].max():.0f}")
 
 
+REF_EXAMPLES = "/root/reference/examples"
+REF_BINARY = os.path.join(ROOT, "oracle", "_ref", "mechafm-omp")
+
+
+def make_example() -> None:
+    """example_graphene: the reference's own example job (examples/input.scan,
+    graphene.xyz, parameters.dat) and the per-height files the reference
+    program writes for it, each file's lines sorted (the reference writes
+    them in thread order), concatenated under '## <name>' headers."""
+    dst = os.path.join(OUT, "example_graphene")
+    if os.path.isdir(dst):
+        shutil.rmtree(dst)
+    os.makedirs(dst)
+    for fn in ("input.scan", "graphene.xyz", "parameters.dat"):
+        shutil.copy(os.path.join(REF_EXAMPLES, fn), os.path.join(dst, fn))
+    with tempfile.TemporaryDirectory() as work:
+        for fn in ("input.scan", "graphene.xyz", "parameters.dat"):
+            shutil.copy(os.path.join(dst, fn), os.path.join(work, fn))
+        out = os.path.join(work, "out")
+        os.makedirs(out)
+        subprocess.check_call([REF_BINARY, "input.scan", out], cwd=work, stdout=subprocess.DEVNULL,
+                              env=dict(os.environ, OMP_NUM_THREADS=str(os.cpu_count() or 8)))
+        parts = []
+        for fn in sorted(os.listdir(out)):
+            if fn.startswith("scan-"):
+                lines = sorted(open(os.path.join(out, fn)).read().splitlines())
+                parts.append("## " + fn + "\n" + "\n".join(lines) + "\n")
+        with gzip.open(os.path.join(dst, "outputs.txt.gz"), "wt", 9) as f:
+            f.write("".join(parts))
+    print(f"example_graphene: {len(parts)} files")
+
+
 def main(names=None) -> None:
     if not os.path.exists(HARNESS):
         raise SystemExit("build the reference harness first: make -C oracle ref")
-    for name in names or CASES:
-        make_case(name)
+    for name in names or list(CASES) + ["example_graphene"]:
+        if name == "example_graphene":
+            make_example()
+        else:
+            make_case(name)
 
 
 if __name__ == "__main__":

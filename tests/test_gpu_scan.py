@@ -17,7 +17,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import golden_cases, golden_eval, golden_records, materialize
+from conftest import GOLDEN, golden_cases, golden_eval, golden_records, materialize
 
 pytestmark = pytest.mark.gpu
 
@@ -68,6 +68,45 @@ def test_plain64_matches_reference(gpu_ctx, case, tmp_path):
         # every column the kernel produces (the angle is derived here with numpy)
         cols = [c for c in range(16) if c != 14]
         np.testing.assert_array_equal(tab[:, cols], ref[:, cols])
+
+
+@pytest.mark.parametrize("precision", ["plain64", "fast64"])
+def test_reference_example_files(gpu_ctx, tmp_path, precision):
+    """The reference's own example job (examples/input.scan: 72-atom graphene,
+    41 x 31 px x 41 heights, FIRE + midpoint to |F| < 1e-3) through the
+    job-level API: every per-height file equals the one the reference program
+    writes, line for line once sorted (the reference writes lines in thread
+    order), and the step total is exact (the sum of the reference files' step
+    column; the reference's own statistics.txt undercounts it through an
+    unsynchronised += under OpenMP, simulation.cpp:120)."""
+    import gzip
+    import shutil
+    from mechafm_amd import abi
+    src = os.path.join(GOLDEN, "example_graphene")
+    job, out = tmp_path / "job", tmp_path / "out"
+    job.mkdir()
+    for fn in ("input.scan", "graphene.xyz", "parameters.dat"):
+        shutil.copy(os.path.join(src, fn), job / fn)
+    expected, name = {}, None
+    with gzip.open(os.path.join(src, "outputs.txt.gz"), "rt") as f:
+        for line in f.read().splitlines():
+            if line.startswith("## "):
+                name = line[3:]
+                expected[name] = []
+            else:
+                expected[name].append(line)
+    sim = abi.Simulation(str(job / "input.scan"), gpu_ctx, outdir=str(out))
+    st = sim.run(precision)
+    sim.write_output(8)
+    assert len(expected) == 41
+    for fn, lines in expected.items():
+        got = sorted(open(out / fn).read().splitlines())
+        assert len(got) == len(lines), fn
+        bad = sum(a != b for a, b in zip(got, lines))
+        assert bad == 0, f"{fn}: {bad} of {len(lines)} lines differ"
+    assert st["total_steps"] == sum(int(l.split()[-1]) for ls in expected.values() for l in ls)
+    stats = open(out / "statistics.txt").read()
+    assert f"Needed {st['total_steps']} minimization steps" in stats
 
 
 @pytest.mark.parametrize("case", CASES)
