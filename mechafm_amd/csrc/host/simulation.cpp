@@ -19,6 +19,7 @@
 #include <thread>
 
 #include "../ctx_ref.hpp"
+#include "cube_values.hpp"
 #include "format.hpp"
 
 namespace mafm {
@@ -324,9 +325,9 @@ Cube readCube(const std::string& path, bool with_values) {   // cube_io.cpp:10-1
     for (int a = 0; a < std::abs(c.n_atoms); ++a)
         for (int q = 0; q < 5; ++q) num("atom record");
     if (with_values) {
-        size_t total = (size_t)c.n[0] * c.n[1] * c.n[2];
-        c.values.resize(total);
-        for (size_t i = 0; i < total; ++i) c.values[i] = num("volumetric data");
+        const size_t total = (size_t)c.n[0] * c.n[1] * c.n[2];
+        if (!parse_cube_values(p, end, total, c.values))
+            fail(MAFM_ERR_INVALID, "Could not read the volumetric data from the cube file. Check the format of the file.");
     }
     return c;
 }
