@@ -69,6 +69,11 @@ mafm_status mafm_sim_write_output(mafm_sim* sim, int threads);
  * deltaf-%06.3f.dat files in the output folder; desc NULL = the script's
  * amplitude, cantilever and frequency */
 mafm_status mafm_sim_write_deltaf(mafm_sim* sim, const mafm_deltaf_desc* desc);
+/* the warnings raised while reading the inputs and the input-settings echo,
+ * in the reference's console format (messages.cpp:29-56, parse.cpp:407-473,
+ * 575-581); copies at most size - 1 characters and a terminating NUL into
+ * buf, sets *needed (if not NULL) to the full length + 1 */
+mafm_status mafm_sim_settings(mafm_sim* sim, char* buf, size_t size, size_t* needed);
 const char* mafm_sim_last_error(const mafm_sim* sim);
 /* frees the simulation and the grids it built; completes a pending
  * mafm_destroy of its context when it is the last simulation open there */

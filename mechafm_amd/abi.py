@@ -153,6 +153,7 @@ SIGNATURES = {
     "mafm_sim_scan_desc": (ctypes.c_int, [_vp, _P(RunOpts), _P(ScanDesc)]),
     "mafm_sim_run": (ctypes.c_int, [_vp, _P(RunOpts), _P(ScanStats)]),
     "mafm_batch_run": (ctypes.c_int, [ctypes.c_int, _P(_vp), _P(RunOpts), _P(ScanStats)]),
+    "mafm_sim_settings": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t, _P(ctypes.c_size_t)]),
     "mafm_sim_records": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _vp]),
     "mafm_sim_write_output": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mafm_sim_write_deltaf": (ctypes.c_int, [_vp, _P(DeltaFDesc)]),
@@ -365,6 +366,14 @@ class Simulation:
         st = ScanStats()
         self.check(lib().mafm_sim_run(self.h, ctypes.byref(o), ctypes.byref(st)))
         return st.as_dict()
+
+    def settings(self) -> str:
+        """Input warnings and the settings echo, as the reference prints them."""
+        need = ctypes.c_size_t()
+        self.check(lib().mafm_sim_settings(self.h, None, 0, ctypes.byref(need)))
+        buf = ctypes.create_string_buffer(need.value)
+        self.check(lib().mafm_sim_settings(self.h, buf, need.value, None))
+        return buf.value.decode()
 
     def records(self) -> np.ndarray:
         i = self.info()

@@ -49,7 +49,14 @@ int main(int argc, char** argv) {
     mafm_sim* sim = nullptr;
     auto t0 = std::chrono::steady_clock::now();
     st = mafm_sim_open(input.c_str(), outdir.c_str(), ctx, &sim);
-    if (st == MAFM_OK) st = mafm_sim_initialize(sim);
+    if (st == MAFM_OK) {                          // input warnings and settings, as the reference prints them
+        size_t need = 0;
+        mafm_sim_settings(sim, nullptr, 0, &need);
+        std::string text(need, '\0');
+        mafm_sim_settings(sim, &text[0], need, nullptr);
+        fputs(text.c_str(), stdout);
+        st = mafm_sim_initialize(sim);
+    }
     if (st != MAFM_OK) {
         int rc = die("set-up", mafm_sim_last_error(sim));
         mafm_sim_close(sim);
@@ -60,6 +67,7 @@ int main(int argc, char** argv) {
     mafm_sim_info_get(sim, &info);
     printf("+- 3D data grid is: %d x %d x %d (%lld in total)\n", info.nx, info.ny, info.nz,
            (long long)info.nx * info.ny * info.nz);
+    printf("+- Starting simulation\n");
     mafm_scan_stats ss{};
     st = mafm_sim_run(sim, &ro, &ss);
     if (st == MAFM_OK) st = mafm_sim_write_output(sim, 0);

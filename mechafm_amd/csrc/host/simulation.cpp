@@ -393,17 +393,20 @@ void Simulation::readInputFile() {                // parse.cpp:83-405
         else if (key == "vdw_pbc") o.vdw_pbc = on_off(key, val);
         else if (key == "surface_normal") {
             std::string v = lower(val);
+            o.normal_text = val;
             if (v == "z") o.normal = NORMAL_Z;
             else if (v == "y") o.normal = NORMAL_Y;
             else if (v == "x") o.normal = NORMAL_X;
             else fail(MAFM_ERR_INVALID, "Option %s must be either X, Y or Z!", key.c_str());
         } else if (key == "minterm") {
+            o.minterm_text = val;
             if (val == "e") o.minterm = MIN_E;
             else if (val == "f") o.minterm = MIN_F;
             else if (val == "ef") o.minterm = MIN_EF;
             else fail(MAFM_ERR_INVALID, "Option %s must be either e, f or ef!", key.c_str());
         } else if (key == "units") {
             InteractionParameters& ip = interaction_parameters_;
+            o.units_text = val;
             if (val == "kcal/mol" || val == "kcal") { o.units = U_KCAL; ip.qbase = 332.06371; }
             else if (val == "kJ/mol" || val == "kJ") { o.units = U_KJ; ip.qbase = 1389.354563; }
             else if (val == "eV") { o.units = U_EV; ip.qbase = 14.39964901; }
@@ -493,6 +496,76 @@ void Simulation::readXYZFile() {                  // parse.cpp:477-585
     double charge_check = 0;
     for (int i = 0; i < system.n_atoms_; ++i) charge_check += std::pow(system.charges_[i], 2);
     o.xyz_charges = !(std::fabs(charge_check) < 1e-10);
+    if (!o.xyz_charges)                            // parse.cpp:575-581
+        warnings_.push_back("The RMSQ-error for the charges read from the XYZ-file is zero.\n"
+                            "+-     Charges will be read from the parameter file.\n"
+                            "+-     If you want zero charge, set the charge in the parameter file to zero.\n"
+                            "+-");
+}
+
+std::string Simulation::settingsText() const {
+    const InputOptions& o = options_;
+    std::string s;
+    for (const std::string& w : warnings_) s += "+- WARNING: " + w + "\n";
+    auto put = [&s](const char* fmt, ...) {
+        char line[1024];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(line, sizeof line, fmt, ap);
+        va_end(ap);
+        s += "+- ";
+        s += line;
+        s += "\n";
+    };
+    auto onoff = [](bool b) { return b ? "on" : "off"; };
+    put("");
+    put("Input settings for        %s:", o.inputfile.c_str());
+    put("");
+    put("xyzfile:                  %-s", o.xyzfile.c_str());
+    put("paramfile:                %-s", o.paramfile.c_str());
+    put("tipatom:                  %-s", o.tipatom.c_str());
+    put("dummyatom:                %-s", o.dummyatom.c_str());
+    put("");
+    put("units:                    %-s", o.units_text.c_str());
+    put("");
+    put("minterm:                  %-s", o.minterm_text.c_str());
+    put("etol:                     %-8.4f", o.etol);
+    put("ftol:                     %-8.4f", o.ftol);
+    put("dt:                       %-8.4f", o.dt);
+    put("maxsteps:                 %-8d", o.maxsteps);
+    put("");
+    put("area:                     %-8.4f %-8.4f", o.area.x, o.area.y);
+    put("center:                   %-8.4f %-8.4f", o.center.x, o.center.y);
+    put("zhigh:                    %-8.4f", o.zhigh);
+    put("zlow:                     %-8.4f", o.zlow);
+    put("dx:                       %-8.4f", o.dx);
+    put("dy:                       %-8.4f", o.dy);
+    put("dz:                       %-8.4f", o.dz);
+    put("");
+    put("surface_normal:           %-s", o.normal_text.c_str());
+    put("vdw_pbc:                  %-s", onoff(o.vdw_pbc));
+    if (o.vdw_pbc && !o.use_external_potential) {
+        put("cell_a:                   %-8.4f %-8.4f %-8.4f", o.cell_a.x, o.cell_a.y, o.cell_a.z);
+        put("cell_b:                   %-8.4f %-8.4f %-8.4f", o.cell_b.x, o.cell_b.y, o.cell_b.z);
+        put("cell_c:                   %-8.4f %-8.4f %-8.4f", o.cell_c.x, o.cell_c.y, o.cell_c.z);
+    }
+    put("");
+    put("coulomb:                  %-s", onoff(o.coulomb));
+    put("tip_dummy_coulomb:        %-s", onoff(o.tip_dummy_coulomb));
+    put("use_external_potential:   %-s", onoff(o.use_external_potential));
+    if (o.use_external_potential) put("e_potential_file:         %-s", o.e_potential_file.c_str());
+    put("");
+    put("flexible:                 %-s", onoff(o.flexible));
+    put("rigidgrid:                %-s", onoff(o.rigidgrid));
+    put("");
+    put("minimiser:         %-s", o.minimiser_type == FIRE ? "FIRE" : "SD");
+    put("integrator:        %-s", o.integrator_type == EULER ? "euler" : (o.integrator_type == RK4 ? "rk4" : "midpoint"));
+    put("");
+    put("bufsize:           %-8d", o.bufsize);
+    put("gzip:              %-s", onoff(o.gzip));
+    put("statistics:        %-s", onoff(o.statistics));
+    put("");
+    return s;
 }
 
 void Simulation::readParameterFile() {            // parse.cpp:588-697
@@ -1115,6 +1188,18 @@ mafm_status mafm_sim_write_deltaf(mafm_sim* s, const mafm_deltaf_desc* d) {
     if (!s) return MAFM_ERR_INVALID;
     const mafm_deltaf_desc def{};
     return guarded(s, [&]() { s->sim.writeDeltaF(d ? *d : def); });
+}
+
+mafm_status mafm_sim_settings(mafm_sim* s, char* buf, size_t size, size_t* needed) {
+    if (!s || (size && !buf)) return MAFM_ERR_INVALID;
+    const std::string t = s->sim.settingsText();
+    if (needed) *needed = t.size() + 1;
+    if (size) {
+        const size_t n = std::min(size - 1, t.size());
+        memcpy(buf, t.data(), n);
+        buf[n] = 0;
+    }
+    return MAFM_OK;
 }
 
 mafm_status mafm_sim_records(mafm_sim* s, size_t first, size_t count, mafm_record* out) {

@@ -82,6 +82,8 @@ struct InputOptions {
     bool xyz_charges = false;
     MinimiserType minimiser_type = FIRE;
     IntegratorType integrator_type = MIDPOINT;
+    // option values as written in the input file, for the settings echo
+    std::string units_text = "kcal/mol", minterm_text, normal_text = "z";
 };
 
 struct AtomParameters {
@@ -273,6 +275,10 @@ class Simulation {
     // Delta f images (simple-vafm-image.py:76-204) from this job's records,
     // one deltaf-%06.3f.dat per image named by its lowest height
     void writeDeltaF(const mafm_deltaf_desc& d);
+    // warnings raised while reading the inputs, then the input settings, in
+    // the reference's console format (messages.cpp:29-56, parse.cpp:407-473)
+    std::string settingsText() const;
+    std::vector<std::string> warnings_;
 
     InputOptions options_;
     InteractionParameters interaction_parameters_;

@@ -149,8 +149,12 @@ def make_example() -> None:
             shutil.copy(os.path.join(dst, fn), os.path.join(work, fn))
         out = os.path.join(work, "out")
         os.makedirs(out)
-        subprocess.check_call([REF_BINARY, "input.scan", out], cwd=work, stdout=subprocess.DEVNULL,
-                              env=dict(os.environ, OMP_NUM_THREADS=str(os.cpu_count() or 8)))
+        run = subprocess.run([REF_BINARY, "input.scan", out], cwd=work, capture_output=True, text=True, check=True,
+                             env=dict(os.environ, OMP_NUM_THREADS=str(os.cpu_count() or 8)))
+        # the console text up to the scan (warnings on stderr, settings echo on stdout)
+        echo = run.stdout.split("+- 3D data grid")[0]
+        with open(os.path.join(dst, "console.txt"), "w") as f:
+            f.write(run.stderr + echo)
         parts = []
         for fn in sorted(os.listdir(out)):
             if fn.startswith("scan-"):

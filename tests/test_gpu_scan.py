@@ -71,7 +71,7 @@ def test_plain64_matches_reference(gpu_ctx, case, tmp_path):
 
 
 @pytest.mark.parametrize("precision", ["plain64", "fast64"])
-def test_reference_example_files(gpu_ctx, tmp_path, precision):
+def test_reference_example_files(gpu_ctx, tmp_path, precision, monkeypatch):
     """The reference's own example job (examples/input.scan: 72-atom graphene,
     41 x 31 px x 41 heights, FIRE + midpoint to |F| < 1e-3) through the
     job-level API: every per-height file equals the one the reference program
@@ -95,7 +95,8 @@ def test_reference_example_files(gpu_ctx, tmp_path, precision):
                 expected[name] = []
             else:
                 expected[name].append(line)
-    sim = abi.Simulation(str(job / "input.scan"), gpu_ctx, outdir=str(out))
+    monkeypatch.chdir(job)                       # the input named as the reference's run named it
+    sim = abi.Simulation("input.scan", gpu_ctx, outdir=str(out))
     st = sim.run(precision)
     sim.write_output(8)
     assert len(expected) == 41
@@ -107,6 +108,11 @@ def test_reference_example_files(gpu_ctx, tmp_path, precision):
     assert st["total_steps"] == sum(int(l.split()[-1]) for ls in expected.values() for l in ls)
     stats = open(out / "statistics.txt").read()
     assert f"Needed {st['total_steps']} minimization steps" in stats
+    # console: the charge warning and the settings echo as the reference
+    # prints them (its credits banner between the two is not reproduced)
+    ref = open(os.path.join(src, "console.txt")).read()
+    warn, echo = ref.split("+ - - -")[0], "+- \n+- Input settings" + ref.split("+- \n+- Input settings")[1]
+    assert sim.settings() == warn + echo
 
 
 @pytest.mark.parametrize("case", CASES)
