@@ -6,17 +6,25 @@ A step is one full scan of the workload (every pixel column, every height,
 FIRE + midpoint to convergence) in one launch of the work-stealing kernel,
 with the molecules' tables already resident in HBM.  Default workload: the
 batched one (64 seeded planar molecules, N ~ U[20, 400], LJ + Coulomb,
-64 x 64 px x 40 z each), FP32 pair terms summed in register blocks folded
-into FP64, FP64 minimiser state.  The FP64 workloads (c1, c2) default to the
-FP64 fast path.
+64 x 64 px x 40 z each).
+
+Headline arithmetic: fast64 (FP64 pair terms with refined hardware
+reciprocals, FP64 minimiser), the arithmetic that keeps every position
+within 1e-4 A / 1e-3 relative Fz of the plain FP64 scan.  The FP32 scan
+(fast32: FP32 pair terms summed in register blocks folded into FP64) is
+timed beside it and reported with the fraction of positions it keeps within
+that bar: an FP32 convergence test can come out one FIRE step away from the
+FP64 one, which moves that position and the rest of its column
+(``--margins`` prints the decision-margin analysis behind this).
 
 Printed JSON line: metric/value (relaxed tip positions per second, whole
-job), the roofline of the scan kernel (algorithmic VALU FLOPs per launch
-over its HIP-event duration, against the 157.3 TFLOP/s FP32 or 78.6 TFLOP/s
-FP64 vector peak; lookup bytes against 8 TB/s for a force-grid scan),
-the plain FP64 GPU scan on the same workload (speed and the fraction of
-positions within 1e-4 A / 1e-3 relative Fz of it), and a CPU baseline: the
-reference program timed on this host on a sample of the same workload.
+job), the roofline of the headline scan kernel (algorithmic VALU FLOPs per
+launch over its HIP-event duration, against the 78.6 TFLOP/s FP64 or
+157.3 TFLOP/s FP32 vector peak; lookup bytes against 8 TB/s for a
+force-grid scan), the FP32 scan, the plain FP64 GPU scan on the same
+workload (speed and the fraction of positions within 1e-4 A / 1e-3 relative
+Fz of it), and a CPU baseline: the reference program timed on this host on
+a sample of the same workload.
 """
 import argparse
 import json
@@ -32,7 +40,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FP32_VALU_PEAK = 157.3e12      # MI355X FP32 vector, FLOP/s
+FP32_VALU_PEAK = 157.3e12      # MI355X FP32 vector (packed math), FLOP/s
+FP32_UNPACKED_PEAK = 78.6e12   # one FP32 FMA per lane per 4-cycle wave issue, FLOP/s
 FP64_VALU_PEAK = 78.6e12       # MI355X FP64 vector, FLOP/s
 HBM_PEAK = 8.0e12              # MI355X HBM3E, B/s
 LOOKUP_BYTES = 8 * 16          # one trilinear lookup: 8 corners of {fx, fy, fz, e} in FP32 (x2 in FP64)
@@ -221,14 +230,111 @@ def adjoint_bench(tmp, args):
     print(json.dumps(out), flush=True)
 
 
+def bar_compare(recs, prec_recs):
+    """Per-position |dr| (max over x, y, z) and relative Fz error of recs
+    against prec_recs, and the 1e-4 A / 1e-3 bar."""
+    dr = np.concatenate([np.abs(np.stack([a["rx"] - b["rx"], a["ry"] - b["ry"], a["rz"] - b["rz"]])).max(0)
+                         for a, b in zip(recs, prec_recs)])
+    dfz = np.concatenate([np.abs(a["fz"] - b["fz"]) / np.maximum(np.abs(b["fz"]), 1e-2)
+                          for a, b in zip(recs, prec_recs)])
+    steps = sum(int((a["n_steps"] != b["n_steps"]).sum()) for a, b in zip(recs, prec_recs))
+    ok = (dr <= 1e-4) & (dfz <= 1e-3)
+    return {"within_1e-4A_1e-3Fz": float(ok.mean()), "max_dr": float(dr.max()), "median_dr": float(np.median(dr)),
+            "step_count_mismatches": steps}
+
+
+def timed_scan(abi, sims, precision, args, dev, options, positions, tag):
+    """Tune (unless fixed), warm up, then time args.steps full scans; returns
+    the wall rate, the kernel times and the last records."""
+    G, wps = args.lanes, args.waves
+    if not args.no_tune and (G == 0 or wps == 0):
+        from mechafm_amd import tune
+        best = tune.tune(sims, precision, compensation=args.compensation, options=options,
+                         lanes=[G] if G else tune.DEFAULT_LANES, use_cache=False, device=dev,
+                         log=lambda m: print(f"# tune {tag} " + m, file=sys.stderr, flush=True))
+        G, wps = best["lanes_per_pixel"], best["waves_per_simd"]
+    kw = dict(lanes_per_pixel=G, waves_per_simd=wps, compensation=args.compensation,
+              waves_per_group=args.group, options=options)
+    for _ in range(args.warmup):
+        run_batch(abi, sims, precision, **kw)
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = run_batch(abi, sims, precision, **kw)
+        kms.append(st["kernel_ms"])
+    wall = time.perf_counter() - t0
+    recs = [s.records().copy() for s in sims]
+    return {"value": positions * args.steps / wall, "wall": wall, "ms_per_step": wall / args.steps * 1e3,
+            "kernel_s": float(np.mean(kms)) / 1e3, "stats": st, "recs": recs}
+
+
+def roofline_of(sims, run, precision, info):
+    """Roofline object of a timed scan: algorithmic VALU FLOPs (site terms,
+    tip-dummy terms and FIRE updates per the iteration counts of the records)
+    over the kernel's HIP-event duration against the FP32 (fast32) or FP64
+    vector peak; for a force-grid scan, lookup bytes against HBM."""
+    flops = algorithmic_flops(sims, run["recs"])
+    k = run["kernel_s"]
+    peak = FP32_VALU_PEAK if precision == "fast32" else FP64_VALU_PEAK
+    r = {"bound": "valu", "achieved": flops / k / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+         "frac": flops / k / peak, "traffic": None, "kernel_ms": k * 1e3, "flops_per_launch": flops}
+    if precision == "fast32":
+        # one VALU instruction per 4 cycles and lane without packed FP32 math
+        r["frac_of_unpacked_fp32"] = flops / k / FP32_UNPACKED_PEAK
+    if info["rigidgrid"]:
+        gb = grid_lookup_bytes(sims, run["recs"], fp64=precision != "fast32")
+        r = {"bound": "hbm", "achieved": gb / k / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+             "frac": gb / k / HBM_PEAK, "traffic": None, "kernel_ms": k * 1e3, "bytes_per_launch": gb}
+    return r
+
+
+def margins_report(abi, sims, ctx, positions, nz):
+    """Why FP32 pair sums cannot keep the bar on every position: the FP32
+    and FP64 scans with their decision margins (MAFM_SCAN_MARGINS) against
+    the plain FP64 scan.  For every column whose FIRE step counts leave the
+    FP64 ones, the margins at the first height where they differ; for every
+    column, its smallest margins; and the share of columns an FP64 replay
+    would have to rescan to catch them all at a given band."""
+    def go(prec, options):
+        st = run_batch(abi, sims, prec, options=options)
+        recs = np.concatenate([s.records().copy() for s in sims])
+        m = ctx.margins(0, positions)[:, :3] if options & abi.SCAN_MARGINS else None
+        return st, recs, m
+    st64, r64, m64 = go("fast64", abi.SCAN_MARGINS)
+    st32, r32, m32 = go("fast32", abi.SCAN_MARGINS)
+    stp, rp, _ = go("plain64", 0)
+    out = {"positions": positions, "fast64_ms": st64["kernel_ms"], "fast32_ms": st32["kernel_ms"],
+           "plain64_ms": stp["kernel_ms"], "fast64_vs_plain64": bar_compare([r64], [rp]),
+           "fast32_vs_plain64": bar_compare([r32], [rp])}
+    ncol = positions // nz
+    s32, s64 = r32["n_steps"].reshape(ncol, nz), rp["n_steps"].reshape(ncol, nz)
+    diff = s32 != s64
+    div = np.where(diff.any(1))[0]
+    kstar = diff[div].argmax(1)
+    M32, M64 = m32.reshape(ncol, nz, 3), m64.reshape(ncol, nz, 3)
+    at64 = M64[div, kstar]
+    kind = np.argmin(at64, 1)
+    out["divergent_columns"] = int(len(div))
+    out["columns"] = int(ncol)
+    out["first_divergence_decision"] = {n: int((kind == i).sum()) for i, n in enumerate(("force", "energy", "power"))}
+    out["fp64_margin_at_first_divergence_quantiles_50_90_99_100"] = [
+        float(v) for v in np.quantile(at64.min(1), [0.5, 0.9, 0.99, 1.0])] if len(div) else []
+    colmin = M32.min(1).min(1)
+    out["replay_band_sweep"] = [{"band": b, "columns_flagged": float((colmin < b).mean()),
+                                 "divergent_columns_missed": int((colmin[div] >= b).sum())}
+                                for b in (1e-7, 1e-6, 1e-5, 3e-5, 1e-4)]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
-    ap.add_argument("--precision", default=None, choices=["fast32", "fast64", "plain64"],
-                    help="default: fast64 for the FP64 workloads (c1, c2), fast32 for the others")
+    ap.add_argument("--precision", default="fast64", choices=["fast32", "fast64", "plain64"],
+                    help="arithmetic of the headline scan (fast64: the one that keeps every position within "
+                         "1e-4 A of the plain FP64 scan)")
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--group", type=int, default=0, help="waves per workgroup (0 = automatic)")
@@ -237,15 +343,16 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-fast32", action="store_true", help="skip the FP32 scan reported beside a fast64 headline")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (calibrates HBM byte counters)")
     ap.add_argument("--compensation", type=int, default=0,
                     help="FP32 sums: 0 blocks of 4-6 sites folded into FP64, 1 Kahan, 2 plain")
     ap.add_argument("--adjoint", action="store_true",
                     help="the adjoint workload (150 atoms, 128x128 px x 30 z): gradient timing and finite differences")
+    ap.add_argument("--margins", action="store_true",
+                    help="decision-margin analysis of the FP32 scan against FP64 (no timing line)")
     args = ap.parse_args()
-    if args.precision is None:
-        args.precision = "fast64" if args.config in ("c1", "c2") else "fast32"
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (pixel columns would shard "
               "across GPUs with no exchange during the scan). Run with --gpus 1.", file=sys.stderr)
@@ -282,73 +389,57 @@ def main():
         print(f"# workload {args.config}: {len(sims)} systems, {positions} positions, set-up "
               f"{time.perf_counter() - t0:.1f} s on {dev['name']} ({dev['compute_units']} CUs)",
               file=sys.stderr, flush=True)
+        if args.margins:
+            print(json.dumps({"margins_" + args.config: margins_report(abi, sims, ctx, positions, shape[2])}),
+                  flush=True)
+            return
 
-        G, wps = args.lanes, args.waves
         options = abi.SCAN_REPRODUCIBLE if args.reproducible else 0
-        if args.precision != "plain64" and not args.no_tune and (G == 0 or wps == 0):
-            from mechafm_amd import tune
-            best = tune.tune(sims, args.precision, compensation=args.compensation, options=options,
-                             lanes=[G] if G else tune.DEFAULT_LANES, use_cache=False, device=dev,
-                             log=lambda m: print("# tune " + m, file=sys.stderr, flush=True))
-            G, wps = best["lanes_per_pixel"], best["waves_per_simd"]
-        kw = dict(lanes_per_pixel=G, waves_per_simd=wps, compensation=args.compensation,
-                  waves_per_group=args.group, options=options)
-        for _ in range(args.warmup):
-            run_batch(abi, sims, args.precision, **kw)
-        kms = []
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            st = run_batch(abi, sims, args.precision, **kw)
-            kms.append(st["kernel_ms"])
-        wall = time.perf_counter() - t0
-        value = positions * args.steps / wall
-        recs = [s.records().copy() for s in sims]
-        flops = algorithmic_flops(sims, recs)
-        k_avg = float(np.mean(kms)) / 1e3
-        achieved = flops / k_avg
-        peak = FP32_VALU_PEAK if args.precision == "fast32" else FP64_VALU_PEAK
+        head = timed_scan(abi, sims, args.precision, args, dev, options, positions, args.precision)
+        st = head["stats"]
         out = {
             "metric": "relaxed tip positions/s at 1e-4 A error; % FP32 VALU peak vs plain FP64",
-            "value": value, "unit": "positions/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "f32" if args.precision == "fast32" else "f64", "data": "synthetic",
+            "value": head["value"], "unit": "positions/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": {"fast32": "f32"}.get(args.precision, "f64"), "data": "synthetic",
             "config": {"workload": {"c3": "64 seeded planar molecules, N~U[20,400], LJ+Coulomb, "
                                           "64x64 px x 40 z each, FIRE+midpoint, one batched launch"}.get(
                 args.config, args.config), "systems": len(sims), "positions": positions,
                 "precision": args.precision, "lanes_per_pixel": st["lanes_per_pixel"],
                 "waves_per_simd": st["waves_per_simd"], "workgroup_threads": st["block_threads"],
                 "total_steps": st["total_steps"],
-                "compensation": {1: "kahan", 2: "none"}.get(args.compensation, "block-fp64"),
                 "group_utilisation": st["active_group_trips"] / max(1, st["group_trips"]),
                 "wide_trip_fraction": st["wide_trips"] * 64 / st["lanes_per_pixel"] / max(1, st["group_trips"]),
                 "unconverged": st["unconverged"]},
-            "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                         "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                         "kernel_ms": k_avg * 1e3, "flops_per_launch": flops},
+            "roofline": roofline_of(sims, head, args.precision, info),
         }
-        if info["rigidgrid"]:
-            # force-grid scan: no site terms, the work is the trilinear gathers
-            gb = grid_lookup_bytes(sims, recs, fp64=args.precision != "fast32")
-            out["roofline"] = {"bound": "hbm", "achieved": gb / k_avg / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                               "frac": gb / k_avg / HBM_PEAK, "traffic": None, "kernel_ms": k_avg * 1e3,
-                               "bytes_per_launch": gb}
+        if args.precision == "fast32":
+            out["config"]["compensation"] = {1: "kahan", 2: "none"}.get(args.compensation, "block-fp64")
         traffic = os.path.join(ROOT, "profiles", f"traffic_scan_{args.config}_{args.precision}.json")
         if os.path.exists(traffic):
             out["roofline"]["traffic"] = json.load(open(traffic)).get("hbm_bytes_per_launch")
+        f32 = None
+        if args.precision == "fast64" and not args.no_fast32:
+            # the FP32 scan beside it: faster, but off the bar on a fraction of positions
+            f32 = timed_scan(abi, sims, "fast32", args, dev, options, positions, "fast32")
+            out["fast32"] = {"value": f32["value"], "ms_per_step": f32["ms_per_step"],
+                             "lanes_per_pixel": f32["stats"]["lanes_per_pixel"],
+                             "waves_per_simd": f32["stats"]["waves_per_simd"],
+                             "roofline": roofline_of(sims, f32, "fast32", info)}
+            t32 = os.path.join(ROOT, "profiles", f"traffic_scan_{args.config}_fast32.json")
+            if os.path.exists(t32):
+                out["fast32"]["roofline"]["traffic"] = json.load(open(t32)).get("hbm_bytes_per_launch")
         if not args.no_plain and args.precision != "plain64":
             t0 = time.perf_counter()
             pst = run_batch(abi, sims, "plain64")
             pwall = time.perf_counter() - t0
             prec = [s.records() for s in sims]
-            dr = np.concatenate([np.abs(np.stack([a["rx"] - b["rx"], a["ry"] - b["ry"], a["rz"] - b["rz"]])).max(0)
-                                 for a, b in zip(recs, prec)])
-            dfz = np.concatenate([np.abs(a["fz"] - b["fz"]) / np.maximum(np.abs(b["fz"]), 1e-2)
-                                  for a, b in zip(recs, prec)])
-            ok = (dr <= 1e-4) & (dfz <= 1e-3)
-            out["plain_fp64"] = {"value": positions / (pst["kernel_ms"] / 1e3), "kernel_ms": pst["kernel_ms"],
-                                 "wall_s": pwall, "speedup": (pst["kernel_ms"] / 1e3) / k_avg,
-                                 "within_1e-4A_1e-3Fz": float(ok.mean()), "max_dr": float(dr.max()),
-                                 "median_dr": float(np.median(dr))}
+            p_s = pst["kernel_ms"] / 1e3
+            out["plain_fp64"] = dict({"value": positions / p_s, "kernel_ms": pst["kernel_ms"], "wall_s": pwall,
+                                      "speedup": p_s / head["kernel_s"]}, **bar_compare(head["recs"], prec))
+            if f32 is not None:
+                out["fast32"].update({"speedup_vs_plain_fp64": p_s / f32["kernel_s"]},
+                                     **bar_compare(f32["recs"], prec))
         if not args.no_cpu:
             out["cpu_baseline"] = cpu_baseline(args.config, tmp)
         print(json.dumps(out), flush=True)

@@ -133,7 +133,14 @@ enum { MAFM_PREC_PLAIN64 = 0, MAFM_PREC_FAST64 = 1, MAFM_PREC_FAST32 = 2 };
  * one after another with all 64 lanes (shorter tail, same arithmetic, but
  * the summation order of a position then depends on timing: results agree
  * to FP32 rounding, not bit for bit). */
-enum { MAFM_SCAN_REPRODUCIBLE = 1 };
+/* MARGINS: keep every position's decision margins for mafm_margins: how
+ * near the height's minimiser decisions -- the convergence test
+ * (minimiser.cpp:15-37) and FIRE's velocity reset on P = v.f < 0
+ * (minimiser.cpp:107-126) -- came to coming out the other way.  A decision
+ * taken on FP32 pair sums flips against the FP64 trajectory when its
+ * margin is below the FP32 error of the tested quantity; the margins show
+ * where. */
+enum { MAFM_SCAN_REPRODUCIBLE = 1, MAFM_SCAN_MARGINS = 4 };
 
 typedef struct {
     int32_t nx, ny, nz;
@@ -249,6 +256,14 @@ mafm_status mafm_set_systems(mafm_ctx* ctx, int n_systems, const mafm_system_des
 mafm_status mafm_scan(mafm_ctx* ctx, const mafm_scan_desc* desc, mafm_scan_stats* stats);
 /* records of all systems: index ((system*nx + i)*ny + j)*nz + k */
 mafm_status mafm_records(mafm_ctx* ctx, size_t first, size_t count, mafm_record* out);
+/* Decision margins of the last scan (FAST kernels run with
+ * MAFM_SCAN_MARGINS), 4 floats per position in record order: the smallest
+ * force margin (|F^2 - ftol^2| / (2 ftol), counted where the energy part of
+ * the test passed), energy margin (||E - E_prev| - etol|, counted where the
+ * force part passed) and power margin (|P| / |v|) of the height's minimiser
+ * iterations (+inf where none was counted), then 0.  No reference
+ * counterpart: a diagnostic of the FP32 arithmetic. */
+mafm_status mafm_margins(mafm_ctx* ctx, size_t first, size_t count, float* out);
 mafm_status mafm_eval_tip_surface(mafm_ctx* ctx, int system, size_t n, const double* pos, double* fe);
 mafm_status mafm_adjoint(mafm_ctx* ctx, int system, const mafm_adjoint_desc* desc, mafm_adjoint_out* out);
 

@@ -19,6 +19,7 @@ MAFM_OK = 0
 STATUS = {0: "OK", 1: "INVALID", 2: "HIP", 3: "OUT_OF_GRID", 4: "STATE", 5: "FFT_IMAG", 6: "NO_DEVICE"}
 PREC = {"plain64": 0, "fast64": 1, "fast32": 2}
 SCAN_REPRODUCIBLE = 1
+SCAN_MARGINS = 4
 VDW_LJ, VDW_MORSE = 0, 1
 
 
@@ -140,6 +141,7 @@ SIGNATURES = {
     "mafm_set_systems": (ctypes.c_int, [_vp, ctypes.c_int, _P(SystemDesc)]),
     "mafm_scan": (ctypes.c_int, [_vp, _P(ScanDesc), _P(ScanStats)]),
     "mafm_records": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _vp]),
+    "mafm_margins": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _P(ctypes.c_float)]),
     "mafm_eval_tip_surface": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_size_t, _P(ctypes.c_double),
                                              _P(ctypes.c_double)]),
     "mafm_adjoint": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
@@ -276,6 +278,13 @@ class Context:
     def records(self, first: int, count: int) -> np.ndarray:
         out = np.zeros(count, dtype=RECORD_DTYPE)
         self.check(lib().mafm_records(self.h, first, count, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def margins(self, first: int, count: int) -> np.ndarray:
+        """(count, 4) float32 decision margins {f, e, p, 0} of the last scan
+        run with SCAN_MARGINS (mafm_margins)."""
+        out = np.zeros((count, 4), dtype=np.float32)
+        self.check(lib().mafm_margins(self.h, first, count, out.ctypes.data_as(_P(ctypes.c_float))))
         return out
 
     def adjoint(self, system: int, n_sites: int, tip: np.ndarray, dummy: np.ndarray, g: np.ndarray,

@@ -56,6 +56,39 @@ def _read_job(input_path: str):
     return types, q, atoms, opts["tipatom"][0], qbase
 
 
+def _inv3(m):
+    """Inverse of a row-major 3x3 by cofactors, operation for operation as
+    the library computes the cell inverse (Matrix3::inverse,
+    matrices.hpp:89-115)."""
+    det = m[0] * (m[4] * m[8] - m[5] * m[7])
+    det += m[1] * (m[5] * m[6] - m[3] * m[8])
+    det += m[2] * (m[3] * m[7] - m[4] * m[6])
+    inv_det = 1 / det
+    a, b, c, d, e, f, g, h, i = m
+    return [inv_det * (e * i - f * h), -inv_det * (b * i - c * h), inv_det * (b * f - c * e),
+            -inv_det * (d * i - f * g), inv_det * (a * i - c * g), -inv_det * (a * f - c * d),
+            inv_det * (d * h - e * g), -inv_det * (a * h - b * g), inv_det * (a * e - b * d)]
+
+
+def wrap_dummy_xy(cell, offset, x: np.ndarray, y: np.ndarray):
+    """System::setDummyXY's wrap of pixels (x, y) into the unit cell
+    (system.cpp:139-162): the dummy position the scan actually used when the
+    job has a cell.  Every product and sum is rounded on its own, in the
+    order the scan kernels use, so pixels on a cell face wrap the same way."""
+    m = [float(v) for v in cell]
+    inv = _inv3(m)
+    o = [float(v) for v in offset]
+    tx, ty, tz = x - o[0], y - o[1], 0.0 - o[2]
+    qx = inv[0] * tx + inv[1] * ty + inv[2] * tz
+    qy = inv[3] * tx + inv[4] * ty + inv[5] * tz
+    qz = inv[6] * tx + inv[7] * ty + inv[8] * tz
+    qx = qx - np.floor(qx)
+    qy = qy - np.floor(qy)
+    ox = m[0] * qx + m[1] * qy + m[2] * qz
+    oy = m[3] * qx + m[4] * qy + m[5] * qz
+    return ox + o[0], oy + o[1]
+
+
 class ScanModel:
     """One rigid LJ (+ Coulomb) job whose image is differentiable."""
 
@@ -131,7 +164,12 @@ class ScanModel:
         stats = self.ctx.scan(self.scan_desc)
         rec = self.ctx.records(0, self.shape[0] * self.shape[1] * self.shape[2])
         self.last_records, self.last_stats = rec, stats
-        dummy = np.stack([rec["x"], rec["y"], rec["z"]], 1)
+        # records report the requested pixel (x, y); with a unit cell the
+        # scan ran at its wrapped image, where the tip relaxed
+        x, y = rec["x"].copy(), rec["y"].copy()
+        if self.desc.tip_pbc:
+            x, y = wrap_dummy_xy(self.desc.cell, self.desc.offset, x, y)
+        dummy = np.stack([x, y, rec["z"]], 1)
         tip = dummy + np.stack([rec["rx"], rec["ry"], rec["rz"]], 1)
         return rec["fz"].copy(), tip, dummy
 

@@ -24,6 +24,7 @@ hipError_t launch_grid_build(const ScanArgs& a, int mol, const int* n, const dou
 hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_blocks, int W, int lds_sites,
                             bool scalar_sites, bool mid_family, hipStream_t st);
 hipError_t launch_to_f32(const double4* in, float4* out, long n, hipStream_t st);
+hipError_t launch_record_totals(const DevRecord* rec, long n, int64_t* out, hipStream_t st);
 bool deltaf_weights(const mafm_deltaf_desc& d, int nz, double zhigh, double dz_input, mafm_deltaf_weights_out& o,
                     std::vector<double>* z_low, char* err, size_t err_len);
 hipError_t launch_delta_f(const DevRecord* rec, const mafm_deltaf_weights_out& w, int nx, int ny, int nz,
@@ -129,6 +130,9 @@ struct mafm_ctx {
     DevBuf<int32_t> d_order;
     DevBuf<int64_t> d_counters;
     DevBuf<double> d_tmp;
+    DevBuf<float4> d_margins;      // per-position decision margins (MAFM_SCAN_MARGINS)
+    bool have_margins = false;
+    uint64_t scan_generation = 0;  // bumped by every mafm_scan (records are replaced)
     std::vector<int64_t> mol_cost;
     size_t n_records = 0;
     bool have_systems = false;
@@ -321,6 +325,118 @@ static mafm_status build_tables(mafm_ctx* c, int n, const mafm_system_desc* sys,
     return MAFM_OK;
 }
 
+// Launch shape of a FAST scan: cols[m] pixel columns of molecule m; lanes /
+// waves_per_simd / waves_per_group 0 = automatic.  Uploads the work-stealing
+// state into a.
+struct FastPlan {
+    int G = 1, wps = 0, W = 1, n_blocks = 0, lds_sites = 0;
+    bool scalar_sites = false, mid_family = false;
+};
+
+static mafm_status plan_fast(mafm_ctx* c, const mafm_scan_desc* d, bool f32, const std::vector<long>& cols, int lanes,
+                             int waves_per_simd, int waves_per_group, ScanArgs& a, FastPlan& p) {
+    const int n_mols = (int)c->mols.size();
+    long total_cols = 0;
+    for (long v : cols) total_cols += v;
+    // kernel family: FIRE + midpoint on non-periodic systems runs the
+    // specialised kernels (built for three waves per SIMD); periodic
+    // systems, whose image loop needs more registers, and every other
+    // minimiser run the generic ones (two waves per SIMD)
+    bool periodic = false;
+    for (const DevMol& m : c->mols) periodic |= m.n_images > 1;
+    p.mid_family = d->minimiser == MAFM_MIN_FIRE && d->integrator == MAFM_INT_MIDPOINT && !periodic;
+    p.wps = waves_per_simd > 0 ? waves_per_simd : (p.mid_family ? 3 : 2);
+    const long lanes_total = (long)c->cus * 4 * p.wps * 64;
+    int64_t max_sites = 1;
+    for (int m = 0; m < n_mols; ++m)
+        if (cols[m] > 0) max_sites = std::max<int64_t>(max_sites, c->mols[m].force_grid >= 0 ? 1 : c->mols[m].n_sites);
+    if (lanes > 0) {
+        p.G = lanes;
+    } else {
+        p.G = 1;
+        // split pixels over lanes until the columns give every resident lane
+        // two pixels' share of work, and a lane keeps >= 8 sites
+        while (p.G < 64 && (double)total_cols * p.G < 2.0 * lanes_total && p.G * 8 <= max_sites) p.G <<= 1;
+    }
+    // a workgroup stages its molecule's sites in LDS (64 KiB at most); larger
+    // molecules are streamed with scalar loads, one lane per pixel
+    const size_t site_bytes = f32 ? sizeof(Site32) : sizeof(SiteF64);
+    p.scalar_sites = (size_t)max_sites * site_bytes > 64 * 1024;
+    if (p.scalar_sites && p.G > 1) {
+        if (lanes > 0)
+            return fail(c, MAFM_ERR_INVALID, "lanes_per_pixel > 1 needs the sites of one molecule to fit 64 KiB of LDS");
+        p.G = 1;
+    }
+    p.lds_sites = (int)max_sites;
+    // waves per workgroup share one LDS copy of the sites: grow W until the
+    // LDS a CU holds admits wps waves per SIMD
+    p.W = 1;
+    if (waves_per_group > 0) {
+        p.W = waves_per_group;
+    } else if (!p.scalar_sites) {
+        const size_t lds_bytes = std::max<size_t>((size_t)max_sites * site_bytes, 1);
+        const long blocks_per_cu = (long)(c->lds_per_cu / lds_bytes);
+        while (p.W < 8 && blocks_per_cu * p.W < 4L * p.wps) ++p.W;
+    }
+    const long groups_per_wave = 64 / p.G;
+    const long need_waves = std::max<long>((total_cols + groups_per_wave - 1) / groups_per_wave, 1);
+    int n_waves = (int)std::min<long>((long)c->cus * 4 * p.wps, need_waves);
+    n_waves = std::max(n_waves, 1);
+    p.n_blocks = (n_waves + p.W - 1) / p.W;
+    // Molecules by rank: decreasing work per column (sites x images), so the
+    // longest columns start first.  Initial workgroup shares: when the batch
+    // needs more workgroups than the grid holds, fill ranks in order with
+    // enough workgroups to take every column of the molecule at once
+    // (longest-processing-time first); otherwise share in proportion to each
+    // molecule's total work.
+    std::vector<int32_t> zero(n_mols, 0), order(n_mols);
+    for (int m = 0; m < n_mols; ++m) order[m] = m;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        const bool wx = cols[x] > 0, wy = cols[y] > 0;
+        if (wx != wy) return wx;
+        return c->mol_cost[x] > c->mol_cost[y];
+    });
+    const long cols_per_block = groups_per_wave * (long)p.W;
+    long need_blocks_total = 0;
+    for (int m = 0; m < n_mols; ++m) need_blocks_total += (cols[m] + cols_per_block - 1) / cols_per_block;
+    std::vector<double> share(n_mols, 0.0);
+    if (need_blocks_total > p.n_blocks) {
+        long left = p.n_blocks;
+        for (int k = 0; k < n_mols && left > 0; ++k) {
+            const long b = std::min((cols[order[k]] + cols_per_block - 1) / cols_per_block, left);
+            share[k] = (double)b;
+            left -= b;
+        }
+    } else {
+        for (int k = 0; k < n_mols; ++k) share[k] = (double)c->mol_cost[order[k]] * (double)cols[order[k]];
+    }
+    double tot = 0;
+    for (double v : share) tot += v;
+    if (!(tot > 0)) {
+        tot = 1;
+        share[0] = 1;
+    }
+    std::vector<float> cum(n_mols + 1, 0.f);
+    double run = 0;
+    for (int k = 0; k < n_mols; ++k) {
+        run += share[k];
+        cum[k + 1] = (float)(run / tot);
+    }
+    cum[n_mols] = 1.0f;
+    HIPCHK(c, c->d_next.reserve(n_mols));
+    HIPCHK(c, c->d_cum.reserve(n_mols + 1));
+    HIPCHK(c, c->d_order.reserve(n_mols));
+    HIPCHK(c, hipMemcpyAsync(c->d_next.p, zero.data(), n_mols * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_cum.p, cum.data(), (n_mols + 1) * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_order.p, order.data(), n_mols * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    // the host vectors die with this call: the copies must be done first
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    a.next_col = c->d_next.p;
+    a.mol_cum = c->d_cum.p;
+    a.mol_order = c->d_order.p;
+    return MAFM_OK;
+}
+
 extern "C" {
 
 mafm_status mafm_create(int device, mafm_ctx** out) {
@@ -367,6 +483,7 @@ static void destroy_now(mafm_ctx* c) {
     c->d_order.release();
     c->d_counters.release();
     c->d_tmp.release();
+    c->d_margins.release();
     for (auto& kv : c->twiddles) kv.second.release();
     c->d_flag.release();
     (void)hipEventDestroy(c->ev0);
@@ -543,16 +660,26 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
     if ((int64_t)d->nx * d->ny > INT32_MAX) return fail(c, MAFM_ERR_INVALID, "more than 2^31 pixel columns");
     if (!std::isfinite(d->dz) || d->dz == 0 || !std::isfinite(d->dt))
         return fail(c, MAFM_ERR_INVALID, "dz must be finite and non-zero, dt finite");
+    if (d->precision != MAFM_PREC_PLAIN64 && d->lanes_per_pixel > 0) {
+        const int G = d->lanes_per_pixel;
+        if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16 && G != 32 && G != 64)
+            return fail(c, MAFM_ERR_INVALID, "lanes_per_pixel must be a power of two <= 64");
+    }
+    if (d->waves_per_group != 0 && (d->waves_per_group < 1 || d->waves_per_group > 8))
+        return fail(c, MAFM_ERR_INVALID, "waves_per_group must be 1 .. 8");
     HIPCHK(c, hipSetDevice(c->device));
     const int n_mols = (int)c->mols.size();
     const size_t per_mol = (size_t)d->nx * d->ny * d->nz;
     const size_t n_rec = per_mol * n_mols;
+    const long cols_per_mol = (long)d->nx * d->ny;
     for (int m = 0; m < n_mols; ++m) c->mols[m].rec_off = (int64_t)(per_mol * m);
     HIPCHK(c, hipMemcpyAsync(c->d_mols.p, c->mols.data(), n_mols * sizeof(DevMol), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, c->d_rec.reserve(n_rec));
     c->n_records = n_rec;
     c->last_scan = *d;
     c->have_scan = true;
+    c->have_margins = false;
+    ++c->scan_generation;
     ScanArgs a = base_args(c);
     a.nx = d->nx; a.ny = d->ny; a.nz = d->nz;
     a.dx = d->dx; a.dy = d->dy; a.dz = d->dz; a.zhigh = d->zhigh;
@@ -561,112 +688,34 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
     a.rec = c->d_rec.p;
     // last groups of a wave: regroup over 64 lanes once that halves the sites per lane
     a.wide_tail = (d->options & MAFM_SCAN_REPRODUCIBLE) ? 0 : 2;
+    a.inv_2ftol = d->ftol > 0 ? 1.0 / (2.0 * d->ftol) : 0.0;
+    const bool fast = d->precision != MAFM_PREC_PLAIN64;
+    if (fast && (d->options & MAFM_SCAN_MARGINS)) {
+        HIPCHK(c, c->d_margins.reserve(n_rec));
+        a.margins = c->d_margins.p;
+        c->have_margins = true;
+    }
     HIPCHK(c, hipMemsetAsync(c->d_counters.p, 0, 8 * sizeof(int64_t), c->stream));
 
-    const long cols_per_mol = (long)d->nx * d->ny;
-    const long total_cols = cols_per_mol * n_mols;
-    int G = 1, wps = 0, n_waves = 0, lds_sites = 0, W = 1, n_blocks = 0;
-    bool scalar_sites = false, mid_family = false;
-    if (d->precision != MAFM_PREC_PLAIN64) {
-        // kernel family: FIRE + midpoint on non-periodic systems runs the
-        // specialised kernels (FP32: built for three waves per SIMD); periodic
-        // systems, whose image loop needs more registers, and every other
-        // minimiser run the generic ones (two waves per SIMD)
-        bool periodic = false;
-        for (const DevMol& m : c->mols) periodic |= m.n_images > 1;
-        mid_family = d->minimiser == MAFM_MIN_FIRE && d->integrator == MAFM_INT_MIDPOINT && !periodic;
-        const bool mid32 = mid_family && d->precision == MAFM_PREC_FAST32;
-        wps = d->waves_per_simd > 0 ? d->waves_per_simd : (mid32 ? 3 : 2);
-        const long lanes = (long)c->cus * 4 * wps * 64;
-        int64_t max_sites = 1;
-        for (const DevMol& m : c->mols) max_sites = std::max<int64_t>(max_sites, m.force_grid >= 0 ? 1 : m.n_sites);
-        if (d->lanes_per_pixel > 0) {
-            G = d->lanes_per_pixel;
-        } else {
-            G = 1;
-            while (G < 64 && (double)total_cols * G < 4.0 * lanes && G * 8 <= max_sites) G <<= 1;
-        }
-        if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16 && G != 32 && G != 64)
-            return fail(c, MAFM_ERR_INVALID, "lanes_per_pixel must be a power of two <= 64");
-        // a workgroup stages its molecule's sites in LDS (64 KiB at most); larger
-        // molecules are streamed with scalar loads, one lane per pixel
-        const size_t site_bytes = d->precision == MAFM_PREC_FAST32 ? sizeof(Site32) : sizeof(SiteF64);
-        scalar_sites = (size_t)max_sites * site_bytes > 64 * 1024;
-        if (scalar_sites && G > 1) {
-            if (d->lanes_per_pixel > 0)
-                return fail(c, MAFM_ERR_INVALID, "lanes_per_pixel > 1 needs the sites of one molecule to fit 64 KiB of LDS");
-            G = 1;
-        }
-        lds_sites = (int)max_sites;
-        // waves per workgroup share one LDS copy of the sites: grow W until the
-        // LDS a CU holds admits wps waves per SIMD
-        if (d->waves_per_group > 0) {
-            W = d->waves_per_group;
-            if (W < 1 || W > 8) return fail(c, MAFM_ERR_INVALID, "waves_per_group must be 1 .. 8");
-        } else if (!scalar_sites) {
-            const size_t lds_bytes = std::max<size_t>((size_t)max_sites * site_bytes, 1);
-            const long blocks_per_cu = (long)(c->lds_per_cu / lds_bytes);
-            while (W < 8 && blocks_per_cu * W < 4L * wps) ++W;
-        }
-        const long groups_per_wave = 64 / G;
-        const long need_waves = (total_cols + groups_per_wave - 1) / groups_per_wave;
-        n_waves = (int)std::min<long>((long)c->cus * 4 * wps, need_waves);
-        n_waves = std::max(n_waves, 1);
-        n_blocks = (n_waves + W - 1) / W;
-        n_waves = n_blocks * W;
-        // Molecules by rank: decreasing work per column (sites x images), so
-        // the longest columns start first.  Initial workgroup shares: when the
-        // batch needs more workgroups than the grid holds, fill ranks in order
-        // with enough workgroups to take every column of the molecule at once
-        // (longest-processing-time first); otherwise share in proportion to
-        // each molecule's total work.
-        std::vector<int32_t> zero(n_mols, 0), order(n_mols);
-        for (int m = 0; m < n_mols; ++m) order[m] = m;
-        std::stable_sort(order.begin(), order.end(),
-                         [&](int x, int y) { return c->mol_cost[x] > c->mol_cost[y]; });
-        const long cols_per_block = (64 / G) * (long)W;
-        const long need_blocks = (cols_per_mol + cols_per_block - 1) / cols_per_block;
-        std::vector<double> share(n_mols, 0.0);
-        if (need_blocks * n_mols > n_blocks) {
-            long left = n_blocks;
-            for (int k = 0; k < n_mols && left > 0; ++k) {
-                const long b = std::min(need_blocks, left);
-                share[k] = (double)b;
-                left -= b;
-            }
-        } else {
-            for (int k = 0; k < n_mols; ++k) share[k] = (double)c->mol_cost[order[k]];
-        }
-        double tot = 0;
-        for (double v : share) tot += v;
-        std::vector<float> cum(n_mols + 1, 0.f);
-        double run = 0;
-        for (int k = 0; k < n_mols; ++k) {
-            run += share[k];
-            cum[k + 1] = (float)(run / tot);
-        }
-        cum[n_mols] = 1.0f;
-        HIPCHK(c, c->d_next.reserve(n_mols));
-        HIPCHK(c, c->d_cum.reserve(n_mols + 1));
-        HIPCHK(c, c->d_order.reserve(n_mols));
-        HIPCHK(c, hipMemcpyAsync(c->d_next.p, zero.data(), n_mols * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->d_cum.p, cum.data(), (n_mols + 1) * sizeof(float), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->d_order.p, order.data(), n_mols * sizeof(int32_t), hipMemcpyHostToDevice,
-                                 c->stream));
-        a.next_col = c->d_next.p;
-        a.mol_cum = c->d_cum.p;
-        a.mol_order = c->d_order.p;
+    FastPlan plan;
+    if (fast) {
+        std::vector<long> cols(n_mols, cols_per_mol);
+        mafm_status s = plan_fast(c, d, d->precision == MAFM_PREC_FAST32, cols, d->lanes_per_pixel,
+                                  d->waves_per_simd, d->waves_per_group, a, plan);
+        if (s != MAFM_OK) return s;
     }
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    if (d->precision == MAFM_PREC_PLAIN64) {
+    if (!fast) {
         HIPCHK(c, launch_scan_plain(a, c->stream));
     } else {
         // 0 / 3: block-compensated (default), 1: Kahan, 2: plain FP32
         const int comp = d->compensation == 1 ? 1 : (d->compensation == 2 ? 0 : 2);
-        HIPCHK(c, launch_scan_fast(a, d->precision == MAFM_PREC_FAST32, comp, G, n_blocks, W, lds_sites, scalar_sites,
-                                   mid_family, c->stream));
+        HIPCHK(c, launch_scan_fast(a, d->precision == MAFM_PREC_FAST32, comp, plan.G, plan.n_blocks, plan.W,
+                                   plan.lds_sites, plan.scalar_sites, plan.mid_family, c->stream));
     }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    // the FAST kernels leave the step / convergence / grid totals to the records
+    if (fast) HIPCHK(c, launch_record_totals(c->d_rec.p, (long)n_rec, c->d_counters.p, c->stream));
     int64_t counters[8];
     HIPCHK(c, hipMemcpyAsync(counters, c->d_counters.p, sizeof counters, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -687,17 +736,29 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
         }
         // algorithmic pair evaluations: (steps * evals + one getOutput eval) per position
         stats->pair_evals = n_mols ? (counters[0] * evals_per_step + (int64_t)per_mol * n_mols) * (pe / n_mols) : 0;
-        stats->lanes_per_pixel = G;
+        stats->lanes_per_pixel = fast ? plan.G : 1;
         stats->group_trips = counters[4];
         stats->active_group_trips = counters[5];
         stats->wide_trips = counters[6];
-        stats->waves_per_simd = wps;
-        stats->grid_blocks = d->precision == MAFM_PREC_PLAIN64 ? (int)((total_cols + 255) / 256) : n_blocks;
-        stats->block_threads = d->precision == MAFM_PREC_PLAIN64 ? 256 : 64 * W;
+        stats->waves_per_simd = fast ? plan.wps : 0;
+        stats->grid_blocks = fast ? plan.n_blocks : (int)((cols_per_mol * n_mols + 255) / 256);
+        stats->block_threads = fast ? 64 * plan.W : 256;
     }
     if (counters[3] != 0)
         return fail(c, MAFM_ERR_OUT_OF_GRID, "%lld positions needed a force-grid point past the grid",
                     (long long)counters[3]);
+    return MAFM_OK;
+}
+
+mafm_status mafm_margins(mafm_ctx* c, size_t first, size_t count, float* out) {
+    if (!c || (!out && count)) return MAFM_ERR_INVALID;
+    if (!c->have_margins) return fail(c, MAFM_ERR_STATE, "the last scan did not record decision margins");
+    if (first + count > c->n_records)
+        return fail(c, MAFM_ERR_INVALID, "margins [%zu, %zu) beyond the %zu of the last scan", first, first + count,
+                    c->n_records);
+    if (count == 0) return MAFM_OK;
+    HIPCHK(c, hipMemcpyAsync(out, c->d_margins.p + first, count * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return MAFM_OK;
 }
 
@@ -963,5 +1024,7 @@ void ctx_retain(mafm_ctx* c) {
 void ctx_release(mafm_ctx* c) {
     if (c && --c->sims == 0 && c->destroy_pending) destroy_now(c);
 }
+
+uint64_t ctx_scan_generation(const mafm_ctx* c) { return c ? c->scan_generation : 0; }
 
 }  // namespace mafm

@@ -101,6 +101,13 @@ struct ScanArgs {
     int64_t* counters;        // [0] total steps [1] unconverged [2] grid warnings [3] out-of-grid
     int32_t wide_tail;        // > 0: regroup a wave's last groups over 64 lanes when that cuts
                               // the sites per lane by this factor; 0: never
+    // Decision margins (FAST kernels, diagnostics).  Per minimiser
+    // iteration: the force's distance from ftol (|f2 - ftol^2| / (2 ftol)),
+    // counted when the energy part of the test passes, the energy
+    // difference's distance from etol, counted when the force part passes,
+    // and FIRE's power P = v.f over |v|; per position the height's minima.
+    float4* margins;          // null: not recorded
+    double inv_2ftol;         // 1 / (2 ftol)
 };
 
 enum { REC_UNCONVERGED = 1, REC_GRID_CLAMP = 2, REC_OUT_OF_GRID = 4 };
@@ -118,12 +125,31 @@ __device__ inline cptr<T> as_const(const T* p) {
     return (cptr<T>)(p);
 }
 
-// Matrix3::multiply for a row-major 3x3 (matrices.hpp:146-153).
+// Matrix3::multiply for a row-major 3x3 (matrices.hpp:146-153), every product
+// and sum rounded on its own as in the reference.
 template <typename P>
 __device__ __host__ inline void mat_vec(P m, double x, double y, double z, double& ox, double& oy, double& oz) {
+#pragma clang fp contract(off)
     ox = m[0] * x + m[1] * y + m[2] * z;
     oy = m[3] * x + m[4] * y + m[5] * z;
     oz = m[6] * x + m[7] * y + m[8] * z;
+}
+
+// System::setDummyXY's wrap of a pixel into the unit cell (system.cpp:139-162):
+// cell-frame coordinates of (x, y, 0) - offset, the fractional parts of the
+// first two, back to Cartesian.  Each product and sum is rounded on its own
+// (no fused multiply-add), as the reference computes it, so a pixel on a cell
+// face wraps the same way in every kernel and in host code that restates it.
+template <typename P>
+__device__ __host__ inline void wrap_xy(P cell, P inv, P off, double x, double y, double& px, double& py) {
+#pragma clang fp contract(off)
+    double qx, qy, qz, ox, oy, oz;
+    mat_vec(inv, x - off[0], y - off[1], 0 - off[2], qx, qy, qz);
+    qx -= floor(qx);
+    qy -= floor(qy);
+    mat_vec(cell, qx, qy, qz, ox, oy, oz);
+    px = ox + off[0];
+    py = oy + off[1];
 }
 
 }  // namespace mafm

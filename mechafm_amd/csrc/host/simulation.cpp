@@ -891,13 +891,23 @@ void Simulation::run(const mafm_run_opts& r, mafm_scan_stats* stats) {   // simu
     if (!initialized_) fail(MAFM_ERR_STATE, "run before initialize");
     check(ctx_, mafm_set_systems(ctx_, 1, &desc_), "set system");
     mafm_scan_desc d = scanDesc(r);
+    scan_generation_ = 0;
     check(ctx_, mafm_scan(ctx_, &d, &last_stats_), "scan");
     record_base_ = 0;
     system_index_ = 0;
+    scan_generation_ = ctx_scan_generation(ctx_);
     if (stats) *stats = last_stats_;
 }
 
+void Simulation::requireOwnRecords() const {
+    if (scan_generation_ == 0) fail(MAFM_ERR_STATE, "no records: this simulation has not run");
+    if (scan_generation_ != ctx_scan_generation(ctx_))
+        fail(MAFM_ERR_STATE, "the context has run another scan since this simulation's: its records are gone "
+                             "(run the simulation again)");
+}
+
 void Simulation::fetchRecords() {
+    requireOwnRecords();
     size_t n = (size_t)n_points_[0] * n_points_[1] * n_points_[2];
     records_.resize(n);
     check(ctx_, mafm_records(ctx_, record_base_, n, records_.data()), "records");
@@ -1025,6 +1035,7 @@ void Simulation::writeDeltaF(const mafm_deltaf_desc& d) {
     check(ctx_, mafm_deltaf_weights(ctx_, &d, nz, o.zhigh, o.dz, &w), "frequency-shift window");
     if (w.n_images == 0) fail(MAFM_ERR_INVALID, "%d heights do not fill a %d-height window", nz, w.n_weights);
     std::vector<double> df((size_t)w.n_images * nx * ny), zlow(w.n_images);
+    requireOwnRecords();
     check(ctx_, mafm_delta_f(ctx_, system_index_, &d, df.data(), zlow.data()), "frequency shift");
     for (int p = 0; p < w.n_images; ++p) {
         char name[512];
@@ -1168,6 +1179,10 @@ mafm_status mafm_batch_run(int n, mafm_sim* const* sims, const mafm_run_opts* o,
                 throw Error(MAFM_ERR_INVALID, "batch members must share the scan options");
             descs.push_back(s.systemDesc());
         }
+        for (int m = 0; m < n; ++m) {
+            sims[m]->sim.records_.clear();
+            sims[m]->sim.scan_generation_ = 0;
+        }
         mafm_status st = mafm_set_systems(ctx, n, descs.data());
         if (st != MAFM_OK) throw Error(st, mafm_last_error(ctx));
         mafm_scan_stats ss{};
@@ -1175,10 +1190,10 @@ mafm_status mafm_batch_run(int n, mafm_sim* const* sims, const mafm_run_opts* o,
         if (st != MAFM_OK) throw Error(st, mafm_last_error(ctx));
         const size_t per = (size_t)d0.nx * d0.ny * d0.nz;
         for (int m = 0; m < n; ++m) {
-            sims[m]->sim.records_.clear();
             sims[m]->sim.record_base_ = per * m;
             sims[m]->sim.system_index_ = m;
             sims[m]->sim.last_stats_ = ss;
+            sims[m]->sim.scan_generation_ = mafm::ctx_scan_generation(ctx);
         }
         if (stats) *stats = ss;
     });
@@ -1205,6 +1220,7 @@ mafm_status mafm_sim_settings(mafm_sim* s, char* buf, size_t size, size_t* neede
 mafm_status mafm_sim_records(mafm_sim* s, size_t first, size_t count, mafm_record* out) {
     if (!s) return MAFM_ERR_INVALID;
     return guarded(s, [&]() {
+        s->sim.requireOwnRecords();
         mafm_status st = mafm_records(s->sim.ctx_, s->sim.record_base_ + first, count, out);
         if (st != MAFM_OK) throw Error(st, mafm_last_error(s->sim.ctx_));
     });

@@ -295,6 +295,10 @@ class Simulation {
     // and it is system `system_index_` of the launch
     size_t record_base_ = 0;
     int system_index_ = 0;
+    // the context's scan count right after this simulation's scan (0: none):
+    // another scan on the context replaces the records
+    uint64_t scan_generation_ = 0;
+    void requireOwnRecords() const;
 
   private:
     void calculateTipDummyDistance();

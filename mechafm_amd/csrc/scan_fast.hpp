@@ -52,6 +52,25 @@ __device__ inline double rsq64(double x) {
     return y;
 }
 
+// Pair-term reciprocals: the hardware seeds are good to ~5e-8 relative
+// (v_rsq_f64, v_rcp_f64 on gfx950), one Newton step takes them to ~4e-15,
+// two orders below what the FP64 trajectories need to keep the reference's
+// FIRE decisions.
+#ifndef MAFM_PAIR64_NEWTON
+#define MAFM_PAIR64_NEWTON 1
+#endif
+__device__ inline double rsq64p(double x) {
+    if (MAFM_PAIR64_NEWTON == 2) return rsq64(x);
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y, y, 1.0);          // 1 - x y^2
+    return fma(0.5 * y, e, y);
+}
+__device__ inline double rcp64p(double x) {
+    if (MAFM_PAIR64_NEWTON == 2) return rcp64(x);
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
 // Site source of an evaluation: the copy of the molecule's table staged in
 // the workgroup's LDS (the G lanes of a pixel read G different sites, groups
 // read the same ones: broadcast), or, SC = true, for a molecule too large for
@@ -352,10 +371,10 @@ __device__ inline void pair64(const SiteF64& st, double tx, double ty, double tz
     if (SEG <= 1) {
         double ir = 0, ir2;
         if (SEG == 0) {
-            ir = rsq64(r2);
+            ir = rsq64p(r2);
             ir2 = ir * ir;
         } else {
-            ir2 = rcp64(r2);
+            ir2 = rcp64p(r2);
         }
         const double ir6 = ir2 * ir2 * ir2;
         const double fl = fma(st.p1, ir6, -st.p0);   // 12 es12 / r^6 - 6 es6
@@ -368,7 +387,7 @@ __device__ inline void pair64(const SiteF64& st, double tx, double ty, double tz
             fs = fl * ir6 * ir2;
         }
     } else {
-        const double ir = rsq64(r2);
+        const double ir = rsq64p(r2);
         const double r = r2 * ir;
         const double x = exp(st.p1 * (st.p2 - r));
         const double xm = x - 1.0;
@@ -385,19 +404,23 @@ __device__ inline void pair64(const SiteF64& st, double tx, double ty, double tz
     fz = fs * dz;
 }
 
+#ifndef MAFM_F64_BLOCK
+#define MAFM_F64_BLOCK 4
+#endif
 template <int SEG, bool SC, bool E>
 __device__ inline void seg_sum64(cptr<DevMol> M, const Sites<SC, SiteF64>& s, int n, int gl, int S, double px,
                                  double py, double pz, double acc[4]) {
     const int n_img = M->n_images;
-    const int full = (n / (4 * S)) * (4 * S);
-    for (int t0 = 0; t0 < full; t0 += 4 * S) {
-        SiteF64 st[4];
+    constexpr int B = MAFM_F64_BLOCK;
+    const int full = (n / (B * S)) * (B * S);
+    for (int t0 = 0; t0 < full; t0 += B * S) {
+        SiteF64 st[B];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) st[u] = s[t0 + u * S + gl];
+        for (int u = 0; u < B; ++u) st[u] = s[t0 + u * S + gl];
         for (int im = 0; im < n_img; ++im) {
             const double tx = px - M->shift[im][0], ty = py - M->shift[im][1], tz = pz - M->shift[im][2];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < B; ++u) {
                 double fx, fy, fz, e;
                 pair64<SEG>(st[u], tx, ty, tz, fx, fy, fz, e);
                 acc[0] += fx; acc[1] += fy; acc[2] += fz;
@@ -610,6 +633,27 @@ __device__ inline bool converged(const ScanArgs& a, double fx, double fy, double
     }
 }
 
+// How near an iteration's decisions came to flipping, folded into the
+// height's running minima (force units for f and p, energy units for e):
+//   f  |f2 - ftol^2| / (2 ftol), the force's distance from ftol to first
+//      order, counted when the energy part of the test passes;
+//   e  ||ediff| - etol|, counted when the force part passes
+//      (checkConvergence, minimiser.cpp:15-37);
+// FIRE's P < 0 test (minimiser.cpp:107-126) is folded in by fire_margin.
+__device__ inline void test_margins(const ScanArgs& a, double f2, double ediff, float& mgf, float& mge) {
+    const double ae = fabs(ediff);
+    if (a.minterm == 1 || (a.minterm == 2 && ae < a.etol))
+        mgf = fminf(mgf, (float)(fabs(f2 - a.ftol * a.ftol) * a.inv_2ftol));
+    if (a.minterm == 0 || (a.minterm == 2 && f2 < a.ftol * a.ftol)) mge = fminf(mge, (float)fabs(ae - a.etol));
+}
+
+// P = v.f over |v| (iv = 1 / |v|): the component of the force along the
+// velocity, whose sign decides FIRE's velocity reset.  v = 0 gives P = 0
+// exactly on any trajectory that also has v = 0, so no margin is counted.
+__device__ inline void fire_margin(double p, double v2, double iv, float& mgp) {
+    if (v2 > 0) mgp = fminf(mgp, (float)(fabs(p) * iv));
+}
+
 // Molecules are visited by rank (decreasing work per column: the longest
 // columns start first and the tail of the launch is made of short ones).
 // Rank of workgroup `block`'s first molecule from the initial shares.
@@ -624,11 +668,11 @@ __device__ int pick_rank(const ScanArgs& a, int block, int n_blocks) {
 }
 
 // Next rank after rk whose molecule has columns left, or -1.
-__device__ int next_rank(const ScanArgs& a, int rk, int n_cols) {
+__device__ int next_rank(const ScanArgs& a, int rk) {
     for (int q = 1; q <= a.n_mols; ++q) {
         int cand = rk + q;
         if (cand >= a.n_mols) cand -= a.n_mols;
-        if (__atomic_load_n(&a.next_col[a.mol_order[cand]], __ATOMIC_RELAXED) < n_cols) return cand;
+        if (__atomic_load_n(&a.next_col[a.mol_order[cand]], __ATOMIC_RELAXED) < a.nx * a.ny) return cand;
     }
     return -1;
 }
@@ -650,6 +694,7 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
     const bool leader = gl == 0;
     const int n_cols = a.nx * a.ny;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const bool track = a.margins != nullptr;
 
     int rk = __builtin_amdgcn_readfirstlane(pick_rank(a, blockIdx.x, gridDim.x));
     int m = a.mol_order[rk];
@@ -687,8 +732,7 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
     double ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0, cx = 0, cy = 0, cz = 0, ce = 0;
     double Px = 0, Py = 0, Pz = 1.0e3;
     bool ts = false;
-    int64_t steps = 0;
-    int32_t unconv = 0, warns = 0, oogs = 0;
+    float mgf = INFINITY, mge = INFINITY, mgp = INFINITY;   // the height's decision margins
     const double dt_max = 10 * a.dt;
     const bool align = MID || (a.minimiser != 0 && a.integrator == 1);
 
@@ -715,16 +759,7 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
                 j = col - i * a.ny;
                 const double x = i * a.dx, y = j * a.dy;
                 double px = x, py = y;
-                if (M->tip_pbc) {
-                    double qx, qy, qz;
-                    mat_vec(M->cell_inv, x - M->offset[0], y - M->offset[1], 0 - M->offset[2], qx, qy, qz);
-                    qx -= floor(qx);
-                    qy -= floor(qy);
-                    double ox, oy, oz;
-                    mat_vec(M->cell, qx, qy, qz, ox, oy, oz);
-                    px = ox + M->offset[0];
-                    py = oy + M->offset[1];
-                }
+                if (M->tip_pbc) wrap_xy(M->cell, M->cell_inv, M->offset, x, y, px, py);
                 dmx = px; dmy = py; dmz = a.zhigh;
                 tx = px; ty = py; tz = a.zhigh - M->tip_dummy_d;
                 vx = vy = vz = 0;
@@ -733,6 +768,7 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
                 k = 0;
                 // begin z-step
                 n_tot = 1; dt = a.dt; alpha = 0.1; nnn = 0; flags = 0;
+                mgf = mge = mgp = INFINITY;
                 if (n_tot >= a.maxsteps) { ph = PH_OUT; ts = true; Px = tx; Py = ty; Pz = tz; }
                 else { prev_e = e; ph = 0; ts = false; Px = tx; Py = ty; Pz = tz; }
             }
@@ -744,11 +780,11 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
                 // m's counter is spent, so every wave of the workgroup arrives
                 // here after its last column; one lane picks for all of them
                 __syncthreads();
-                if (threadIdx.x == 0) next_shared = next_rank(a, rk, n_cols);
+                if (threadIdx.x == 0) next_shared = next_rank(a, rk);
                 __syncthreads();
                 nr = next_shared;
             } else {
-                nr = next_rank(a, rk, n_cols);
+                nr = next_rank(a, rk);
             }
             nr = __builtin_amdgcn_readfirstlane(nr);
             if (nr < 0) break;
@@ -861,11 +897,9 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
                 rec.e = r[3];
                 rec.n_steps = n_tot;
                 rec.flags = flags | (n_tot >= a.maxsteps ? REC_UNCONVERGED : 0);
-                a.rec[M->rec_off + ((long)col * a.nz + k)] = rec;
-                steps += n_tot;
-                unconv += n_tot >= a.maxsteps;
-                warns += (flags & REC_GRID_CLAMP) != 0;
-                oogs += (flags & REC_OUT_OF_GRID) != 0;
+                const long ri = M->rec_off + ((long)col * a.nz + k);
+                a.rec[ri] = rec;
+                if (a.margins) a.margins[ri] = make_float4(mgf, mge, mgp, 0.f);
             }
             dmz -= a.dz;
             tz -= a.dz;
@@ -875,6 +909,7 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
                 continue;
             }
             n_tot = 1; dt = a.dt; alpha = 0.1; nnn = 0; flags = 0;
+            mgf = mge = mgp = INFINITY;
             if (n_tot >= a.maxsteps) { ph = PH_OUT; ts = true; }
             else { prev_e = e; ph = 0; ts = false; }
             Px = tx; Py = ty; Pz = tz;
@@ -884,6 +919,7 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
         if (!MID && a.minimiser == 0) {
             // SDMinimisation (minimiser.cpp:39-65): evaluate, test, move
             fx = r[0]; fy = r[1]; fz = r[2]; e = r[3];
+            if (track) test_margins(a, fx * fx + fy * fy + fz * fz, e - prev_e, mgf, mge);
             if (converged(a, fx, fy, fz, e - prev_e)) {
                 ph = PH_OUT;
             } else {
@@ -944,6 +980,7 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
         if (!step_done) continue;
 
         // FIREMinimisation iteration tail (minimiser.cpp:102-126)
+        if (track) test_margins(a, fx * fx + fy * fy + fz * fz, e - prev_e, mgf, mge);
         if (converged(a, fx, fy, fz, e - prev_e)) {
             ph = PH_OUT;
         } else {
@@ -955,7 +992,9 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
                 hx = fx * ifl; hy = fy * ifl; hz = fz * ifl;
             }
             const double v2 = vx * vx + vy * vy + vz * vz;
-            const double vl = v2 > 0 ? v2 * rsq64(v2) : 0.0;
+            const double iv = v2 > 0 ? rsq64(v2) : 0.0;
+            const double vl = v2 * iv;
+            if (track) fire_margin(p, v2, iv, mgp);
             vx = (1 - alpha) * vx + alpha * hx * vl;
             vy = (1 - alpha) * vy + alpha * hy * vl;
             vz = (1 - alpha) * vz + alpha * hz * vl;
@@ -978,17 +1017,13 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
         ts = ph == PH_OUT;
         Px = tx; Py = ty; Pz = tz;
     }
+    // step, convergence and grid-warning totals are summed from the records
+    // after the launch
     if (lane == 0) {
         atomicAdd((unsigned long long*)&a.counters[4], (unsigned long long)(trips * (64 / G)));
         if (wide_trips) atomicAdd((unsigned long long*)&a.counters[6], (unsigned long long)wide_trips);
     }
-    if (leader) {
-        if (active) atomicAdd((unsigned long long*)&a.counters[5], (unsigned long long)active);
-        if (steps) atomicAdd((unsigned long long*)&a.counters[0], (unsigned long long)steps);
-        if (unconv) atomicAdd((unsigned long long*)&a.counters[1], (unsigned long long)unconv);
-        if (warns) atomicAdd((unsigned long long*)&a.counters[2], (unsigned long long)warns);
-        if (oogs) atomicAdd((unsigned long long*)&a.counters[3], (unsigned long long)oogs);
-    }
+    if (leader && active) atomicAdd((unsigned long long*)&a.counters[5], (unsigned long long)active);
 }
 
 // Register budgets: the FIRE + midpoint FP32 kernel is held to three waves
@@ -1000,8 +1035,11 @@ __global__ void __launch_bounds__(512, MID ? 3 : 2) k_scan_fast32(ScanArgs a) {
     scan_fast_body<G, true, COMP, SC, MID>(a);
 }
 
+#ifndef MAFM_F64_MID_WAVES
+#define MAFM_F64_MID_WAVES 3
+#endif
 template <int G, bool SC, bool MID>
-__global__ void __launch_bounds__(512, 2) k_scan_fast64(ScanArgs a) {
+__global__ void __launch_bounds__(512, MID ? MAFM_F64_MID_WAVES : 2) k_scan_fast64(ScanArgs a) {
     scan_fast_body<G, false, 0, SC, MID>(a);
 }
 

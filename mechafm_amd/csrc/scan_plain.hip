@@ -322,16 +322,7 @@ __global__ void __launch_bounds__(256) k_scan_plain(ScanArgs a) {
     s.e = 0;
     // System::setDummyXY (system.cpp:139-162) and setDummyZ (system.hpp:47-50)
     double px = x, py = y;
-    if (m.tip_pbc) {
-        double qx, qy, qz;
-        mat_vec(m.cell_inv, x - m.offset[0], y - m.offset[1], 0 - m.offset[2], qx, qy, qz);
-        qx -= floor(qx);
-        qy -= floor(qy);
-        double ox, oy, oz;
-        mat_vec(m.cell, qx, qy, qz, ox, oy, oz);
-        px = ox + m.offset[0];
-        py = oy + m.offset[1];
-    }
+    if (m.tip_pbc) wrap_xy(m.cell, m.cell_inv, m.offset, x, y, px, py);
     s.dummy = {px, py, a.zhigh};
     s.tip = {px, py, a.zhigh - m.tip_dummy_d};
     int64_t steps = 0, unconv = 0;
@@ -394,16 +385,7 @@ __global__ void __launch_bounds__(256) k_grid_build(ScanArgs a, int mol, int nx,
     double x = i * sx + ox;
     double y = j * sy + oy;
     double px = x, py = y;
-    if (m.tip_pbc) {
-        double qx, qy, qz;
-        mat_vec(m.cell_inv, x - m.offset[0], y - m.offset[1], 0 - m.offset[2], qx, qy, qz);
-        qx -= floor(qx);
-        qy -= floor(qy);
-        double rx, ry, rz;
-        mat_vec(m.cell, qx, qy, qz, rx, ry, rz);
-        px = rx + m.offset[0];
-        py = ry + m.offset[1];
-    }
+    if (m.tip_pbc) wrap_xy(m.cell, m.cell_inv, m.offset, x, y, px, py);
     V3 t = {px, py, k * sz + oz};
     V3 f; double e;
     eval_tip(c, t, t, true, f, e);

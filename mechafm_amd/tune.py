@@ -9,8 +9,10 @@ tuner measures candidates on the workload itself (whole launches, kernel
 time from HIP events) and caches the winner under a signature of the
 workload and device, so a repeated workload is tuned once.
 
-Search: every G with 2 waves per SIMD, then 1 and 3 waves per SIMD for the
-best G (7 launches by default); waves_per_group stays automatic.
+Search: every G jointly with 2 and 3 waves per SIMD (the FIRE + midpoint
+kernels are built for three resident waves, the others for two), then one
+wave per SIMD for the best G (11 launches by default); waves_per_group stays
+automatic.
 """
 from __future__ import annotations
 
@@ -22,6 +24,9 @@ from typing import Callable, Dict, Iterable, List, Optional, Sequence, Tuple
 from . import abi
 
 DEFAULT_LANES = (1, 2, 4, 8, 16)
+WAVES = (2, 3)
+LDS_BYTES = 64 * 1024                      # one molecule's sites staged per workgroup
+SITE_BYTES = {"fast32": 32, "fast64": 64}  # bytes per site in LDS
 CACHE_ENV = "MAFM_TUNE_CACHE"
 
 
@@ -39,14 +44,18 @@ def _round2(x: float) -> float:
 
 def signature(sims: Sequence[abi.Simulation], precision: str, compensation: int = 0, options: int = 0,
               device: Optional[dict] = None) -> str:
-    """Key of a workload: device, arithmetic, scan geometry, minimiser, and
-    the size distribution of the systems (sites per system rounded to two
-    significant digits)."""
+    """Key of a workload: device, arithmetic, scan geometry, minimiser, the
+    size distribution of the systems (sites per system rounded to two
+    significant digits) and whether every system's sites fit the 64 KiB LDS
+    copy (otherwise the kernel streams them with one lane per pixel, so a
+    shape tuned on a system that fits does not carry over)."""
     info = sims[0].info()
     d = sims[0].scan_desc(precision)
     systems = []
+    lds_fit = True
     for s in sims:
         sd = s.system()
+        lds_fit &= sd.force_grid >= 0 or sd.n_sites * SITE_BYTES.get(precision, 64) <= LDS_BYTES
         systems.append((int(_round2(sd.n_sites)), int(sd.n_images or 1), int(sd.force_grid >= 0),
                         int(sd.potential_grid >= 0)))
     systems.sort()
@@ -57,6 +66,7 @@ def signature(sims: Sequence[abi.Simulation], precision: str, compensation: int 
         "systems": len(sims), "sites_total": _round2(sum(x[0] for x in systems)),
         "sites_max": max(x[0] for x in systems), "images": max(x[1] for x in systems),
         "force_grid": any(x[2] for x in systems), "potential_grid": any(x[3] for x in systems),
+        "lds_fit": lds_fit,
     }
     return json.dumps(key, sort_keys=True)
 
@@ -124,12 +134,12 @@ def tune(sims: Sequence[abi.Simulation], precision: str = "fast32", compensation
         return ms
 
     for g in lanes:
-        measure(g, 2)
+        for w in WAVES:
+            measure(g, w)
     if not tried:
         raise RuntimeError("no launch shape was accepted for this workload")
     g_best = min(tried, key=lambda t: t[2])[0]
-    for w in (1, 3):
-        measure(g_best, w)
+    measure(g_best, 1)
     g, w, ms = min(tried, key=lambda t: t[2])
     best = {"lanes_per_pixel": g, "waves_per_simd": w, "kernel_ms": ms,
             "tried": [{"lanes_per_pixel": a, "waves_per_simd": b, "kernel_ms": c} for a, b, c in tried]}

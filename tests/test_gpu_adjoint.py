@@ -42,8 +42,20 @@ def test_adjoint_matches_fp64_reference(gpu_ctx, tmp_path):
             assert np.linalg.norm(a - b) <= tol * np.linalg.norm(b), prec
 
 
-def test_autograd_matches_finite_differences(gpu_ctx, tmp_path):
-    m = _model(gpu_ctx, tmp_path)
+# "cell": vdw_pbc with a 9 A cell (the reference allows it only without
+# Coulomb, parse.cpp:390-399): the sites carry the +-1 in-plane images and
+# every scan wraps the dummy into the cell (System::setDummyXY), so the
+# relaxed tips sit at the wrapped pixels (pixel (0, 0) at (9, 9) here), not
+# at the pixel coordinates the records report
+CELL = dict(coulomb="off", vdw_pbc="on", cell_a="9.0 0.0 0.0", cell_b="0.0 9.0 0.0", cell_c="0.0 0.0 40.0")
+
+
+@pytest.mark.parametrize("cell", [False, True], ids=["open", "cell"])
+def test_autograd_matches_finite_differences(gpu_ctx, tmp_path, cell):
+    m = _model(gpu_ctx, tmp_path, **(CELL if cell else {}))
+    if cell:
+        rec = m.forward_np(*(t.detach().numpy() for t in m.site_constants(m.eps, m.sig, m.q)), m.pos.numpy())[2]
+        assert np.abs(rec[:, 0] - m.last_records["x"]).max() > 1.0, "no pixel wrapped: the case tests nothing"
     rng = np.random.default_rng(4)
     with torch.no_grad():
         target = m.image(m.eps * torch.from_numpy(1 + 0.05 * rng.choice([-1, 1], len(m.eps))),

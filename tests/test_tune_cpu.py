@@ -34,14 +34,17 @@ def fake_runner(times, calls):
 
 def test_search_picks_fastest_and_caches(tmp_path):
     sims = [FakeSim(120), FakeSim(300)]
-    times = {(1, 2): 50.0, (2, 2): 40.0, (4, 2): 45.0, (8, 2): 60.0, (16, 2): 90.0, (2, 1): 38.0, (2, 3): 39.0}
+    # G and waves per SIMD searched jointly: the best shape (1, 3) is one a
+    # search of G at two waves alone would miss (G = 2 wins there)
+    times = {(1, 2): 50.0, (1, 3): 36.0, (2, 2): 40.0, (2, 3): 39.0, (4, 2): 45.0, (4, 3): 44.0,
+             (8, 2): 60.0, (8, 3): 58.0, (16, 2): 90.0, (16, 3): 85.0, (1, 1): 70.0}
     calls = []
     cache = str(tmp_path / "tune.json")
     best = tune.tune(sims, "fast32", cache=cache, run=fake_runner(times, calls))
-    assert (best["lanes_per_pixel"], best["waves_per_simd"]) == (2, 1) and not best["cached"]
-    assert calls == [(1, 2), (2, 2), (4, 2), (8, 2), (16, 2), (2, 1), (2, 3)]
+    assert (best["lanes_per_pixel"], best["waves_per_simd"]) == (1, 3) and not best["cached"]
+    assert calls == [(1, 2), (1, 3), (2, 2), (2, 3), (4, 2), (4, 3), (8, 2), (8, 3), (16, 2), (16, 3), (1, 1)]
     again = tune.tune(sims, "fast32", cache=cache, run=fake_runner(times, calls))
-    assert again["cached"] and again["lanes_per_pixel"] == 2 and len(calls) == 7
+    assert again["cached"] and again["lanes_per_pixel"] == 1 and len(calls) == 11
     assert len(json.load(open(cache))) == 1
 
 
@@ -57,6 +60,8 @@ def test_signature_separates_workloads():
     assert a != tune.signature([FakeSim(400)], "fast32")
     assert a != tune.signature([FakeSim(120)], "fast64")
     assert a != tune.signature([FakeSim(120, 9)], "fast32")
+    # 2040 and 2049 FP32 sites round alike, but only the first fits 64 KiB of LDS
+    assert tune.signature([FakeSim(2040)], "fast32") != tune.signature([FakeSim(2049)], "fast32")
 
 
 def test_plain_has_nothing_to_tune():
