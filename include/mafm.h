@@ -254,6 +254,21 @@ mafm_status mafm_fft3d(mafm_ctx* ctx, const int32_t n[3], int inverse, const dou
 
 mafm_status mafm_set_systems(mafm_ctx* ctx, int n_systems, const mafm_system_desc* systems);
 mafm_status mafm_scan(mafm_ctx* ctx, const mafm_scan_desc* desc, mafm_scan_stats* stats);
+/* mafm_scan without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory during a scan
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The systems come from an earlier mafm_set_systems (set-up
+ * allocates, this call does not); the records go to the caller's device
+ * buffer `records` of `capacity` records, in mafm_records' order; the
+ * work-stealing state and counters use a fixed device workspace of the
+ * library (one per GPU, about 48 KiB, kept until the process ends), so at
+ * most 4096 systems per call; MAFM_SCAN_MARGINS is refused.  Calls on one
+ * context are serialised on its stream; calls from two contexts on one GPU
+ * that overlap in time race on the shared workspace.  Afterwards the
+ * context holds no records of its own (mafm_records and mafm_delta_f
+ * report MAFM_ERR_STATE / an empty range until the next mafm_scan). */
+mafm_status mafm_scan_noalloc(mafm_ctx* ctx, const mafm_scan_desc* desc, mafm_record* records, size_t capacity,
+                              mafm_scan_stats* stats);
 /* records of all systems: index ((system*nx + i)*ny + j)*nz + k */
 mafm_status mafm_records(mafm_ctx* ctx, size_t first, size_t count, mafm_record* out);
 /* Decision margins of the last scan (FAST kernels run with

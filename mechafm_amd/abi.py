@@ -140,6 +140,7 @@ SIGNATURES = {
     "mafm_fft3d": (ctypes.c_int, [_vp, _P(ctypes.c_int32), ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_double)]),
     "mafm_set_systems": (ctypes.c_int, [_vp, ctypes.c_int, _P(SystemDesc)]),
     "mafm_scan": (ctypes.c_int, [_vp, _P(ScanDesc), _P(ScanStats)]),
+    "mafm_scan_noalloc": (ctypes.c_int, [_vp, _P(ScanDesc), _vp, ctypes.c_size_t, _P(ScanStats)]),
     "mafm_records": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _vp]),
     "mafm_margins": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _P(ctypes.c_float)]),
     "mafm_eval_tip_surface": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_size_t, _P(ctypes.c_double),

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -25,6 +26,7 @@ hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_
                             bool scalar_sites, bool mid_family, hipStream_t st);
 hipError_t launch_to_f32(const double4* in, float4* out, long n, hipStream_t st);
 hipError_t launch_record_totals(const DevRecord* rec, long n, int64_t* out, hipStream_t st);
+hipError_t noalloc_scratch_address(void** out);
 bool deltaf_weights(const mafm_deltaf_desc& d, int nz, double zhigh, double dz_input, mafm_deltaf_weights_out& o,
                     std::vector<double>* z_low, char* err, size_t err_len);
 hipError_t launch_delta_f(const DevRecord* rec, const mafm_deltaf_weights_out& w, int nx, int ny, int nz,
@@ -132,7 +134,9 @@ struct mafm_ctx {
     DevBuf<double> d_tmp;
     DevBuf<float4> d_margins;      // per-position decision margins (MAFM_SCAN_MARGINS)
     bool have_margins = false;
-    uint64_t scan_generation = 0;  // bumped by every mafm_scan (records are replaced)
+    uint64_t scan_generation = 0;  // bumped by every scan (the records are replaced)
+    void* scratch = nullptr;       // device address of the fixed workspace (mafm_scan_noalloc)
+    std::unique_ptr<ScanScratch> scratch_img{new ScanScratch()};   // its host image
     std::vector<int64_t> mol_cost;
     size_t n_records = 0;
     bool have_systems = false;
@@ -334,7 +338,8 @@ struct FastPlan {
 };
 
 static mafm_status plan_fast(mafm_ctx* c, const mafm_scan_desc* d, bool f32, const std::vector<long>& cols, int lanes,
-                             int waves_per_simd, int waves_per_group, ScanArgs& a, FastPlan& p) {
+                             int waves_per_simd, int waves_per_group, ScanArgs& a, FastPlan& p,
+                             ScanScratch* host_ws = nullptr, ScanScratch* dev_ws = nullptr) {
     const int n_mols = (int)c->mols.size();
     long total_cols = 0;
     for (long v : cols) total_cols += v;
@@ -423,6 +428,16 @@ static mafm_status plan_fast(mafm_ctx* c, const mafm_scan_desc* d, bool f32, con
         cum[k + 1] = (float)(run / tot);
     }
     cum[n_mols] = 1.0f;
+    if (dev_ws) {
+        // fixed workspace: the caller copies the host image over in one go
+        std::copy(zero.begin(), zero.end(), host_ws->next_col);
+        std::copy(cum.begin(), cum.end(), host_ws->cum);
+        std::copy(order.begin(), order.end(), host_ws->order);
+        a.next_col = dev_ws->next_col;
+        a.mol_cum = dev_ws->cum;
+        a.mol_order = dev_ws->order;
+        return MAFM_OK;
+    }
     HIPCHK(c, c->d_next.reserve(n_mols));
     HIPCHK(c, c->d_cum.reserve(n_mols + 1));
     HIPCHK(c, c->d_order.reserve(n_mols));
@@ -436,6 +451,8 @@ static mafm_status plan_fast(mafm_ctx* c, const mafm_scan_desc* d, bool f32, con
     a.mol_order = c->d_order.p;
     return MAFM_OK;
 }
+
+static void destroy_now(mafm_ctx* c);
 
 extern "C" {
 
@@ -451,6 +468,12 @@ mafm_status mafm_create(int device, mafm_ctx** out) {
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         c->d_counters.reserve(8) != hipSuccess) {
         delete c;
+        return MAFM_ERR_HIP;
+    }
+    // the allocation-free scan's workspace: its address on this device, taken
+    // once with the device current
+    if (noalloc_scratch_address(&c->scratch) != hipSuccess || !c->scratch) {
+        destroy_now(c);
         return MAFM_ERR_HIP;
     }
     hipDeviceProp_t p;
@@ -649,7 +672,12 @@ mafm_status mafm_set_systems(mafm_ctx* c, int n, const mafm_system_desc* sys) {
     return MAFM_OK;
 }
 
-mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* stats) {
+// The scan behind mafm_scan (records and work-stealing state in buffers the
+// context allocates: out = nullptr, ws = nullptr) and mafm_scan_noalloc
+// (records into the caller's device buffer out of `cap` records, state in
+// the fixed workspace ws).
+static mafm_status scan_impl(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* stats, DevRecord* out, size_t cap,
+                             ScanScratch* ws) {
     if (!c || !d) return MAFM_ERR_INVALID;
     if (!c->have_systems) return fail(c, MAFM_ERR_STATE, "mafm_scan before mafm_set_systems");
     if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return fail(c, MAFM_ERR_INVALID, "empty scan grid");
@@ -667,25 +695,43 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
     }
     if (d->waves_per_group != 0 && (d->waves_per_group < 1 || d->waves_per_group > 8))
         return fail(c, MAFM_ERR_INVALID, "waves_per_group must be 1 .. 8");
-    HIPCHK(c, hipSetDevice(c->device));
     const int n_mols = (int)c->mols.size();
     const size_t per_mol = (size_t)d->nx * d->ny * d->nz;
     const size_t n_rec = per_mol * n_mols;
     const long cols_per_mol = (long)d->nx * d->ny;
+    if (ws) {
+        if (n_mols > NOALLOC_MAX_SYSTEMS)
+            return fail(c, MAFM_ERR_INVALID, "%d systems: the allocation-free scan takes at most %d", n_mols,
+                        NOALLOC_MAX_SYSTEMS);
+        if (!out) return fail(c, MAFM_ERR_INVALID, "the allocation-free scan needs a device record buffer");
+        if (cap < n_rec)
+            return fail(c, MAFM_ERR_INVALID, "record buffer holds %zu records, the scan writes %zu", cap, n_rec);
+        if (d->options & MAFM_SCAN_MARGINS)
+            return fail(c, MAFM_ERR_INVALID, "the allocation-free scan records no decision margins");
+    }
+    HIPCHK(c, hipSetDevice(c->device));
     for (int m = 0; m < n_mols; ++m) c->mols[m].rec_off = (int64_t)(per_mol * m);
     HIPCHK(c, hipMemcpyAsync(c->d_mols.p, c->mols.data(), n_mols * sizeof(DevMol), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, c->d_rec.reserve(n_rec));
-    c->n_records = n_rec;
-    c->last_scan = *d;
-    c->have_scan = true;
-    c->have_margins = false;
+    // the records of an earlier scan are no longer described by the context
     ++c->scan_generation;
+    c->have_margins = false;
+    if (out) {
+        c->have_scan = false;
+        c->n_records = 0;
+    } else {
+        HIPCHK(c, c->d_rec.reserve(n_rec));
+        c->n_records = n_rec;
+        c->last_scan = *d;
+        c->have_scan = true;
+        out = c->d_rec.p;
+    }
     ScanArgs a = base_args(c);
+    if (ws) a.counters = ws->counters;
     a.nx = d->nx; a.ny = d->ny; a.nz = d->nz;
     a.dx = d->dx; a.dy = d->dy; a.dz = d->dz; a.zhigh = d->zhigh;
     a.minimiser = d->minimiser; a.integrator = d->integrator; a.minterm = d->minterm;
     a.maxsteps = d->maxsteps; a.etol = d->etol; a.ftol = d->ftol; a.dt = d->dt;
-    a.rec = c->d_rec.p;
+    a.rec = out;
     // last groups of a wave: regroup over 64 lanes once that halves the sites per lane
     a.wide_tail = (d->options & MAFM_SCAN_REPRODUCIBLE) ? 0 : 2;
     a.inv_2ftol = d->ftol > 0 ? 1.0 / (2.0 * d->ftol) : 0.0;
@@ -695,14 +741,20 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
         a.margins = c->d_margins.p;
         c->have_margins = true;
     }
-    HIPCHK(c, hipMemsetAsync(c->d_counters.p, 0, 8 * sizeof(int64_t), c->stream));
+    // host image of the fixed workspace: zero counters plus the launch plan
+    ScanScratch* img = ws ? c->scratch_img.get() : nullptr;
+    if (ws) memset(img->counters, 0, sizeof img->counters);
+    else HIPCHK(c, hipMemsetAsync(c->d_counters.p, 0, 8 * sizeof(int64_t), c->stream));
 
     FastPlan plan;
     if (fast) {
         std::vector<long> cols(n_mols, cols_per_mol);
         mafm_status s = plan_fast(c, d, d->precision == MAFM_PREC_FAST32, cols, d->lanes_per_pixel,
-                                  d->waves_per_simd, d->waves_per_group, a, plan);
+                                  d->waves_per_simd, d->waves_per_group, a, plan, img, ws);
         if (s != MAFM_OK) return s;
+    }
+    if (ws) {
+        HIPCHK(c, hipMemcpyAsync(ws, img, sizeof(ScanScratch), hipMemcpyHostToDevice, c->stream));
     }
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     if (!fast) {
@@ -715,9 +767,9 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
     }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     // the FAST kernels leave the step / convergence / grid totals to the records
-    if (fast) HIPCHK(c, launch_record_totals(c->d_rec.p, (long)n_rec, c->d_counters.p, c->stream));
+    if (fast) HIPCHK(c, launch_record_totals(out, (long)n_rec, a.counters, c->stream));
     int64_t counters[8];
-    HIPCHK(c, hipMemcpyAsync(counters, c->d_counters.p, sizeof counters, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(counters, a.counters, sizeof counters, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     float ms = 0;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -748,6 +800,18 @@ mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* sta
         return fail(c, MAFM_ERR_OUT_OF_GRID, "%lld positions needed a force-grid point past the grid",
                     (long long)counters[3]);
     return MAFM_OK;
+}
+
+mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* stats) {
+    return scan_impl(c, d, stats, nullptr, 0, nullptr);
+}
+
+mafm_status mafm_scan_noalloc(mafm_ctx* c, const mafm_scan_desc* d, mafm_record* records, size_t capacity,
+                              mafm_scan_stats* stats) {
+    if (!c || !d) return MAFM_ERR_INVALID;
+    if (!c->scratch) return fail(c, MAFM_ERR_STATE, "no device workspace on this context");
+    return scan_impl(c, d, stats, reinterpret_cast<DevRecord*>(records), capacity,
+                     static_cast<ScanScratch*>(c->scratch));
 }
 
 mafm_status mafm_margins(mafm_ctx* c, size_t first, size_t count, float* out) {

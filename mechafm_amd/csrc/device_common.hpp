@@ -112,6 +112,17 @@ struct ScanArgs {
 
 enum { REC_UNCONVERGED = 1, REC_GRID_CLAMP = 2, REC_OUT_OF_GRID = 4 };
 
+// Work-stealing state and counters of one scan in a fixed layout: the
+// workspace of the allocation-free scan entry point (mafm_scan_noalloc), kept
+// in a file-scope device array of this size (scan_noalloc.hip).
+constexpr int NOALLOC_MAX_SYSTEMS = 4096;
+struct ScanScratch {
+    int64_t counters[8];
+    int32_t next_col[NOALLOC_MAX_SYSTEMS];
+    float cum[NOALLOC_MAX_SYSTEMS + 1];
+    int32_t order[NOALLOC_MAX_SYSTEMS];
+};
+
 // Read-only tables are read through the constant address space: with a
 // wave-uniform index the compiler then emits scalar loads into SGPRs (a plain
 // global pointer would get per-lane vector loads, because the kernels also
