@@ -58,3 +58,30 @@ def gpu_ctx():
     ctx = abi.Context(0)
     yield ctx
     ctx.close()
+
+
+SCALE = os.path.join(GOLDEN, "scale")
+
+
+def materialize_scale(case: str, dst: str):
+    """Regenerate a scale case's inputs with the seeded generator its
+    inputs.json names and check every file against the SHA-256 the
+    reference read; returns the input.scan paths (one per system)."""
+    import hashlib
+    import json
+    from mechafm_amd import workloads as W
+    spec = json.load(open(os.path.join(SCALE, case, "inputs.json")))
+    wls = getattr(W, spec["generator"])(dst, **spec["kwargs"])
+    wls = wls if isinstance(wls, list) else [wls]
+    assert len(wls) == len(spec["systems"])
+    for wl, sysd in zip(wls, spec["systems"]):
+        folder = os.path.dirname(wl.input_path)
+        for fn, digest in sysd["files"].items():
+            got = hashlib.sha256(open(os.path.join(folder, fn), "rb").read()).hexdigest()
+            assert got == digest, f"{case}: regenerated {sysd['folder']}/{fn} differs from the reference's input"
+    return [wl.input_path for wl in wls]
+
+
+def scale_records(case: str):
+    d = np.load(os.path.join(SCALE, case, "records.npz"))
+    return [tuple(int(v) for v in s) for s in d["shapes"]], d["records"]

@@ -14,6 +14,12 @@ sources -- and stores
 * for grid / external-potential cases, ``eval.npz``: tip positions and the
   reference's System::evalTipSurfaceForces at them.
 
+Scale cases (tests/golden/scale/<case>/) hold the reference's records for
+workloads at their own size, whose inputs are too large to commit (the
+2000-atom slab's 160 x 160 x 80 cube): ``inputs.json`` names the seeded
+generator call and the SHA-256 of every file it writes, so a test
+regenerates the inputs and checks they are the ones the reference read.
+
 Only data is committed; no reference source or binary enters the repository.
 """
 from __future__ import annotations
@@ -129,6 +135,64 @@ def make_case(name: str) -> None:
         print(f"{name}: {rec.shape[0]} records, steps mean {rec[:, 3].mean():.1f} max {rec[:, 3].max():.0f}")
 
 
+# scale cases: name -> (generator name in mechafm_amd.workloads, kwargs, eval samples)
+SCALE = {
+    # the periodic slab at full size (2000 atoms x 9 images, 160 x 160 x 80
+    # Hartree cube), 4 x 4 pixels spread over the cell, all 40 heights
+    "c4_full": ("make_c4", dict(dx="13.28125", dy="13.28125"), 512),
+    # all 64 molecules of the batched workload, 2 x 2 pixels each, 40 heights
+    "c3_batch": ("make_c3", dict(area="0.1 0.1"), 0),
+}
+SCALE_DIR = os.path.join(OUT, "scale")
+
+
+def file_hashes(folder: str) -> dict:
+    import hashlib
+    return {fn: hashlib.sha256(open(os.path.join(folder, fn), "rb").read()).hexdigest()
+            for fn in sorted(os.listdir(folder)) if not fn.endswith(".bin")}
+
+
+def make_scale_case(name: str) -> None:
+    import json
+    gen, kw, n_eval = SCALE[name]
+    dst = os.path.join(SCALE_DIR, name)
+    if os.path.isdir(dst):
+        shutil.rmtree(dst)
+    os.makedirs(dst)
+    with tempfile.TemporaryDirectory() as work:
+        wls = getattr(W, gen)(work, **kw)
+        wls = wls if isinstance(wls, list) else [wls]
+        shapes, recs, hashes = [], [], []
+        for wl in wls:
+            wdir = os.path.dirname(wl.input_path)
+            rec_bin = os.path.join(wdir, "records.bin")
+            harness("scan", wl.input_path, rec_bin)
+            raw = open(rec_bin, "rb").read()
+            hdr = np.frombuffer(raw[:16], dtype=np.int32)
+            recs.append(np.frombuffer(raw[16:], dtype=np.float64).reshape(-1, int(hdr[3])))
+            shapes.append(hdr[:3].copy())
+            os.remove(rec_bin)
+            hashes.append({"folder": os.path.relpath(wdir, work), "files": file_hashes(wdir)})
+        np.savez_compressed(os.path.join(dst, "records.npz"), shapes=np.array(shapes), records=np.concatenate(recs))
+        json.dump({"generator": gen, "kwargs": kw, "systems": hashes}, open(os.path.join(dst, "inputs.json"), "w"),
+                  indent=1)
+        if n_eval:
+            wl = wls[0]
+            wdir = os.path.dirname(wl.input_path)
+            rng = np.random.default_rng(99)
+            ax, ay = (float(v) for v in wl.options["area"].split())
+            zl, zh = float(wl.options["zlow"]), float(wl.options["zhigh"])
+            pos = np.stack([rng.uniform(0, ax, n_eval), rng.uniform(0, ay, n_eval),
+                            rng.uniform(zl - 3.76, zh - 3.76, n_eval)], axis=1)
+            pin, pout = os.path.join(wdir, "pos.bin"), os.path.join(wdir, "eval.bin")
+            pos.astype(np.float64).tofile(pin)
+            harness("eval", wl.input_path, pin, pout)
+            ev = np.fromfile(pout, dtype=np.float64).reshape(-1, 4)
+            np.savez_compressed(os.path.join(dst, "eval.npz"), pos=pos, fe=ev)
+    n = sum(len(r) for r in recs)
+    print(f"{name}: {len(wls)} systems, {n} records, steps mean {np.concatenate(recs)[:, 3].mean():.1f}")
+
+
 REF_EXAMPLES = "/root/reference/examples"
 REF_BINARY = os.path.join(ROOT, "oracle", "_ref", "mechafm-omp")
 
@@ -168,9 +232,11 @@ def make_example() -> None:
 def main(names=None) -> None:
     if not os.path.exists(HARNESS):
         raise SystemExit("build the reference harness first: make -C oracle ref")
-    for name in names or list(CASES) + ["example_graphene"]:
+    for name in names or list(CASES) + ["example_graphene"] + list(SCALE):
         if name == "example_graphene":
             make_example()
+        elif name in SCALE:
+            make_scale_case(name)
         else:
             make_case(name)
 

@@ -172,3 +172,22 @@ def test_fft_electrostatics_plane_wave():
     for c in range(3):
         f = g["fe"][:, c].reshape(n)
         np.testing.assert_allclose(f, damp * 2 * np.pi * kvec[c] * np.sin(phase), atol=1e-8 * abs(damp))
+
+
+def test_scale_inputs_regenerate(tmp_path):
+    """The scale cases' inputs come back from the seeded generators byte for
+    byte (SHA-256 of every file the reference read)."""
+    from conftest import materialize_scale
+    for case in ("c3_batch", "c4_full"):
+        materialize_scale(case, str(tmp_path / case))
+
+
+def test_oracle_matches_c3_batch(tmp_path):
+    """The oracle on all 64 molecules of the batched workload (2 x 2 pixels,
+    40 heights each) equals the reference's records bit for bit."""
+    from conftest import materialize_scale, scale_records
+    paths = materialize_scale("c3_batch", str(tmp_path))
+    shapes, ref = scale_records("c3_batch")
+    got = np.concatenate([O.scan(O.setup(p))[0] for p in paths])
+    np.testing.assert_array_equal(got[:, 3], ref[:, 3])
+    np.testing.assert_array_equal(got, ref)
