@@ -11,8 +11,9 @@
 #include "../../../include/mafm.h"
 #include "../../../include/mafm_sim.h"
 
-static int die(const char* what, const char* msg) {
-    fprintf(stderr, "+- ERROR: %s: %s\n", what, msg);
+// error() of the reference (messages.cpp:12-27): the message on stderr, exit code 1
+static int die(const char* msg) {
+    fprintf(stderr, "+- ERROR (on process 0): %s\n", msg);
     return 1;
 }
 
@@ -28,7 +29,7 @@ int main(int argc, char** argv) {
             if (p == "plain64") ro.precision = MAFM_PREC_PLAIN64;
             else if (p == "fast64") ro.precision = MAFM_PREC_FAST64;
             else if (p == "fast32") ro.precision = MAFM_PREC_FAST32;
-            else return die("precision", "expected plain64, fast64 or fast32");
+            else return die("--precision expects plain64, fast64 or fast32");
         } else if (!strcmp(argv[i], "--device") && i + 1 < argc) {
             device = atoi(argv[++i]);
         } else if (!strcmp(argv[i], "--deltaf")) {
@@ -38,27 +39,33 @@ int main(int argc, char** argv) {
         } else if (outdir.empty()) {
             outdir = argv[i];
         } else {
-            return die("command line", "Too many command line arguments!");
+            return die("Too many command line arguments!");
         }
     }
-    if (input.empty()) return die("command line", "Specify an input file to be read!");
-    printf("+- MechAFM rigid probe-particle scan on gfx950 (libmafm)\n");
+    if (input.empty()) return die("Specify an input file to be read!");
     mafm_ctx* ctx = nullptr;
     mafm_status st = mafm_create(device, &ctx);
-    if (st != MAFM_OK) return die("device", "no usable HIP device");
+    if (st != MAFM_OK) return die("no usable HIP device");
     mafm_sim* sim = nullptr;
     auto t0 = std::chrono::steady_clock::now();
+    printf("+- MechAFM rigid probe-particle scan on gfx950 (libmafm)\n");
     st = mafm_sim_open(input.c_str(), outdir.c_str(), ctx, &sim);
-    if (st == MAFM_OK) {                          // input warnings and settings, as the reference prints them
+    if (st == MAFM_OK) {
+        // input warnings (stderr) and the settings echo (stdout), as the
+        // reference prints them (messages.cpp:29-56, parse.cpp:407-473)
         size_t need = 0;
         mafm_sim_settings(sim, nullptr, 0, &need);
         std::string text(need, '\0');
         mafm_sim_settings(sim, &text[0], need, nullptr);
-        fputs(text.c_str(), stdout);
+        text.resize(strlen(text.c_str()));
+        const size_t echo = text.find("+- \n+- Input settings");
+        fputs(text.substr(0, echo).c_str(), stderr);
+        fflush(stderr);
+        if (echo != std::string::npos) fputs(text.substr(echo).c_str(), stdout);
         st = mafm_sim_initialize(sim);
     }
     if (st != MAFM_OK) {
-        int rc = die("set-up", mafm_sim_last_error(sim));
+        int rc = die(mafm_sim_last_error(sim));
         mafm_sim_close(sim);
         mafm_destroy(ctx);
         return rc;
@@ -74,20 +81,25 @@ int main(int argc, char** argv) {
     if (st == MAFM_OK && deltaf) st = mafm_sim_write_deltaf(sim, nullptr);
     auto t1 = std::chrono::steady_clock::now();
     if (st != MAFM_OK) {
-        int rc = die("scan", mafm_sim_last_error(sim));
+        int rc = die(mafm_sim_last_error(sim));
         mafm_sim_close(sim);
         mafm_destroy(ctx);
         return rc;
     }
+    // the reference's closing lines (mechafm.cpp:130-139, 197-200)
     long long n = (long long)info.nx * info.ny * info.nz;
+    const double wall = std::chrono::duration<double>(t1 - t0).count();
     printf("+- Simulation run finished\n+- Statistics:\n");
     printf("+-     Computed %lld tip positions\n", n);
     printf("+-     Needed %lld minimization steps in total\n", (long long)ss.total_steps);
     printf("+-     Which means approximately %.2f minimization steps per tip position\n",
            (double)ss.total_steps / (double)n);
-    printf("+-     The scan kernel took %.3f seconds (%.0f positions/s)\n", ss.kernel_ms / 1e3,
-           n / (ss.kernel_ms / 1e3));
-    printf("+-     The entire simulation took %.2f seconds\n", std::chrono::duration<double>(t1 - t0).count());
+    printf("+-     The simulation wall time is %.2f seconds\n", wall);
+    printf("+-     The entire simulation took %.2f seconds\n", wall);
+    printf("+- \n");
+    printf("+- How many x,y points did each process handle:\n");
+    printf("+-     Process %2d: %6d x,y points\n", 0, info.nx * info.ny);
+    printf("+- The scan kernel took %.3f seconds (%.0f positions/s)\n", ss.kernel_ms / 1e3, n / (ss.kernel_ms / 1e3));
     mafm_sim_close(sim);
     mafm_destroy(ctx);
     return 0;

@@ -84,9 +84,11 @@ bool on_off(const std::string& key, const std::string& v) {
     fail(MAFM_ERR_INVALID, "Option %s must be either on or off!", key.c_str());
 }
 
-std::vector<std::string> read_lines(const std::string& path, const char* what) {
+// lines of a text file; a missing file is the reference's error `missing`
+// (a printf format taking the path)
+std::vector<std::string> read_lines(const std::string& path, const char* missing) {
     std::ifstream f(path);
-    if (!f) fail(MAFM_ERR_INVALID, "%s %s does not exist!", what, path.c_str());
+    if (!f) fail(MAFM_ERR_INVALID, missing, path.c_str());
     std::vector<std::string> lines;
     std::string l;
     while (std::getline(f, l)) lines.push_back(l);
@@ -355,7 +357,7 @@ void Simulation::parseCommandLine(const std::string& input, const std::string& o
 
 void Simulation::readInputFile() {                // parse.cpp:83-405
     InputOptions& o = options_;
-    for (const std::string& line : read_lines(o.inputfile, "The file")) {
+    for (const std::string& line : read_lines(o.inputfile, "The file %s does not exist!")) {
         if (skip_line(line)) continue;
         std::vector<std::string> t = split(line);
         const std::string key = lower(t[0]);
@@ -457,7 +459,7 @@ void Simulation::readInputFile() {                // parse.cpp:83-405
 
 void Simulation::readXYZFile() {                  // parse.cpp:477-585
     InputOptions& o = options_;
-    std::vector<std::string> lines = read_lines(o.xyzfile, "No such file:");
+    std::vector<std::string> lines = read_lines(o.xyzfile, "No such file: %s!");
     int n_atoms = 0;
     bool first = true, realxyz = false;
     for (const std::string& line : lines) {
@@ -571,7 +573,7 @@ std::string Simulation::settingsText() const {
 void Simulation::readParameterFile() {            // parse.cpp:588-697
     InputOptions& o = options_;
     InteractionParameters& p = interaction_parameters_;
-    for (const std::string& line : read_lines(o.paramfile, "No parameter file")) {
+    for (const std::string& line : read_lines(o.paramfile, "No parameter file %s found!")) {
         if (skip_line(line)) continue;
         std::vector<std::string> t = split(line);
         if (t[0] == "atom") {
@@ -1022,7 +1024,10 @@ void Simulation::writeOutput(int threads) {
             fprintf(f, "    Needed %lld minimization steps in total\n", (long long)last_stats_.total_steps);
             fprintf(f, "    Which means approximately %.2f minimization steps per tip position\n",
                     (double)last_stats_.total_steps / n_pts);
-            fprintf(f, "    The scan kernel time is %.2f seconds\n", last_stats_.kernel_ms / 1000.0);
+            // the reference's two times (mechafm.cpp:137-150), from the job's opening
+            const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - opened_).count();
+            fprintf(f, "    The simulation wall time is %.2f seconds\n", wall);
+            fprintf(f, "    The entire simulation took %.2f seconds\n", wall);
             fclose(f);
         }
     }

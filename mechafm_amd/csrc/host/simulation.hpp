@@ -16,6 +16,7 @@
 #include <map>
 #include <memory>
 #include <stdexcept>
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -287,6 +288,8 @@ class Simulation {
     int n_points_[3] = {0, 0, 0};
     std::vector<mafm_record> records_;
     mafm_scan_stats last_stats_{};
+    // when the job was opened: the wall times of statistics.txt run from here
+    std::chrono::steady_clock::time_point opened_ = std::chrono::steady_clock::now();
     double grid_build_ms_ = 0;
     int force_grid_n_[3] = {0, 0, 0};
     mafm_ctx* ctx_;

@@ -404,14 +404,14 @@ __device__ inline void pair64(const SiteF64& st, double tx, double ty, double tz
     fz = fs * dz;
 }
 
-#ifndef MAFM_F64_BLOCK
-#define MAFM_F64_BLOCK 4
-#endif
-template <int SEG, bool SC, bool E>
+// B: sites per register block (loaded once, evaluated against every image,
+// B independent dependency chains).  The FIRE + midpoint kernels, built for
+// three waves per SIMD, use 2: with 4 the block's FP64 temporaries push ~45
+// values of the minimiser state into scratch, rewritten every loop trip.
+template <int SEG, bool SC, bool E, int B>
 __device__ inline void seg_sum64(cptr<DevMol> M, const Sites<SC, SiteF64>& s, int n, int gl, int S, double px,
                                  double py, double pz, double acc[4]) {
     const int n_img = M->n_images;
-    constexpr int B = MAFM_F64_BLOCK;
     const int full = (n / (B * S)) * (B * S);
     for (int t0 = 0; t0 < full; t0 += B * S) {
         SiteF64 st[B];
@@ -440,22 +440,22 @@ __device__ inline void seg_sum64(cptr<DevMol> M, const Sites<SC, SiteF64>& s, in
     }
 }
 
-template <bool SC, bool E>
+template <bool SC, bool E, int B>
 __device__ void site_sums64(cptr<DevMol> M, const Sites<SC, SiteF64> s0, double px, double py, double pz, int gl,
                             int S, double out[4]) {
     double acc[4] = {0, 0, 0, 0};
     Sites<SC, SiteF64> s = s0;
     int n = M->n_seg[0];
-    seg_sum64<0, SC, E>(M, s, n, gl, S, px, py, pz, acc);
+    seg_sum64<0, SC, E, B>(M, s, n, gl, S, px, py, pz, acc);
     s.advance(n);
     n = M->n_seg[1];
-    seg_sum64<1, SC, E>(M, s, n, gl, S, px, py, pz, acc);
+    seg_sum64<1, SC, E, B>(M, s, n, gl, S, px, py, pz, acc);
     s.advance(n);
     n = M->n_seg[2];
-    if (n) seg_sum64<2, SC, E>(M, s, n, gl, S, px, py, pz, acc);
+    if (n) seg_sum64<2, SC, E, B>(M, s, n, gl, S, px, py, pz, acc);
     s.advance(n);
     n = M->n_seg[3];
-    if (n) seg_sum64<3, SC, E>(M, s, n, gl, S, px, py, pz, acc);
+    if (n) seg_sum64<3, SC, E, B>(M, s, n, gl, S, px, py, pz, acc);
     out[0] = acc[0];
     out[1] = acc[1];
     out[2] = acc[2];
@@ -687,6 +687,7 @@ __device__ int next_rank(const ScanArgs& a, int rk) {
 template <int G, bool F32, int COMP, bool SC, bool MID>
 __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanArgs& a) {
     static_assert(!SC || G == 1, "scalar site streaming serves one lane per pixel");
+    constexpr int B64 = MID ? 2 : 4;   // FP64 sites per register block
     const int lane = threadIdx.x & 63;
     const bool shared_lds = !SC && blockDim.x > 64;
     const int gl = lane % G;
@@ -850,8 +851,8 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
                     } else {
                         const SiteF64* w = SC ? a.segf64 + M->seg_off : reinterpret_cast<const SiteF64*>(lds_raw);
                         Sites<false, SiteF64> src64{w, w};
-                        if (need_e) site_sums64<false, true>(M, src64, qx, qy, qz, sub, Ge, part);
-                        else site_sums64<false, false>(M, src64, qx, qy, qz, sub, Ge, part);
+                        if (need_e) site_sums64<false, true, B64>(M, src64, qx, qy, qz, sub, Ge, part);
+                        else site_sums64<false, false, B64>(M, src64, qx, qy, qz, sub, Ge, part);
                     }
                 }
                 // the energy (q = 3) only on trips that use it
@@ -871,8 +872,8 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
                     else site_sums32<SC, COMP, false>(M, src, Px, Py, Pz, gl, G, r);
                 } else {
                     Sites<SC, SiteF64> src{a.segf64 + M->seg_off, reinterpret_cast<const SiteF64*>(lds_raw)};
-                    if (need_e) site_sums64<SC, true>(M, src, Px, Py, Pz, gl, G, r);
-                    else site_sums64<SC, false>(M, src, Px, Py, Pz, gl, G, r);
+                    if (need_e) site_sums64<SC, true, B64>(M, src, Px, Py, Pz, gl, G, r);
+                    else site_sums64<SC, false, B64>(M, src, Px, Py, Pz, gl, G, r);
                 }
                 if (G > 1) {
 #pragma unroll
