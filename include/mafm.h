@@ -29,6 +29,11 @@
  *   mafm_potential_grid    ElectrostaticPotentialInteraction's constructor
  *                          (interactions.cpp:53-234)
  *   mafm_fft3d             fft_data_grid / ffti_data_grid (fft.hpp:25, 34)
+ *   mafm_scan_flexible     Simulation::run with flexible on (simulation.cpp:67-143):
+ *                          the whole interaction list of buildInteractions
+ *                          (simulation.cpp:201-216, 503-693) relaxed per
+ *                          position; mafm_flex_state keeps what
+ *                          System::makeXYZFile writes (system.cpp:54-65)
  *   mafm_adjoint           no counterpart (gradients of an image loss)
  *   mafm_deltaf_weights /  the frequency-shift post-processing of
  *   mafm_delta_f           simple-vafm-image.py:76-204 (Giessibl weighting of
@@ -231,6 +236,51 @@ typedef struct {
     double* d_pos;            /* (n_sites, 3) dL/d site position */
 } mafm_adjoint_out;
 
+/* ---- flexible surface (flexible on) ------------------------------------ */
+/* With flexible on every free surface atom relaxes with the tip: the
+ * interaction list of Simulation::buildInteractions (simulation.cpp:201-216)
+ * then holds, in this order, the tip-surface terms (simulation.cpp:297-318),
+ * the tip-dummy terms (:503-515), the bonds, angles, dihedrals and
+ * non-bonded pairs of buildSurfaceSurfaceInteractions (:557-693) and the
+ * substrate wall and xy tethers of buildSubstrateInteractions (:517-531).
+ * mafm_flex_term is one Interaction of that list; the forces on each atom
+ * are summed in list order, as Interaction::eval adds them.  Atom 0 is the
+ * dummy, 1 the tip. */
+enum {
+    MAFM_FX_LJ = 0,            /* LJInteraction: atom[0..1], p = es6, es12; shift added to atom[1] */
+    MAFM_FX_MORSE = 1,         /* MorseInteraction: atom[0..1], p = De, a, re; shift */
+    MAFM_FX_COULOMB = 2,       /* CoulombInteraction: atom[0..1], p = qq */
+    MAFM_FX_TIP_HARMONIC = 3,  /* TipHarmonicInteraction (lateral): atom[0..1], p = k, r0 */
+    MAFM_FX_BOND = 4,          /* HarmonicInteraction: atom[0..1], p = k, r0 */
+    MAFM_FX_ANGLE = 5,         /* HarmonicAngleInteraction: atom = shared, i1, i2, p = k, theta0 */
+    MAFM_FX_DIHEDRAL = 6,      /* HarmonicDihedralInteraction: atom[0..3], p = k, sigma0 */
+    MAFM_FX_SUBSTRATE = 7,     /* SubstrateInteraction: atom[0], p = multiplier, ulj, ushift, z0, rc, sigma
+                                  (the constructor's derived constants, interactions.hpp:222-229) */
+    MAFM_FX_XY_HARMONIC = 8    /* XYHarmonicInteraction: atom[0], p = k, x0, y0 */
+};
+
+typedef struct {
+    int32_t kind;              /* MAFM_FX_* */
+    int32_t atom[4];           /* atom indices; unused slots ignored */
+    int32_t reserved;
+    double p[6];
+    double shift[3];           /* LJ / Morse periodic image shift */
+} mafm_flex_term;
+
+typedef struct {
+    int32_t n_atoms;           /* dummy + tip + surface atoms */
+    int32_t n_terms;
+    const double* positions;   /* n_atoms x 3 after centring; rows 0 and 1 are set per column */
+    const int32_t* fixed;      /* n_atoms: 1 = never moves (fixed[0] must be 1, fixed[1] 0) */
+    const mafm_flex_term* terms;   /* the list, tip-surface terms (atom[0] = 1) first */
+    int32_t tip_pbc;           /* System::setDummyXY wraps the dummy into the cell */
+    int32_t state_column;      /* column i*ny + j whose relaxed atoms are kept at every height
+                                  (makeXYZFile, simulation.cpp:114-116), or -1 */
+    double tip_dummy_d;
+    double cell[9];            /* row-major, columns = cell vectors */
+    double offset[3];
+} mafm_flex_desc;
+
 /* ---- entry points ------------------------------------------------------ */
 mafm_status mafm_create(int device, mafm_ctx** out);
 /* Frees the context and its device memory. Simulations opened on it
@@ -280,6 +330,19 @@ mafm_status mafm_records(mafm_ctx* ctx, size_t first, size_t count, mafm_record*
  * counterpart: a diagnostic of the FP32 arithmetic. */
 mafm_status mafm_margins(mafm_ctx* ctx, size_t first, size_t count, float* out);
 mafm_status mafm_eval_tip_surface(mafm_ctx* ctx, int system, size_t n, const double* pos, double* fe);
+/* The scan with a flexible surface: the pixel loop of Simulation::run
+ * (simulation.cpp:78-140) over one System copy per column, whose surface
+ * atoms stay relaxed from height to height, with FIREMinimisation /
+ * SDMinimisation and the integrators over the whole atom list
+ * (minimiser.cpp:39-129, integrators.cpp:8-111).  FP64 in the reference's
+ * operation order (desc->precision is not used).  The records replace the
+ * context's, in mafm_records' order for one system; the systems of
+ * mafm_set_systems are kept. */
+mafm_status mafm_scan_flexible(mafm_ctx* ctx, const mafm_flex_desc* sys, const mafm_scan_desc* desc,
+                               mafm_scan_stats* stats);
+/* Relaxed atom positions of the state column of the last flexible scan,
+ * n_atoms x 3 doubles per height for heights [first, first + count). */
+mafm_status mafm_flex_state(mafm_ctx* ctx, int32_t first, int32_t count, double* out);
 mafm_status mafm_adjoint(mafm_ctx* ctx, int system, const mafm_adjoint_desc* desc, mafm_adjoint_out* out);
 
 /* ---- frequency shift (simple-vafm-image.py:76-204) ----

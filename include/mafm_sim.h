@@ -16,6 +16,8 @@
  *   mafm_sim_run           Simulation::run (simulation.cpp:67-143)
  *   mafm_batch_run         many jobs in one launch (no reference counterpart:
  *                          the reference runs one molecule per process)
+ *   mafm_sim_flex_system   the flexible interaction list of buildInteractions
+ *                          (simulation.cpp:201-216, 503-693)
  *   mafm_sim_write_output  openUniverse / writeOutput / closeUniverse
  *                          (mechafm.cpp:59-101, simulation.cpp:145-199) and
  *                          the statistics block of finalize (mechafm.cpp:103-154)
@@ -40,6 +42,8 @@ typedef struct {
     double tip_dummy_d;
     double offset[3];             /* System offset after centring */
     double grid_build_ms;         /* device time spent building grids / potential */
+    int32_t flexible;             /* flexible on: the job scans with mafm_scan_flexible */
+    int32_t n_terms;              /* flexible on: terms of the interaction list */
 } mafm_sim_info;
 
 typedef struct {
@@ -51,11 +55,19 @@ typedef struct {
     uint32_t options;             /* mafm_scan_desc::options */
 } mafm_run_opts;
 
+/* ctx may be NULL: the job is then parsed and, where its set-up needs no
+ * device (no rigidgrid, no external potential), initialised on the host
+ * only -- enough for mafm_sim_system / mafm_sim_flex_system; running it
+ * needs a context. */
 mafm_status mafm_sim_open(const char* input_file, const char* output_dir, mafm_ctx* ctx, mafm_sim** out);
 mafm_status mafm_sim_initialize(mafm_sim* sim);
 mafm_status mafm_sim_info_get(mafm_sim* sim, mafm_sim_info* info);
-/* the lowered system (valid until the sim is closed) */
+/* the lowered system (valid until the sim is closed); rigid jobs */
 mafm_status mafm_sim_system(mafm_sim* sim, mafm_system_desc* desc);
+/* flexible jobs: the whole interaction list as device terms, with the
+ * surface after centring and the fixed flags of the xyz file's sixth column
+ * (valid until the sim is closed) */
+mafm_status mafm_sim_flex_system(mafm_sim* sim, mafm_flex_desc* desc);
 /* the scan descriptor this job's options give (for callers that drive
  * mafm_scan themselves, e.g. with modified site tables) */
 mafm_status mafm_sim_scan_desc(mafm_sim* sim, const mafm_run_opts* opts, mafm_scan_desc* desc);
@@ -65,6 +77,11 @@ mafm_status mafm_sim_records(mafm_sim* sim, size_t first, size_t count, mafm_rec
 /* per-height files scan-%06.3f.dat[.gz] with the reference's line format
  * (simulation.cpp:145-199); threads <= 0 uses every hardware thread */
 mafm_status mafm_sim_write_output(mafm_sim* sim, int threads);
+/* console lines of the last mafm_sim_write_output: with flexible on, one
+ * "+- Wrote state_X-Y-Z.xyz" per height of the centre column, whose relaxed
+ * atoms it writes as System::makeXYZFile does (simulation.cpp:114-116,
+ * system.cpp:54-65); same buffer convention as mafm_sim_settings */
+mafm_status mafm_sim_log(mafm_sim* sim, char* buf, size_t size, size_t* needed);
 /* frequency-shift images of the last run (simple-vafm-image.py:76-204) as
  * deltaf-%06.3f.dat files in the output folder; desc NULL = the script's
  * amplitude, cantilever and frequency */

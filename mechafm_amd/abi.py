@@ -106,7 +106,20 @@ class SimInfo(ctypes.Structure):
                 ("use_external_potential", ctypes.c_int32), ("vdw_pbc", ctypes.c_int32),
                 ("coulomb", ctypes.c_int32), ("force_grid_n", ctypes.c_int32 * 3),
                 ("tip_dummy_d", ctypes.c_double), ("offset", ctypes.c_double * 3),
-                ("grid_build_ms", ctypes.c_double)]
+                ("grid_build_ms", ctypes.c_double), ("flexible", ctypes.c_int32), ("n_terms", ctypes.c_int32)]
+
+
+# flexible on: one term of the interaction list (mafm_flex_term)
+FX_LJ, FX_MORSE, FX_COULOMB, FX_TIP_HARMONIC, FX_BOND, FX_ANGLE, FX_DIHEDRAL, FX_SUBSTRATE, FX_XY_HARMONIC = range(9)
+FLEX_TERM_DTYPE = np.dtype([("kind", "<i4"), ("atom", "<i4", (4,)), ("reserved", "<i4"), ("p", "<f8", (6,)),
+                            ("shift", "<f8", (3,))])
+
+
+class FlexDesc(ctypes.Structure):
+    _fields_ = [("n_atoms", ctypes.c_int32), ("n_terms", ctypes.c_int32),
+                ("positions", ctypes.POINTER(ctypes.c_double)), ("fixed", ctypes.POINTER(ctypes.c_int32)),
+                ("terms", ctypes.c_void_p), ("tip_pbc", ctypes.c_int32), ("state_column", ctypes.c_int32),
+                ("tip_dummy_d", ctypes.c_double), ("cell", ctypes.c_double * 9), ("offset", ctypes.c_double * 3)]
 
 
 class RunOpts(ctypes.Structure):
@@ -143,6 +156,8 @@ SIGNATURES = {
     "mafm_scan_noalloc": (ctypes.c_int, [_vp, _P(ScanDesc), _vp, ctypes.c_size_t, _P(ScanStats)]),
     "mafm_records": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _vp]),
     "mafm_margins": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _P(ctypes.c_float)]),
+    "mafm_scan_flexible": (ctypes.c_int, [_vp, _P(FlexDesc), _P(ScanDesc), _P(ScanStats)]),
+    "mafm_flex_state": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, _P(ctypes.c_double)]),
     "mafm_eval_tip_surface": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_size_t, _P(ctypes.c_double),
                                              _P(ctypes.c_double)]),
     "mafm_adjoint": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
@@ -153,6 +168,8 @@ SIGNATURES = {
     "mafm_sim_initialize": (ctypes.c_int, [_vp]),
     "mafm_sim_info_get": (ctypes.c_int, [_vp, _P(SimInfo)]),
     "mafm_sim_system": (ctypes.c_int, [_vp, _P(SystemDesc)]),
+    "mafm_sim_flex_system": (ctypes.c_int, [_vp, _P(FlexDesc)]),
+    "mafm_sim_log": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t, _P(ctypes.c_size_t)]),
     "mafm_sim_scan_desc": (ctypes.c_int, [_vp, _P(RunOpts), _P(ScanDesc)]),
     "mafm_sim_run": (ctypes.c_int, [_vp, _P(RunOpts), _P(ScanStats)]),
     "mafm_batch_run": (ctypes.c_int, [ctypes.c_int, _P(_vp), _P(RunOpts), _P(ScanStats)]),
@@ -312,6 +329,33 @@ class Context:
         self.check(lib().mafm_delta_f(self.h, system, ctypes.byref(d), _dp(df), _dp(zl)))
         return df, zl
 
+    def scan_flexible(self, n_atoms: int, positions: np.ndarray, fixed: np.ndarray, terms: np.ndarray,
+                      desc: ScanDesc, tip_pbc: bool = False, state_column: int = -1, tip_dummy_d: float = 0.0,
+                      cell=None, offset=None) -> dict:
+        """The flexible scan over an interaction list (mafm_scan_flexible)."""
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        fx = np.ascontiguousarray(fixed, dtype=np.int32)
+        tm = np.ascontiguousarray(terms, dtype=FLEX_TERM_DTYPE)
+        d = FlexDesc()
+        d.n_atoms, d.n_terms = n_atoms, len(tm)
+        d.positions = _dp(pos)
+        d.fixed = fx.ctypes.data_as(_P(ctypes.c_int32))
+        d.terms = tm.ctypes.data_as(ctypes.c_void_p)
+        d.tip_pbc, d.state_column, d.tip_dummy_d = int(tip_pbc), state_column, tip_dummy_d
+        for a, v in enumerate(np.asarray(cell if cell is not None else np.zeros(9), dtype=np.float64).ravel()):
+            d.cell[a] = v
+        for a, v in enumerate(np.asarray(offset if offset is not None else np.zeros(3), dtype=np.float64)):
+            d.offset[a] = v
+        st = ScanStats()
+        self.check(lib().mafm_scan_flexible(self.h, ctypes.byref(d), ctypes.byref(desc), ctypes.byref(st)))
+        return st.as_dict()
+
+    def flex_state(self, n_atoms: int, first: int, count: int) -> np.ndarray:
+        """(count, n_atoms, 3) relaxed atoms of the state column (mafm_flex_state)."""
+        out = np.zeros((count, n_atoms, 3))
+        self.check(lib().mafm_flex_state(self.h, first, count, _dp(out)))
+        return out
+
     def eval_tip_surface(self, system: int, pos: np.ndarray) -> np.ndarray:
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         out = np.zeros((len(pos), 4))
@@ -322,10 +366,13 @@ class Context:
 class Simulation:
     """One MechAFM job (mafm_sim_*): input files -> device scan -> records / files."""
 
-    def __init__(self, input_path: str, ctx: Context, outdir: Optional[str] = None, initialize: bool = True):
+    def __init__(self, input_path: str, ctx: Optional[Context], outdir: Optional[str] = None,
+                 initialize: bool = True):
+        """ctx None: a host-only job (parse + interaction list, no device)."""
         self.ctx = ctx
         self.h = ctypes.c_void_p()
-        st = lib().mafm_sim_open(input_path.encode(), (outdir or "").encode(), ctx.h, ctypes.byref(self.h))
+        st = lib().mafm_sim_open(input_path.encode(), (outdir or "").encode(), ctx.h if ctx else None,
+                                 ctypes.byref(self.h))
         self.check(st)
         if initialize:
             self.check(lib().mafm_sim_initialize(self.h))
@@ -358,6 +405,27 @@ class Simulation:
         d = SystemDesc()
         self.check(lib().mafm_sim_system(self.h, ctypes.byref(d)))
         return d
+
+    def flex_system(self) -> dict:
+        """Flexible jobs: the interaction list and the surface
+        (mafm_sim_flex_system) as numpy copies."""
+        d = FlexDesc()
+        self.check(lib().mafm_sim_flex_system(self.h, ctypes.byref(d)))
+        n, nt = d.n_atoms, d.n_terms
+        terms = np.ctypeslib.as_array(ctypes.cast(d.terms, _P(ctypes.c_uint8)), shape=(nt * FLEX_TERM_DTYPE.itemsize,))
+        return {"n_atoms": n, "terms": terms.view(FLEX_TERM_DTYPE).copy(),
+                "positions": np.ctypeslib.as_array(d.positions, shape=(n * 3,)).reshape(n, 3).copy(),
+                "fixed": np.ctypeslib.as_array(d.fixed, shape=(n,)).copy(), "tip_pbc": d.tip_pbc,
+                "state_column": d.state_column, "tip_dummy_d": d.tip_dummy_d, "cell": np.array(d.cell[:]),
+                "offset": np.array(d.offset[:])}
+
+    def log(self) -> str:
+        """Console lines of the last write_output (mafm_sim_log)."""
+        need = ctypes.c_size_t()
+        self.check(lib().mafm_sim_log(self.h, None, 0, ctypes.byref(need)))
+        buf = ctypes.create_string_buffer(need.value)
+        self.check(lib().mafm_sim_log(self.h, buf, need.value, None))
+        return buf.value.decode()
 
     def sites(self) -> List[Site]:
         d = self.system()

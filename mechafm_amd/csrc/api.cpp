@@ -16,6 +16,7 @@
 #include "../../include/mafm.h"
 #include "ctx_ref.hpp"
 #include "device_common.hpp"
+#include "flex.hpp"
 
 namespace mafm {
 hipError_t launch_scan_plain(const ScanArgs& a, hipStream_t st);
@@ -133,6 +134,12 @@ struct mafm_ctx {
     DevBuf<int64_t> d_counters;
     DevBuf<double> d_tmp;
     DevBuf<float4> d_margins;      // per-position decision margins (MAFM_SCAN_MARGINS)
+    // flexible scan: term list, index tables, initial positions, kept states, per-block work
+    DevBuf<mafm_flex_term> d_fx_terms;
+    DevBuf<int32_t> d_fx_ints;
+    DevBuf<double> d_fx_pos, d_fx_state, d_fx_work;
+    bool last_flex = false;        // the records come from mafm_scan_flexible
+    int fx_state_atoms = 0, fx_state_heights = 0;   // kept state of the last flexible scan (0: none)
     bool have_margins = false;
     uint64_t scan_generation = 0;  // bumped by every scan (the records are replaced)
     void* scratch = nullptr;       // device address of the fixed workspace (mafm_scan_noalloc)
@@ -507,6 +514,11 @@ static void destroy_now(mafm_ctx* c) {
     c->d_counters.release();
     c->d_tmp.release();
     c->d_margins.release();
+    c->d_fx_terms.release();
+    c->d_fx_ints.release();
+    c->d_fx_pos.release();
+    c->d_fx_state.release();
+    c->d_fx_work.release();
     for (auto& kv : c->twiddles) kv.second.release();
     c->d_flag.release();
     (void)hipEventDestroy(c->ev0);
@@ -676,18 +688,26 @@ mafm_status mafm_set_systems(mafm_ctx* c, int n, const mafm_system_desc* sys) {
 // context allocates: out = nullptr, ws = nullptr) and mafm_scan_noalloc
 // (records into the caller's device buffer out of `cap` records, state in
 // the fixed workspace ws).
-static mafm_status scan_impl(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* stats, DevRecord* out, size_t cap,
-                             ScanScratch* ws) {
-    if (!c || !d) return MAFM_ERR_INVALID;
-    if (!c->have_systems) return fail(c, MAFM_ERR_STATE, "mafm_scan before mafm_set_systems");
+// The scan geometry and minimiser settings every scan entry point checks.
+static mafm_status check_scan_desc(mafm_ctx* c, const mafm_scan_desc* d) {
     if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return fail(c, MAFM_ERR_INVALID, "empty scan grid");
     if (d->minimiser != MAFM_MIN_SD && d->minimiser != MAFM_MIN_FIRE) return fail(c, MAFM_ERR_INVALID, "minimiser");
     if (d->integrator < 0 || d->integrator > 2) return fail(c, MAFM_ERR_INVALID, "integrator");
     if (d->minterm < 0 || d->minterm > 2) return fail(c, MAFM_ERR_INVALID, "minterm");
     if (d->precision < 0 || d->precision > 2) return fail(c, MAFM_ERR_INVALID, "precision");
     if ((int64_t)d->nx * d->ny > INT32_MAX) return fail(c, MAFM_ERR_INVALID, "more than 2^31 pixel columns");
+    if ((int64_t)d->nx * d->ny * d->nz > ((int64_t)1 << 40)) return fail(c, MAFM_ERR_INVALID, "scan grid too large");
     if (!std::isfinite(d->dz) || d->dz == 0 || !std::isfinite(d->dt))
         return fail(c, MAFM_ERR_INVALID, "dz must be finite and non-zero, dt finite");
+    return MAFM_OK;
+}
+
+static mafm_status scan_impl(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* stats, DevRecord* out, size_t cap,
+                             ScanScratch* ws) {
+    if (!c || !d) return MAFM_ERR_INVALID;
+    if (!c->have_systems) return fail(c, MAFM_ERR_STATE, "mafm_scan before mafm_set_systems");
+    mafm_status sd = check_scan_desc(c, d);
+    if (sd != MAFM_OK) return sd;
     if (d->precision != MAFM_PREC_PLAIN64 && d->lanes_per_pixel > 0) {
         const int G = d->lanes_per_pixel;
         if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16 && G != 32 && G != 64)
@@ -715,6 +735,8 @@ static mafm_status scan_impl(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_sta
     // the records of an earlier scan are no longer described by the context
     ++c->scan_generation;
     c->have_margins = false;
+    c->last_flex = false;
+    c->fx_state_atoms = c->fx_state_heights = 0;
     if (out) {
         c->have_scan = false;
         c->n_records = 0;
@@ -814,6 +836,197 @@ mafm_status mafm_scan_noalloc(mafm_ctx* c, const mafm_scan_desc* d, mafm_record*
                      static_cast<ScanScratch*>(c->scratch));
 }
 
+// Number of atoms a flexible term acts on, and the order in which the
+// reference's eval updates them (interactions.cpp:15-51, 252-362): an angle
+// updates atom_i1, atom_i2, then the shared atom.
+static int flex_arity(int kind) {
+    switch (kind) {
+        case MAFM_FX_LJ: case MAFM_FX_MORSE: case MAFM_FX_COULOMB: case MAFM_FX_TIP_HARMONIC: case MAFM_FX_BOND:
+            return 2;
+        case MAFM_FX_ANGLE: return 3;
+        case MAFM_FX_DIHEDRAL: return 4;
+        case MAFM_FX_SUBSTRATE: case MAFM_FX_XY_HARMONIC: return 1;
+        default: return 0;
+    }
+}
+
+static const int* flex_update_order(int kind) {
+    static const int pair[4] = {0, 1, 2, 3}, angle[3] = {1, 2, 0};
+    return kind == MAFM_FX_ANGLE ? angle : pair;
+}
+
+static bool flex_tip_surface(const mafm_flex_term& t) {   // isTipSurface (interactions.hpp:79-137)
+    return (t.kind == MAFM_FX_LJ || t.kind == MAFM_FX_MORSE || t.kind == MAFM_FX_COULOMB) && t.atom[0] == 1;
+}
+
+mafm_status mafm_scan_flexible(mafm_ctx* c, const mafm_flex_desc* s, const mafm_scan_desc* d,
+                               mafm_scan_stats* stats) {
+    if (!c || !s || !d) return MAFM_ERR_INVALID;
+    mafm_status sd = check_scan_desc(c, d);
+    if (sd != MAFM_OK) return sd;
+    const int n = s->n_atoms;
+    if (n < 2 || n > (1 << 20)) return fail(c, MAFM_ERR_INVALID, "n_atoms is %d", n);
+    if (!s->positions || !s->fixed || s->n_terms < 0 || (s->n_terms > 0 && !s->terms))
+        return fail(c, MAFM_ERR_INVALID, "flexible system: missing positions, fixed flags or terms");
+    if (s->fixed[0] != 1 || s->fixed[1] == 1)
+        return fail(c, MAFM_ERR_INVALID, "the dummy (atom 0) must be fixed and the tip (atom 1) free");
+    for (int i = 0; i < 3 * n; ++i)
+        if (i >= 6 && !std::isfinite(s->positions[i]))
+            return fail(c, MAFM_ERR_INVALID, "atom %d has a non-finite position", i / 3);
+    const long n_cols = (long)d->nx * d->ny;
+    if (s->state_column < -1 || s->state_column >= n_cols)
+        return fail(c, MAFM_ERR_INVALID, "state column %d outside the %ld columns", s->state_column, n_cols);
+    // tip-surface prefix, arity, indices
+    int n_ts = 0;
+    while (n_ts < s->n_terms && flex_tip_surface(s->terms[n_ts])) ++n_ts;
+    for (int t = 0; t < s->n_terms; ++t) {
+        const mafm_flex_term& T = s->terms[t];
+        const int ar = flex_arity(T.kind);
+        if (ar == 0) return fail(c, MAFM_ERR_INVALID, "term %d: unknown kind %d", t, T.kind);
+        for (int r = 0; r < ar; ++r)
+            if (T.atom[r] < 0 || T.atom[r] >= n)
+                return fail(c, MAFM_ERR_INVALID, "term %d: atom %d outside 0..%d", t, T.atom[r], n - 1);
+        if (t >= n_ts && flex_tip_surface(T))
+            return fail(c, MAFM_ERR_INVALID, "term %d is a tip-surface term after the first non-tip-surface term", t);
+    }
+    // terms acting on a free atom, their result slots, and each free atom's slots in list order
+    std::vector<int32_t> eval_term, eval_slot, csr_off(n + 1, 0), csr;
+    int n_slots = 0, ts_slots = 0;
+    std::vector<std::vector<int32_t>> per_atom(n);
+    for (int t = 0; t < s->n_terms; ++t) {
+        const mafm_flex_term& T = s->terms[t];
+        const int ar = flex_arity(T.kind);
+        bool any_free = false;
+        for (int r = 0; r < ar; ++r) any_free |= s->fixed[T.atom[r]] != 1;
+        if (!any_free) continue;
+        eval_term.push_back(t);
+        eval_slot.push_back(n_slots);
+        const int* order = flex_update_order(T.kind);
+        for (int q = 0; q < ar; ++q) {
+            const int r = order[q];
+            if (s->fixed[T.atom[r]] != 1) per_atom[T.atom[r]].push_back(n_slots + r);
+        }
+        n_slots += ar;
+        if (t < n_ts) ts_slots = n_slots;
+    }
+    for (int i = 0; i < n; ++i) {
+        csr_off[i + 1] = csr_off[i] + (int32_t)per_atom[i].size();
+        csr.insert(csr.end(), per_atom[i].begin(), per_atom[i].end());
+    }
+    const int n_eval = (int)eval_term.size();
+    const bool rk4 = d->minimiser == MAFM_MIN_FIRE && d->integrator == MAFM_INT_RK4;
+    const size_t block_bytes = flex_block_doubles(n, n_slots, rk4) * sizeof(double);
+    const bool in_lds = block_bytes + 64 <= 64 * 1024;
+    int per_cu = in_lds ? (int)std::min<size_t>(16, c->lds_per_cu / block_bytes) : 8;
+    per_cu = std::max(per_cu, 1);
+    const int n_blocks = (int)std::min<long>(n_cols, (long)c->cus * per_cu);
+    const size_t n_rec = (size_t)n_cols * d->nz;
+
+    HIPCHK(c, hipSetDevice(c->device));
+    ++c->scan_generation;
+    c->have_margins = false;
+    c->have_scan = false;
+    c->n_records = 0;
+    c->last_flex = false;
+    c->fx_state_atoms = c->fx_state_heights = 0;
+    std::vector<int32_t> ints;
+    ints.reserve(2 * n_eval + (n + 1) + csr.size() + n);
+    ints.insert(ints.end(), eval_term.begin(), eval_term.end());
+    ints.insert(ints.end(), eval_slot.begin(), eval_slot.end());
+    ints.insert(ints.end(), csr_off.begin(), csr_off.end());
+    ints.insert(ints.end(), csr.begin(), csr.end());
+    ints.insert(ints.end(), s->fixed, s->fixed + n);
+    HIPCHK(c, c->d_fx_terms.reserve(std::max(s->n_terms, 1)));
+    HIPCHK(c, c->d_fx_ints.reserve(ints.size()));
+    HIPCHK(c, c->d_fx_pos.reserve(3 * (size_t)n));
+    HIPCHK(c, c->d_rec.reserve(n_rec));
+    HIPCHK(c, c->d_next.reserve(1));
+    if (s->state_column >= 0) HIPCHK(c, c->d_fx_state.reserve((size_t)d->nz * n * 3));
+    if (!in_lds) HIPCHK(c, c->d_fx_work.reserve((size_t)n_blocks * flex_block_doubles(n, n_slots, rk4)));
+    if (s->n_terms > 0)
+        HIPCHK(c, hipMemcpyAsync(c->d_fx_terms.p, s->terms, s->n_terms * sizeof(mafm_flex_term),
+                                 hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_fx_ints.p, ints.data(), ints.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_fx_pos.p, s->positions, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_next.p, 0, sizeof(int32_t), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_counters.p, 0, 8 * sizeof(int64_t), c->stream));
+
+    FlexArgs a;
+    memset(&a, 0, sizeof a);
+    a.n_atoms = n; a.n_eval = n_eval; a.n_ts = 0;
+    for (int k = 0; k < n_eval && eval_term[k] < n_ts; ++k) a.n_ts = k + 1;
+    a.n_slots = n_slots; a.ts_slots = ts_slots;
+    a.nx = d->nx; a.ny = d->ny; a.nz = d->nz;
+    a.dx = d->dx; a.dy = d->dy; a.dz = d->dz; a.zhigh = d->zhigh;
+    a.minimiser = d->minimiser; a.integrator = d->integrator; a.minterm = d->minterm;
+    a.maxsteps = d->maxsteps; a.etol = d->etol; a.ftol = d->ftol; a.dt = d->dt;
+    a.terms = c->d_fx_terms.p;
+    a.eval_term = c->d_fx_ints.p;
+    a.eval_slot = a.eval_term + n_eval;
+    a.csr_off = a.eval_slot + n_eval;
+    a.csr = a.csr_off + (n + 1);
+    a.fixed = a.csr + csr.size();
+    a.pos0 = c->d_fx_pos.p;
+    a.tip_pbc = s->tip_pbc ? 1 : 0;
+    a.state_col = s->state_column;
+    memcpy(a.cell, s->cell, sizeof a.cell);
+    if (a.tip_pbc) inv3(a.cell, a.cell_inv);
+    memcpy(a.offset, s->offset, sizeof a.offset);
+    a.tip_dummy_d = s->tip_dummy_d;
+    a.rec = c->d_rec.p;
+    a.next_col = c->d_next.p;
+    a.state = s->state_column >= 0 ? c->d_fx_state.p : nullptr;
+    a.work = in_lds ? nullptr : c->d_fx_work.p;
+
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, launch_scan_flex(a, n_blocks, in_lds, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, launch_record_totals(c->d_rec.p, (long)n_rec, c->d_counters.p, c->stream));
+    int64_t counters[8];
+    HIPCHK(c, hipMemcpyAsync(counters, c->d_counters.p, sizeof counters, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->n_records = n_rec;
+    c->last_scan = *d;
+    c->have_scan = true;
+    c->last_flex = true;
+    if (s->state_column >= 0) {
+        c->fx_state_atoms = n;
+        c->fx_state_heights = d->nz;
+    }
+    if (stats) {
+        memset(stats, 0, sizeof *stats);
+        stats->kernel_ms = ms;
+        stats->positions = (int64_t)n_rec;
+        stats->total_steps = counters[0];
+        stats->unconverged = counters[1];
+        const int64_t evals = d->minimiser == MAFM_MIN_SD ? 1 : (d->integrator == 1 ? 2 : (d->integrator == 2 ? 4 : 1));
+        // term evaluations: every evaluated term per integrator stage, plus the tip-surface terms of each output
+        stats->pair_evals = counters[0] * evals * n_eval + (int64_t)n_rec * a.n_ts;
+        stats->lanes_per_pixel = 64;
+        stats->grid_blocks = n_blocks;
+        stats->block_threads = 64;
+    }
+    return MAFM_OK;
+}
+
+mafm_status mafm_flex_state(mafm_ctx* c, int32_t first, int32_t count, double* out) {
+    if (!c || (!out && count)) return MAFM_ERR_INVALID;
+    if (c->fx_state_heights == 0) return fail(c, MAFM_ERR_STATE, "the last scan kept no flexible state");
+    if (first < 0 || count < 0 || first + count > c->fx_state_heights)
+        return fail(c, MAFM_ERR_INVALID, "heights [%d, %d) beyond the %d of the last scan", first, first + count,
+                    c->fx_state_heights);
+    if (count == 0) return MAFM_OK;
+    const size_t per = 3 * (size_t)c->fx_state_atoms;
+    HIPCHK(c, hipMemcpyAsync(out, c->d_fx_state.p + per * first, per * count * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MAFM_OK;
+}
+
 mafm_status mafm_margins(mafm_ctx* c, size_t first, size_t count, float* out) {
     if (!c || (!out && count)) return MAFM_ERR_INVALID;
     if (!c->have_margins) return fail(c, MAFM_ERR_STATE, "the last scan did not record decision margins");
@@ -837,7 +1050,8 @@ mafm_status mafm_deltaf_weights(mafm_ctx* c, const mafm_deltaf_desc* d, int32_t 
 mafm_status mafm_delta_f(mafm_ctx* c, int system, const mafm_deltaf_desc* d, double* df, double* z_low) {
     if (!c || !d || !df) return MAFM_ERR_INVALID;
     if (!c->have_scan) return fail(c, MAFM_ERR_STATE, "frequency shift before a scan");
-    if (system < 0 || system >= (int)c->mols.size()) return fail(c, MAFM_ERR_INVALID, "system %d", system);
+    const int n_systems = c->last_flex ? 1 : (int)c->mols.size();
+    if (system < 0 || system >= n_systems) return fail(c, MAFM_ERR_INVALID, "system %d", system);
     const mafm_scan_desc& s = c->last_scan;
     mafm_deltaf_weights_out w;
     std::vector<double> zl;
@@ -848,7 +1062,7 @@ mafm_status mafm_delta_f(mafm_ctx* c, int system, const mafm_deltaf_desc* d, dou
     const size_t n = (size_t)w.n_images * s.nx * s.ny;
     DevBuf<double> out;
     HIPCHK(c, out.reserve(n));
-    const DevRecord* rec = c->d_rec.p + c->mols[system].rec_off;
+    const DevRecord* rec = c->d_rec.p + (c->last_flex ? 0 : c->mols[system].rec_off);
     HIPCHK(c, launch_delta_f(rec, w, s.nx, s.ny, s.nz, out.p, c->stream));
     HIPCHK(c, hipMemcpyAsync(df, out.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -1,0 +1,44 @@
+// Device arguments of the flexible scan (scan_flex.hip), shared with the
+// host side that fills them (api.cpp).
+#pragma once
+#include "../../include/mafm.h"
+#include "device_common.hpp"
+
+namespace mafm {
+
+// Every term is evaluated once per integrator stage into result slots, one
+// slot {fx, fy, fz, e} per atom the term acts on, holding the signed force
+// that atom receives (the reference's `forces[i] -= f` is `+= -f`, the same
+// rounding).  Each free atom then sums its slots in the interaction list's
+// order.
+struct FlexArgs {
+    int32_t n_atoms, n_eval, n_ts;    // eval entries [0, n_ts) are the tip-surface terms
+    int32_t n_slots, ts_slots;        // slots [0, ts_slots) belong to the tip-surface terms
+    int32_t nx, ny, nz;
+    double dx, dy, dz, zhigh;
+    int32_t minimiser, integrator, minterm, maxsteps;
+    double etol, ftol, dt;
+    const mafm_flex_term* terms;
+    const int32_t* eval_term;         // n_eval: terms that act on a free atom, in list order
+    const int32_t* eval_slot;         // n_eval: first result slot of each
+    const int32_t* csr_off;           // n_atoms + 1 (empty ranges for fixed atoms)
+    const int32_t* csr;               // result slots of each free atom, in list order
+    const double* pos0;               // n_atoms x 3
+    const int32_t* fixed;             // n_atoms
+    int32_t tip_pbc;
+    int32_t state_col;                // column whose positions are kept per height, -1 none
+    double cell[9], cell_inv[9], offset[3];
+    double tip_dummy_d;
+    DevRecord* rec;
+    int32_t* next_col;                // work counter, zero at launch
+    double* state;                    // nz x n_atoms x 3
+    double* work;                     // per-block state when it does not fit LDS
+};
+
+// Doubles of per-column state one block keeps (positions, integrator
+// stages, velocities, forces, term results).
+size_t flex_block_doubles(int n_atoms, int n_slots, bool rk4);
+// in_lds: the state lives in LDS (else in a.work, one slice per block).
+hipError_t launch_scan_flex(const FlexArgs& a, int n_blocks, bool in_lds, hipStream_t st);
+
+}  // namespace mafm

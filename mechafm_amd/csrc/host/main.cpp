@@ -1,7 +1,7 @@
 // mechafm: the reference's command line (mechafm.cpp:209-233),
 //   mechafm INPUT [OUTDIR] [--precision plain64|fast64|fast32] [--device N] [--deltaf]
 // parse -> initialize (interactions, force grid / external potential on the
-// device) -> one device scan -> per-height output files.
+// device) -> one device scan (rigid or flexible) -> per-height output files.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -48,7 +48,7 @@ int main(int argc, char** argv) {
     if (st != MAFM_OK) return die("no usable HIP device");
     mafm_sim* sim = nullptr;
     auto t0 = std::chrono::steady_clock::now();
-    printf("+- MechAFM rigid probe-particle scan on gfx950 (libmafm)\n");
+    printf("+- MechAFM probe-particle scan on gfx950 (libmafm)\n");
     st = mafm_sim_open(input.c_str(), outdir.c_str(), ctx, &sim);
     if (st == MAFM_OK) {
         // input warnings (stderr) and the settings echo (stdout), as the
@@ -78,6 +78,14 @@ int main(int argc, char** argv) {
     mafm_scan_stats ss{};
     st = mafm_sim_run(sim, &ro, &ss);
     if (st == MAFM_OK) st = mafm_sim_write_output(sim, 0);
+    if (st == MAFM_OK) {
+        // flexible on: the "+- Wrote state_..." lines of makeXYZFile (system.cpp:63)
+        size_t need = 0;
+        mafm_sim_log(sim, nullptr, 0, &need);
+        std::string log(need, '\0');
+        mafm_sim_log(sim, &log[0], need, nullptr);
+        fputs(log.c_str(), stdout);
+    }
     if (st == MAFM_OK && deltaf) st = mafm_sim_write_deltaf(sim, nullptr);
     auto t1 = std::chrono::steady_clock::now();
     if (st != MAFM_OK) {

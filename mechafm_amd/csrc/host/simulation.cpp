@@ -8,6 +8,8 @@
 #include <atomic>
 
 #include <algorithm>
+#include <array>
+#include <initializer_list>
 #include <cctype>
 #include <cerrno>
 #include <cstdarg>
@@ -45,6 +47,7 @@ const double BOHR_TO_ANGST = 0.52917721092;       // cube_io.hpp:17
 }
 
 void check(mafm_ctx* ctx, mafm_status s, const char* what) {
+    if (!ctx) fail(MAFM_ERR_STATE, "%s needs a device context (this simulation was opened without one)", what);
     if (s != MAFM_OK) fail(s, "%s: %s", what, mafm_last_error(ctx));
 }
 
@@ -93,6 +96,37 @@ std::vector<std::string> read_lines(const std::string& path, const char* missing
     std::string l;
     while (std::getline(f, l)) lines.push_back(l);
     return lines;
+}
+
+// Vector3 arithmetic of vectors.hpp:99-122: len() = sqrt(x^2 + y^2 + z^2)
+// summed left to right, normalized() with its 1e-10 cut-off.
+double vlen(const Vec3d& v) { return std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
+double vdot(const Vec3d& a, const Vec3d& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+Vec3d vcross(const Vec3d& a, const Vec3d& b) {
+    return Vec3d(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+Vec3d vnormalized(const Vec3d& v) {
+    const double l = vlen(v);
+    return l > 1e-10 ? Vec3d(v.x / l, v.y / l, v.z / l) : Vec3d();
+}
+
+bool same_pair(const std::string& a, const std::string& b, const std::string& c, const std::string& d) {
+    return (a == c && b == d) || (a == d && b == c);
+}
+
+mafm_flex_term flex_term(int kind, std::initializer_list<int> atoms, std::initializer_list<double> p,
+                         Vec3d shift = Vec3d()) {
+    mafm_flex_term t;
+    memset(&t, 0, sizeof t);
+    t.kind = kind;
+    int k = 0;
+    for (int a : atoms) t.atom[k++] = a;
+    k = 0;
+    for (double v : p) t.p[k++] = v;
+    t.shift[0] = shift.x;
+    t.shift[1] = shift.y;
+    t.shift[2] = shift.z;
+    return t;
 }
 
 }  // namespace
@@ -185,6 +219,63 @@ void CoulombInteraction::lower(Lowering& out) const {
     } else {
         fail(MAFM_ERR_INVALID, "surface-surface interactions need flexible mode, which is not built");
     }
+}
+
+// flexible on: each Interaction becomes one term of the device list
+void LJInteraction::lowerFlex(std::vector<mafm_flex_term>& out) const {
+    out.push_back(flex_term(MAFM_FX_LJ, {i1_, i2_}, {es6_, es12_}, shift_));
+}
+
+void MorseInteraction::lowerFlex(std::vector<mafm_flex_term>& out) const {
+    out.push_back(flex_term(MAFM_FX_MORSE, {i1_, i2_}, {de_, a_, re_}, shift_));
+}
+
+void CoulombInteraction::lowerFlex(std::vector<mafm_flex_term>& out) const {
+    out.push_back(flex_term(MAFM_FX_COULOMB, {i1_, i2_}, {qq_}));
+}
+
+void TipHarmonicInteraction::lowerFlex(std::vector<mafm_flex_term>& out) const {
+    out.push_back(flex_term(MAFM_FX_TIP_HARMONIC, {i1_, i2_}, {k_, r0_}));
+}
+
+void GridInteraction::lowerFlex(std::vector<mafm_flex_term>&) const {
+    fail(MAFM_ERR_INVALID, "Cannot use a flexible molecule with a static force grid!");
+}
+
+void ElectrostaticPotentialInteraction::lowerFlex(std::vector<mafm_flex_term>&) const {
+    fail(MAFM_ERR_INVALID, "External potential can be used only for non-flexible systems!");
+}
+
+void FlexOnlyInteraction::lower(Lowering&) const {
+    fail(MAFM_ERR_INVALID, "bonded and substrate terms exist only with flexible on");
+}
+
+void XYHarmonicInteraction::lowerFlex(std::vector<mafm_flex_term>& out) const {
+    out.push_back(flex_term(MAFM_FX_XY_HARMONIC, {i_}, {k_, p0_.x, p0_.y}));
+}
+
+SubstrateInteraction::SubstrateInteraction(int i, double eps, double sig, double rc, double lambda)
+    : i_(i), sig_(sig), rc_(rc) {   // interactions.hpp:222-229
+    multiplier_ = 2 * PI * eps * std::pow(sig / lambda, 2);
+    ulj_ = (PI / (lambda * lambda)) * 4 * eps * (std::pow(sig / rc, 12) - std::pow(sig / rc, 6));
+    ushift_ = multiplier_ * ((12. / 5) * std::pow(sig / rc, 10) - 3 * std::pow(sig / rc, 4));
+    z0_ = -sig;
+}
+
+void SubstrateInteraction::lowerFlex(std::vector<mafm_flex_term>& out) const {
+    out.push_back(flex_term(MAFM_FX_SUBSTRATE, {i_}, {multiplier_, ulj_, ushift_, z0_, rc_, sig_}));
+}
+
+void HarmonicInteraction::lowerFlex(std::vector<mafm_flex_term>& out) const {
+    out.push_back(flex_term(MAFM_FX_BOND, {i1_, i2_}, {k_, r0_}));
+}
+
+void HarmonicAngleInteraction::lowerFlex(std::vector<mafm_flex_term>& out) const {
+    out.push_back(flex_term(MAFM_FX_ANGLE, {shared_, i1_, i2_}, {k_, theta0_}));
+}
+
+void HarmonicDihedralInteraction::lowerFlex(std::vector<mafm_flex_term>& out) const {
+    out.push_back(flex_term(MAFM_FX_DIHEDRAL, {i1_, i2_, i3_, i4_}, {k_, sigma0_}));
 }
 
 void factorImages(Lowering& low) {
@@ -447,7 +538,6 @@ void Simulation::readInputFile() {                // parse.cpp:83-405
     if (o.vdw_pbc && !o.use_external_potential &&
         (o.cell_a == Vec3d() || o.cell_b == Vec3d() || o.cell_c == Vec3d()))
         fail(MAFM_ERR_INVALID, "The unit cell vectors must be given if periodic vdW is used.");
-    if (o.flexible) fail(MAFM_ERR_INVALID, "flexible on is not supported: this build scans rigid molecules");
     if (!(o.dx > 0 && o.dy > 0 && o.dz > 0)) fail(MAFM_ERR_INVALID, "dx, dy and dz must be positive");
     // openUniverse (mechafm.cpp:47-49)
     n_points_[0] = (int)std::floor(o.area.x / o.dx) + 1;
@@ -491,7 +581,8 @@ void Simulation::readXYZFile() {                  // parse.cpp:477-585
         system.types_[atom_i] = t[0];
         system.positions_[atom_i] = Vec3d(to_d(t[1]), to_d(t[2]), to_d(t[3]));
         system.charges_[atom_i] = ncols >= 5 ? to_d(t[4]) : 0.0;
-        system.fixed_[atom_i] = 1;              // rigid: every surface atom fixed (parse.cpp:563)
+        // the sixth column when flexible, else every surface atom fixed (parse.cpp:548-563)
+        system.fixed_[atom_i] = o.flexible ? (ncols == 6 ? std::atoi(t[5].c_str()) : 0) : 1;
         ++atom_i;
     }
     if (atom_i != system.n_atoms_) fail(MAFM_ERR_INVALID, "fewer atoms in %s than declared", o.xyzfile.c_str());
@@ -614,6 +705,56 @@ void Simulation::readParameterFile() {            // parse.cpp:588-697
         }
     }
     if (!p.harm_set) fail(MAFM_ERR_INVALID, "No harmonic spring parameters found in parameter file!");
+    if (o.flexible) readFlexibleParameters(o.paramfile);
+}
+
+void Simulation::readFlexibleParameters(const std::string& path) {   // parse.cpp:700-782
+    InteractionParameters& p = interaction_parameters_;
+    bool bcheck = false, acheck = false, dcheck = false, scheck = false;
+    for (const std::string& line : read_lines(path, "No parameter file %s found!")) {
+        if (skip_line(line)) continue;
+        std::vector<std::string> t = split(line);
+        auto num = [&](size_t k, double& dst) {
+            if (t.size() > k) dst = to_d(t[k]);
+        };
+        if (t[0] == "bond") {
+            if (bcheck) warnings_.push_back("Parameters for harmonic bond defined multiple times!");
+            num(1, p.bond_k);
+            bcheck = true;
+        } else if (t[0] == "angle") {
+            if (acheck) warnings_.push_back("Parameters for harmonic angle defined multiple times!");
+            num(1, p.angle_k);
+            acheck = true;
+        } else if (t[0] == "dihedral") {
+            if (dcheck) warnings_.push_back("Parameters for harmonic dihedral defined multiple times!");
+            num(1, p.dihedral_k);
+            dcheck = true;
+        } else if (t[0] == "substrate") {
+            if (scheck) warnings_.push_back("Parameters for substrate support defined multiple times!");
+            num(1, p.substrate_eps);
+            num(2, p.substrate_sig);
+            num(3, p.substrate_lambda);
+            num(4, p.substrate_rc);
+            num(5, p.substrate_k);
+            scheck = true;
+        } else if (t[0] == "topobond") {
+            if (t.size() < 4) fail(MAFM_ERR_INVALID, "topobond needs atom1 atom2 and the bond length");
+            PossibleBond pb;
+            pb.a = t[1];
+            pb.b = t[2];
+            pb.r0 = to_d(t[3]);
+            for (const PossibleBond& q : p.possible_bonds)
+                if (same_pair(q.a, q.b, pb.a, pb.b)) {
+                    warnings_.push_back("The topobond for " + pb.a + " and " + pb.b + " is defined at least twice!");
+                    break;
+                }
+            p.possible_bonds.push_back(pb);
+        }
+    }
+    if (!bcheck) fail(MAFM_ERR_INVALID, "No harmonic bond parameters found in parameter file!");
+    if (!acheck) fail(MAFM_ERR_INVALID, "No harmonic angle parameters found in parameter file!");
+    if (!dcheck) fail(MAFM_ERR_INVALID, "No harmonic dihedral parameters found in parameter file!");
+    if (!scheck) fail(MAFM_ERR_INVALID, "No harmonic substrate support parameters found in parameter file!");
 }
 
 void Simulation::calculateTipDummyDistance() {    // simulation.cpp:218-225
@@ -822,6 +963,118 @@ void Simulation::buildInteractions() {            // simulation.cpp:201-216
     if (options_.rigidgrid) buildTipGridInteractions();
     else buildTipSurfaceInteractions();
     buildTipDummyInteractions();
+    if (options_.flexible) {
+        buildSurfaceSurfaceInteractions();
+        buildSubstrateInteractions();
+    }
+}
+
+void Simulation::buildSubstrateInteractions() {   // simulation.cpp:517-531
+    const InteractionParameters& ip = interaction_parameters_;
+    for (int i = 2; i < system.n_atoms_; ++i) {
+        interactions_.emplace_back(
+            new SubstrateInteraction(i, ip.substrate_eps, ip.substrate_sig, ip.substrate_rc, ip.substrate_lambda));
+        if (system.fixed_[i] == 2) {
+            Vec2d p0;
+            p0.x = system.positions_[i].x;
+            p0.y = system.positions_[i].y;
+            interactions_.emplace_back(new XYHarmonicInteraction(i, ip.substrate_k, p0));
+        }
+    }
+}
+
+// Bonds between atoms closer than 1.1 x a listed bond length, the angles of
+// bond pairs sharing an atom, the dihedrals of angle pairs, and non-bonded
+// VDW (+ Coulomb) pairs between atoms that no chain of bonds connects, in the
+// reference's enumeration order (simulation.cpp:557-693); every equilibrium
+// value is the input geometry's own.
+void Simulation::buildSurfaceSurfaceInteractions() {
+    const int n = system.n_atoms_;
+    const std::vector<Vec3d>& P = system.positions_;
+    const InteractionParameters& ip = interaction_parameters_;
+    std::vector<std::vector<int>> adjacent(n);
+    std::vector<std::pair<int, int>> bonds;
+    for (int i = 2; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            double r0 = 0;   // the last matching topobond
+            for (const PossibleBond& pb : ip.possible_bonds)
+                if (same_pair(pb.a, pb.b, system.types_[i], system.types_[j])) r0 = pb.r0;
+            const double d = vlen(P[i] - P[j]);
+            if (d < 1.1 * r0) {
+                bonds.emplace_back(i, j);
+                adjacent[i].push_back(j);
+                adjacent[j].push_back(i);
+                interactions_.emplace_back(new HarmonicInteraction(i, j, ip.bond_k, d));
+            }
+        }
+    // getConnectedAtoms (simulation.cpp:533-555) with distance = n_atoms: every
+    // atom reachable through bonds; an atom without bonds is connected to none
+    std::vector<int> component(n, -1);
+    for (int i = 0; i < n; ++i) {
+        if (component[i] >= 0 || adjacent[i].empty()) continue;
+        std::vector<int> stack{i};
+        component[i] = i;
+        while (!stack.empty()) {
+            const int a = stack.back();
+            stack.pop_back();
+            for (int b : adjacent[a])
+                if (component[b] < 0) {
+                    component[b] = i;
+                    stack.push_back(b);
+                }
+        }
+    }
+    std::vector<std::array<int, 3>> angles;   // shared, atom1, atom2
+    for (size_t i = 0; i < bonds.size(); ++i)
+        for (size_t j = i + 1; j < bonds.size(); ++j) {
+            const std::pair<int, int> b1 = bonds[i], b2 = bonds[j];
+            int shared = -1, a1 = 0, a2 = 0;
+            if (b1.first == b2.first) {
+                shared = b1.first; a1 = b1.second; a2 = b2.second;
+            } else if (b1.first == b2.second) {
+                shared = b1.first; a1 = b1.second; a2 = b2.first;
+            } else if (b1.second == b2.first) {
+                shared = b1.second; a1 = b1.first; a2 = b2.second;
+            } else if (b1.second == b2.second) {
+                shared = b1.second; a1 = b1.first; a2 = b2.first;
+            }
+            if (shared == -1) continue;
+            angles.push_back({shared, a1, a2});
+            const double cos_t = vdot(vnormalized(P[a1] - P[shared]), vnormalized(P[a2] - P[shared]));
+            interactions_.emplace_back(new HarmonicAngleInteraction(shared, a1, a2, ip.angle_k, std::acos(cos_t)));
+        }
+    std::vector<char> improper(n, 0);   // improper centres already used
+    for (size_t i = 0; i < angles.size(); ++i)
+        for (size_t j = i + 1; j < angles.size(); ++j) {
+            const std::array<int, 3>& A = angles[i];
+            const std::array<int, 3>& B = angles[j];
+            int a1 = -1, a2 = -1, a3 = -1, a4 = -1;
+            if (A[0] == B[0] && !improper[A[0]]) {
+                improper[A[0]] = 1;
+                a1 = A[0]; a2 = A[1]; a3 = A[2];
+                a4 = (B[1] == A[1] || B[1] == A[2]) ? B[2] : B[1];
+            } else if (A[1] == B[0] && B[1] == A[0]) {
+                a1 = A[2]; a2 = A[0]; a3 = B[0]; a4 = B[2];
+            } else if (A[2] == B[0] && B[1] == A[0]) {
+                a1 = A[1]; a2 = A[0]; a3 = B[0]; a4 = B[2];
+            } else if (A[1] == B[0] && B[2] == A[0]) {
+                a1 = A[2]; a2 = A[0]; a3 = B[0]; a4 = B[1];
+            } else if (A[2] == B[0] && B[2] == A[0]) {
+                a1 = A[1]; a2 = A[0]; a3 = B[0]; a4 = B[1];
+            }
+            if (a1 == -1) continue;
+            const Vec3d d12 = P[a1] - P[a2], d23 = P[a2] - P[a3], d43 = P[a4] - P[a3];
+            const Vec3d m = vcross(d12, d23), nv = vcross(d43, d23);
+            const double tan_s = vdot(nv, d12) * vlen(d23) / vdot(m, nv);
+            interactions_.emplace_back(
+                new HarmonicDihedralInteraction(a1, a2, a3, a4, ip.dihedral_k, std::atan(tan_s)));
+        }
+    for (int i = 2; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            if (component[i] >= 0 && component[i] == component[j]) continue;
+            addVDWInteraction(i, j);
+            if (options_.coulomb) addCoulombInteraction(i, j);
+        }
 }
 
 void Simulation::initialize() {                   // simulation.cpp:31-65
@@ -841,8 +1094,41 @@ void Simulation::initialize() {                   // simulation.cpp:31-65
     system.setMoleculeZ();
     calculateTipDummyDistance();
     buildInteractions();
-    lowerSystem();
+    if (options_.flexible) lowerFlexSystem();
+    else lowerSystem();
     initialized_ = true;
+}
+
+void Simulation::lowerFlexSystem() {
+    flex_terms_.clear();
+    n_tip_surface_terms_ = 0;
+    for (const auto& it : interactions_) {
+        if (it->isTipSurface() && n_tip_surface_terms_ == (int)flex_terms_.size()) ++n_tip_surface_terms_;
+        it->lowerFlex(flex_terms_);
+    }
+    const int n = system.n_atoms_;
+    flex_pos_.assign(3 * (size_t)n, 0.0);
+    flex_fixed_.assign(n, 0);
+    for (int i = 0; i < n; ++i) {
+        flex_pos_[3 * i] = system.positions_[i].x;
+        flex_pos_[3 * i + 1] = system.positions_[i].y;
+        flex_pos_[3 * i + 2] = system.positions_[i].z;
+        flex_fixed_[i] = system.fixed_[i];
+    }
+    flex_desc_ = mafm_flex_desc{};
+    flex_desc_.n_atoms = n;
+    flex_desc_.n_terms = (int32_t)flex_terms_.size();
+    flex_desc_.positions = flex_pos_.data();
+    flex_desc_.fixed = flex_fixed_.data();
+    flex_desc_.terms = flex_terms_.data();
+    flex_desc_.tip_pbc = system.tipPbc() ? 1 : 0;
+    // current_point == total_points / 2 (simulation.cpp:114)
+    flex_desc_.state_column = (n_points_[0] * n_points_[1]) / 2;
+    flex_desc_.tip_dummy_d = system.getTipDummyDistance();
+    memcpy(flex_desc_.cell, system.getUnitCell().m, sizeof flex_desc_.cell);
+    flex_desc_.offset[0] = system.getOffset().x;
+    flex_desc_.offset[1] = system.getOffset().y;
+    flex_desc_.offset[2] = system.getOffset().z;
 }
 
 void Simulation::lowerSystem() {
@@ -891,10 +1177,14 @@ mafm_scan_desc Simulation::scanDesc(const mafm_run_opts& r) const {
 
 void Simulation::run(const mafm_run_opts& r, mafm_scan_stats* stats) {   // simulation.cpp:67-143
     if (!initialized_) fail(MAFM_ERR_STATE, "run before initialize");
-    check(ctx_, mafm_set_systems(ctx_, 1, &desc_), "set system");
     mafm_scan_desc d = scanDesc(r);
     scan_generation_ = 0;
-    check(ctx_, mafm_scan(ctx_, &d, &last_stats_), "scan");
+    if (options_.flexible) {
+        check(ctx_, mafm_scan_flexible(ctx_, &flex_desc_, &d, &last_stats_), "flexible scan");
+    } else {
+        check(ctx_, mafm_set_systems(ctx_, 1, &desc_), "set system");
+        check(ctx_, mafm_scan(ctx_, &d, &last_stats_), "scan");
+    }
     record_base_ = 0;
     system_index_ = 0;
     scan_generation_ = ctx_scan_generation(ctx_);
@@ -1033,6 +1323,35 @@ void Simulation::writeOutput(int threads) {
     }
 }
 
+void Simulation::writeStates() {                  // system.cpp:54-65
+    requireOwnRecords();
+    if (!options_.flexible || flex_desc_.state_column < 0) return;
+    if (records_.empty()) fetchRecords();
+    const int n = system.n_atoms_, nz = n_points_[2];
+    std::vector<double> st((size_t)nz * n * 3);
+    check(ctx_, mafm_flex_state(ctx_, 0, nz, st.data()), "flexible state");
+    for (int k = 0; k < nz; ++k) {
+        const mafm_record& r = records_[(size_t)flex_desc_.state_column * nz + k];
+        const double* q = &st[(size_t)k * n * 3];
+        char name[1024];
+        snprintf(name, sizeof name, "%sstate_%.1f-%.1f-%.1f.xyz", options_.outputfolder.c_str(), r.x, r.y, q[2]);
+        std::string body = std::to_string(n) + "\n\n";
+        char line[256];
+        for (int i = 0; i < n; ++i) {
+            snprintf(line, sizeof line, "%s %8.4f %8.4f %8.4f\n", system.types_[i].c_str(), q[3 * i], q[3 * i + 1],
+                     q[3 * i + 2]);
+            body += line;
+        }
+        FILE* f = fopen(name, "w");
+        if (!f) fail(MAFM_ERR_INVALID, "cannot write %s", name);
+        const bool ok = fwrite(body.data(), 1, body.size(), f) == body.size();
+        if (fclose(f) != 0 || !ok) fail(MAFM_ERR_INVALID, "cannot write %s", name);
+        log_ += "+- Wrote ";
+        log_ += name;
+        log_ += "\n";
+    }
+}
+
 void Simulation::writeDeltaF(const mafm_deltaf_desc& d) {
     const InputOptions& o = options_;
     const int nx = n_points_[0], ny = n_points_[1], nz = n_points_[2];
@@ -1096,7 +1415,7 @@ static mafm_status guarded(mafm_sim* s, F&& f) {
 extern "C" {
 
 mafm_status mafm_sim_open(const char* input, const char* outdir, mafm_ctx* ctx, mafm_sim** out) {
-    if (!input || !ctx || !out) return MAFM_ERR_INVALID;
+    if (!input || !out) return MAFM_ERR_INVALID;
     *out = nullptr;
     mafm_sim* s = new mafm_sim(ctx);
     mafm_status st = guarded(s, [&]() {
@@ -1122,7 +1441,9 @@ mafm_status mafm_sim_info_get(mafm_sim* s, mafm_sim_info* info) {
     info->ny = m.n_points_[1];
     info->nz = m.n_points_[2];
     info->n_atoms = m.system.n_atoms_ - 2;
-    info->n_sites = m.systemDesc().n_sites;
+    info->n_sites = m.options_.flexible ? m.n_tip_surface_terms_ : m.systemDesc().n_sites;
+    info->flexible = m.options_.flexible;
+    info->n_terms = m.options_.flexible ? m.flexDesc().n_terms : 0;
     info->rigidgrid = m.options_.rigidgrid;
     info->use_external_potential = m.options_.use_external_potential;
     info->vdw_pbc = m.options_.vdw_pbc;
@@ -1142,7 +1463,25 @@ mafm_status mafm_sim_system(mafm_sim* s, mafm_system_desc* d) {
         s->err = "system requested before initialize";
         return MAFM_ERR_STATE;
     }
+    if (s->sim.options_.flexible) {
+        s->err = "a flexible job has a term list (mafm_sim_flex_system), not a site table";
+        return MAFM_ERR_STATE;
+    }
     *d = s->sim.systemDesc();
+    return MAFM_OK;
+}
+
+mafm_status mafm_sim_flex_system(mafm_sim* s, mafm_flex_desc* d) {
+    if (!s || !d) return MAFM_ERR_INVALID;
+    if (!s->sim.initialized_) {
+        s->err = "system requested before initialize";
+        return MAFM_ERR_STATE;
+    }
+    if (!s->sim.options_.flexible) {
+        s->err = "a rigid job has a site table (mafm_sim_system), not a term list";
+        return MAFM_ERR_STATE;
+    }
+    *d = s->sim.flexDesc();
     return MAFM_OK;
 }
 
@@ -1175,6 +1514,7 @@ mafm_status mafm_batch_run(int n, mafm_sim* const* sims, const mafm_run_opts* o,
         for (int m = 0; m < n; ++m) {
             Simulation& s = sims[m]->sim;
             if (!s.initialized_) throw Error(MAFM_ERR_STATE, "batch member not initialised");
+            if (s.options_.flexible) throw Error(MAFM_ERR_INVALID, "flexible jobs run one at a time (mafm_sim_run)");
             if (s.ctx_ != ctx) throw Error(MAFM_ERR_INVALID, "batch members must share one context");
             mafm_scan_desc d = s.scanDesc(o ? *o : def);
             if (d.nx != d0.nx || d.ny != d0.ny || d.nz != d0.nz || d.dx != d0.dx || d.dy != d0.dy ||
@@ -1234,9 +1574,23 @@ mafm_status mafm_sim_records(mafm_sim* s, size_t first, size_t count, mafm_recor
 mafm_status mafm_sim_write_output(mafm_sim* s, int threads) {
     if (!s) return MAFM_ERR_INVALID;
     return guarded(s, [&]() {
+        s->sim.log_.clear();
         s->sim.fetchRecords();
+        s->sim.writeStates();
         s->sim.writeOutput(threads);
     });
+}
+
+mafm_status mafm_sim_log(mafm_sim* s, char* buf, size_t size, size_t* needed) {
+    if (!s || (size && !buf)) return MAFM_ERR_INVALID;
+    const std::string& t = s->sim.log_;
+    if (needed) *needed = t.size() + 1;
+    if (size) {
+        const size_t n = std::min(size - 1, t.size());
+        memcpy(buf, t.data(), n);
+        buf[n] = 0;
+    }
+    return MAFM_OK;
 }
 
 const char* mafm_sim_last_error(const mafm_sim* s) { return s ? s->err.c_str() : "no simulation"; }

@@ -16,6 +16,7 @@
 #include <map>
 #include <memory>
 #include <stdexcept>
+#include <array>
 #include <chrono>
 #include <string>
 #include <vector>
@@ -97,7 +98,13 @@ struct OverwriteParameters {
     double eps = 0, sig = 0, de = 0, alpha = 0, re = 0;
 };
 
-// InteractionParameters (interactions.hpp:47-55), rigid subset.
+// A possible bond between two atom types (PossibleBond, interactions.hpp:42-45).
+struct PossibleBond {
+    std::string a, b;   // unordered pair
+    double r0 = 0;      // expected bond length
+};
+
+// InteractionParameters (interactions.hpp:47-55).
 struct InteractionParameters {
     double qbase = 0;
     bool qbase_set = false;
@@ -105,6 +112,10 @@ struct InteractionParameters {
     bool harm_set = false;
     std::map<std::string, AtomParameters> atom_parameters;
     std::vector<OverwriteParameters> overwrite_parameters;
+    // flexible on (readFlexibleParameters, parse.cpp:700-782)
+    double bond_k = 0, angle_k = 0, dihedral_k = 0;
+    double substrate_eps = 0, substrate_sig = 0, substrate_lambda = 0, substrate_rc = 0, substrate_k = 0;
+    std::vector<PossibleBond> possible_bonds;
 };
 
 // Mixing rules (interactions.hpp:58-64).
@@ -137,6 +148,8 @@ class Interaction {
     virtual ~Interaction() = default;
     virtual bool isTipSurface() const = 0;
     virtual void lower(Lowering& out) const = 0;
+    // flexible on: the term of the whole-list evaluation (mafm_flex_term)
+    virtual void lowerFlex(std::vector<mafm_flex_term>& out) const = 0;
 };
 
 class LJInteraction : public Interaction {
@@ -145,6 +158,7 @@ class LJInteraction : public Interaction {
         : i1_(i1), i2_(i2), es6_(es6), es12_(es12), shift_(shift) {}
     bool isTipSurface() const override { return i1_ == 1; }
     void lower(Lowering& out) const override;
+    void lowerFlex(std::vector<mafm_flex_term>& out) const override;
 
   private:
     int i1_, i2_;
@@ -158,6 +172,7 @@ class MorseInteraction : public Interaction {
         : i1_(i1), i2_(i2), de_(de), a_(a), re_(re), shift_(shift) {}
     bool isTipSurface() const override { return i1_ == 1; }
     void lower(Lowering& out) const override;
+    void lowerFlex(std::vector<mafm_flex_term>& out) const override;
 
   private:
     int i1_, i2_;
@@ -170,6 +185,7 @@ class CoulombInteraction : public Interaction {
     CoulombInteraction(int i1, int i2, double qq) : i1_(i1), i2_(i2), qq_(qq) {}
     bool isTipSurface() const override { return i1_ == 1; }
     void lower(Lowering& out) const override;
+    void lowerFlex(std::vector<mafm_flex_term>& out) const override;
 
   private:
     int i1_, i2_;
@@ -181,10 +197,68 @@ class TipHarmonicInteraction : public Interaction {
     TipHarmonicInteraction(int i1, int i2, double k, double r0) : i1_(i1), i2_(i2), k_(k), r0_(r0) {}
     bool isTipSurface() const override { return false; }
     void lower(Lowering& out) const override;
+    void lowerFlex(std::vector<mafm_flex_term>& out) const override;
 
   private:
     int i1_, i2_;
     double k_, r0_;
+};
+
+// The flexible-surface terms (interactions.hpp:186-303).  They exist only
+// with flexible on and lower to mafm_flex_term.
+class FlexOnlyInteraction : public Interaction {
+  public:
+    bool isTipSurface() const override { return false; }
+    void lower(Lowering& out) const override;
+};
+
+class XYHarmonicInteraction : public FlexOnlyInteraction {
+  public:
+    XYHarmonicInteraction(int i, double k, Vec2d p0) : i_(i), k_(k), p0_(p0) {}
+    void lowerFlex(std::vector<mafm_flex_term>& out) const override;
+  private:
+    int i_;
+    double k_;
+    Vec2d p0_;
+};
+
+class SubstrateInteraction : public FlexOnlyInteraction {
+  public:
+    // the derived constants of the reference's constructor (interactions.hpp:222-229)
+    SubstrateInteraction(int i, double eps, double sig, double rc, double lambda);
+    void lowerFlex(std::vector<mafm_flex_term>& out) const override;
+  private:
+    int i_;
+    double sig_, rc_, multiplier_, ulj_, ushift_, z0_;
+};
+
+class HarmonicInteraction : public FlexOnlyInteraction {
+  public:
+    HarmonicInteraction(int i1, int i2, double k, double r0) : i1_(i1), i2_(i2), k_(k), r0_(r0) {}
+    void lowerFlex(std::vector<mafm_flex_term>& out) const override;
+  private:
+    int i1_, i2_;
+    double k_, r0_;
+};
+
+class HarmonicAngleInteraction : public FlexOnlyInteraction {
+  public:
+    HarmonicAngleInteraction(int shared, int i1, int i2, double k, double theta0)
+        : shared_(shared), i1_(i1), i2_(i2), k_(k), theta0_(theta0) {}
+    void lowerFlex(std::vector<mafm_flex_term>& out) const override;
+  private:
+    int shared_, i1_, i2_;
+    double k_, theta0_;
+};
+
+class HarmonicDihedralInteraction : public FlexOnlyInteraction {
+  public:
+    HarmonicDihedralInteraction(int i1, int i2, int i3, int i4, double k, double sigma0)
+        : i1_(i1), i2_(i2), i3_(i3), i4_(i4), k_(k), sigma0_(sigma0) {}
+    void lowerFlex(std::vector<mafm_flex_term>& out) const override;
+  private:
+    int i1_, i2_, i3_, i4_;
+    double k_, sigma0_;
 };
 
 // ForceGrid bound to a device grid handle (force_grid.hpp:15-51).
@@ -201,6 +275,7 @@ class GridInteraction : public Interaction {
     explicit GridInteraction(const ForceGrid& g) : grid_(g) {}
     bool isTipSurface() const override { return true; }
     void lower(Lowering& out) const override { out.force_grid = grid_.handle; }
+    void lowerFlex(std::vector<mafm_flex_term>& out) const override;
     const ForceGrid& grid() const { return grid_; }
 
   private:
@@ -212,6 +287,7 @@ class ElectrostaticPotentialInteraction : public Interaction {
     explicit ElectrostaticPotentialInteraction(const ForceGrid& g) : grid_(g) {}
     bool isTipSurface() const override { return true; }
     void lower(Lowering& out) const override { out.potential_grid = grid_.handle; }
+    void lowerFlex(std::vector<mafm_flex_term>& out) const override;
     const ForceGrid& grid() const { return grid_; }
 
   private:
@@ -264,11 +340,18 @@ class Simulation {
     void readInputFile();
     void readXYZFile();
     void readParameterFile();
+    void readFlexibleParameters(const std::string& path);
     void initialize();
     void buildInteractions();
     // lower the interaction list into a device system description
     void lowerSystem();
+    // flexible on: lower the whole list into a term table
+    void lowerFlexSystem();
     const mafm_system_desc& systemDesc() const { return desc_; }
+    const mafm_flex_desc& flexDesc() const { return flex_desc_; }
+    // System::makeXYZFile (system.cpp:54-65) of the state column at every
+    // height of the last flexible run; the console lines go to log_
+    void writeStates();
     mafm_scan_desc scanDesc(const mafm_run_opts& o) const;
     void run(const mafm_run_opts& opts, mafm_scan_stats* stats);
     void fetchRecords();
@@ -302,6 +385,9 @@ class Simulation {
     // another scan on the context replaces the records
     uint64_t scan_generation_ = 0;
     void requireOwnRecords() const;
+    // console lines of the output writing ("+- Wrote ..."), in order
+    std::string log_;
+    int n_tip_surface_terms_ = 0;
 
   private:
     void calculateTipDummyDistance();
@@ -311,10 +397,16 @@ class Simulation {
     void buildTipSurfaceInteractions();
     void buildTipGridInteractions();
     void buildTipDummyInteractions();
+    void buildSurfaceSurfaceInteractions();
+    void buildSubstrateInteractions();
     ForceGrid buildElectrostaticPotential();
     std::vector<int> owned_grids_;
     Lowering low_;
     mafm_system_desc desc_{};
+    std::vector<mafm_flex_term> flex_terms_;
+    std::vector<double> flex_pos_;
+    std::vector<int32_t> flex_fixed_;
+    mafm_flex_desc flex_desc_{};
 };
 
 }  // namespace mafm

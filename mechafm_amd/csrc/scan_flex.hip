@@ -1,0 +1,421 @@
+// Flexible scan (flexible on): every free surface atom relaxes with the tip.
+//
+// One wavefront relaxes one pixel column at a time (persistent grid, columns
+// handed out by an atomic counter).  The column's whole state lives with the
+// wave -- in LDS when it fits, else in a per-block slice of HBM: positions,
+// the integrator's evaluation point and stage velocities, velocities, forces
+// and the term results.
+//
+// One integrator stage evaluates the interaction list in two phases:
+//   1. term-parallel: lane l evaluates terms l, l + 64, ... and writes one
+//      result slot {fx, fy, fz, e} per atom the term acts on, the force that
+//      atom receives (interactions.cpp:15-51, 252-362; `forces[i] -= f` is
+//      stored as -f, which adds with the same rounding);
+//   2. atom-parallel: lane l sums the slots of atoms l, l + 64, ... in the
+//      interaction list's order, from zero -- the order in which the
+//      reference's Interaction::eval calls accumulate forces_[i] and
+//      energies_[i] (integrators.cpp:8-111) -- so each force is the
+//      reference's to the last bit given the same term arithmetic.
+// FIRE's power P = sum_i v_i . f_i (minimiser.cpp:107-109) is summed by one
+// lane in atom order.  Fixed atoms (fixed = 1) never move and have zero
+// velocity, so their forces (which the reference computes and multiplies by
+// that zero velocity in P) are not summed.  Built with -ffp-contract=off;
+// IEEE divisions and square roots.  The angle, dihedral and Morse terms call
+// the device acos / sin / atan / exp, which may differ from glibc's in the
+// last place.
+#include "flex.hpp"
+#include "ref_terms.hpp"
+
+namespace mafm {
+namespace {
+
+constexpr double TOL = 1e-10;   // TOLERANCE, globals.hpp:11
+
+__device__ inline V3 ld3(const double* a, int i) { return {a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }
+__device__ inline void st3(double* a, int i, V3 v) {
+    a[3 * i] = v.x;
+    a[3 * i + 1] = v.y;
+    a[3 * i + 2] = v.z;
+}
+__device__ inline V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ inline V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ inline V3 neg(V3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ inline V3 scl(double s, V3 v) { return {s * v.x, s * v.y, s * v.z}; }   // T * Vector3
+__device__ inline V3 scr(V3 v, double s) { return {v.x * s, v.y * s, v.z * s}; }   // Vector3 * T
+__device__ inline V3 dvs(V3 v, double s) { return {v.x / s, v.y / s, v.z / s}; }   // Vector3 / T
+__device__ inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }   // vectors.hpp:116-118
+__device__ inline V3 cross(V3 a, V3 b) {                                                   // vectors.hpp:120-122
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+__device__ inline void put(double* R, int slot, V3 f, double e) {
+    R[4 * slot] = f.x;
+    R[4 * slot + 1] = f.y;
+    R[4 * slot + 2] = f.z;
+    R[4 * slot + 3] = e;
+}
+
+// One Interaction::eval of the list at positions X, into the term's slots R
+// (slot r = the term's atom[r]).
+__device__ void eval_term(const mafm_flex_term& t, const double* X, double* R) {
+    switch (t.kind) {
+        case MAFM_FX_LJ:
+        case MAFM_FX_MORSE: {   // interactions.cpp:15-40
+            const V3 p2 = ld3(X, t.atom[1]);
+            const V3 r = sub(ld3(X, t.atom[0]), add(p2, {t.shift[0], t.shift[1], t.shift[2]}));
+            V3 f;
+            double e;
+            vdw(t.kind == MAFM_FX_LJ ? 0 : 1, t.p[0], t.p[1], t.p[2], r, f, e);
+            put(R, 0, f, e);
+            put(R, 1, neg(f), e);
+            return;
+        }
+        case MAFM_FX_COULOMB: {   // interactions.cpp:42-51
+            V3 f;
+            double e;
+            coulomb(t.p[0], sub(ld3(X, t.atom[0]), ld3(X, t.atom[1])), f, e);
+            put(R, 0, f, e);
+            put(R, 1, neg(f), e);
+            return;
+        }
+        case MAFM_FX_TIP_HARMONIC: {   // interactions.cpp:252-266
+            const V3 r = sub(ld3(X, t.atom[0]), ld3(X, t.atom[1]));
+            const double rl = sqrt(sq(r.x) + sq(r.y));
+            const double dr = rl - t.p[1];
+            const double e = t.p[0] * dr * dr;
+            V3 f = {0, 0, 0};
+            if (rl > TOL) {
+                const double s = -2 * t.p[0] * dr / rl;
+                f = {s * r.x, s * r.y, 0};
+            }
+            put(R, 0, f, e);
+            put(R, 1, neg(f), e);
+            return;
+        }
+        case MAFM_FX_BOND: {   // HarmonicInteraction::eval, interactions.cpp:296-306
+            const V3 r = sub(ld3(X, t.atom[0]), ld3(X, t.atom[1]));
+            const double rl = len3(r);
+            const double dr = rl - t.p[1];
+            const double e = t.p[0] * sq(dr);
+            const V3 f = scl(-2 * t.p[0] * dr / rl, r);
+            put(R, 0, f, e);
+            put(R, 1, neg(f), e);
+            return;
+        }
+        case MAFM_FX_ANGLE: {   // HarmonicAngleInteraction::eval, interactions.cpp:308-332
+            const V3 ps = ld3(X, t.atom[0]);
+            const V3 r1v = sub(ld3(X, t.atom[1]), ps), r2v = sub(ld3(X, t.atom[2]), ps);
+            const double r1 = len3(r1v), r2 = len3(r2v);
+            double cos_t = dot(r1v, r2v) / (r1 * r2);
+            if (cos_t > 1) cos_t = 1;
+            else if (cos_t < -1) cos_t = -1;
+            const double theta = acos(cos_t);
+            const double sin_t = sin(theta);
+            const double d_theta = theta - t.p[1];
+            const double fm = -2 * t.p[0] * d_theta / sin_t;
+            const V3 f1 = scl(-fm / r1, sub(dvs(r2v, r2), dvs(scr(r1v, cos_t), r1)));
+            const V3 f2 = scl(-fm / r2, sub(dvs(r1v, r1), dvs(scr(r2v, cos_t), r2)));
+            const double e = t.p[0] * sq(d_theta);
+            put(R, 0, neg(add(f1, f2)), e);   // forces[shared] -= f1 + f2
+            put(R, 1, f1, e);
+            put(R, 2, f2, e);
+            return;
+        }
+        case MAFM_FX_DIHEDRAL: {   // HarmonicDihedralInteraction::eval, interactions.cpp:334-362
+            const V3 p2 = ld3(X, t.atom[1]), p3 = ld3(X, t.atom[2]);
+            const V3 r12 = sub(ld3(X, t.atom[0]), p2), r23 = sub(p2, p3), r43 = sub(ld3(X, t.atom[3]), p3);
+            const double r23l = len3(r23);
+            const V3 mv = cross(r12, r23), nv = cross(r43, r23);
+            const double m = len3(mv), n = len3(nv);
+            const double tan_s = dot(nv, r12) * r23l / dot(mv, nv);
+            const double sigma = atan(tan_s);
+            const double d_sigma = sigma - t.p[1];
+            const double fm = 2 * t.p[0] * d_sigma;
+            const double e = t.p[0] * sq(d_sigma);
+            put(R, 0, scl(-fm * r23l / (m * m), mv), e);
+            put(R, 1, scl(-fm, sub(scl(dot(r43, r23) / (n * n * r23l), nv),
+                                   scl(((r23l * r23l) + dot(r12, r23)) / (m * m * r23l), mv))), e);
+            put(R, 2, scl(-fm, add(scl(dot(r12, r23) / (m * m * r23l), mv),
+                                   scl(((r23l * r23l) - dot(r43, r23)) / (n * n * r23l), nv))), e);
+            put(R, 3, scl(fm * r23l / (n * n), nv), e);
+            return;
+        }
+        case MAFM_FX_SUBSTRATE: {   // SubstrateInteraction::eval, interactions.cpp:280-294
+            // p: multiplier, ulj, ushift, z0, rc, sigma
+            const double dz = X[3 * t.atom[0] + 2] - t.p[3];
+            const double sig_z = t.p[5] / dz;
+            const double sig_z2 = sig_z * sig_z;
+            const double sig_z4 = sig_z2 * sig_z2;
+            const double sig_z10 = sig_z4 * sig_z4 * sig_z2;
+            double eterm = 0.0, fterm = 0.0;
+            if (dz <= t.p[4]) {
+                eterm += t.p[0] * (2. / 5 * sig_z10 - sig_z4) + t.p[1] * dz * dz - t.p[2];
+                fterm += 4 * t.p[0] / dz * (sig_z10 - sig_z4) + 2 * t.p[1] * dz;
+            }
+            // forces[i].z += fterm: the +0 added to x and y leaves them as they are
+            put(R, 0, {0, 0, fterm}, eterm);
+            return;
+        }
+        default: {   // MAFM_FX_XY_HARMONIC, interactions.cpp:268-278: p = k, x0, y0
+            const double rx = X[3 * t.atom[0]] - t.p[1], ry = X[3 * t.atom[0] + 1] - t.p[2];
+            const double rl = sqrt(sq(rx) + sq(ry));
+            const double e = t.p[0] * rl * rl;
+            V3 f = {0, 0, 0};
+            if (rl > TOL) f = {-2 * t.p[0] * rx, -2 * t.p[0] * ry, 0};
+            put(R, 0, f, e);
+            return;
+        }
+    }
+}
+
+// checkConvergence (minimiser.cpp:15-37)
+__device__ inline bool converged(const FlexArgs& a, V3 f, double ediff) {
+    switch (a.minterm) {
+        case MAFM_TERM_E: return fabs(ediff) < a.etol;
+        case MAFM_TERM_F: return len3(f) < a.ftol;
+        default: return fabs(ediff) < a.etol && len3(f) < a.ftol;
+    }
+}
+
+__host__ __device__ inline size_t block_doubles(int n, int n_slots, bool rk4) {
+    return (size_t)(16 + (rk4 ? 6 : 0)) * n + 4 * (size_t)n_slots + 8;
+}
+
+struct State {
+    double *pos, *pev, *vel, *vst, *frc, *pdot, *res, *bc, *facc, *vacc;
+};
+
+__device__ inline State layout(double* p, int n, int n_slots, bool rk4) {
+    State S;
+    S.pos = p; p += 3 * n;
+    S.pev = p; p += 3 * n;
+    S.vel = p; p += 3 * n;
+    S.vst = p; p += 3 * n;
+    S.frc = p; p += 3 * n;
+    S.pdot = p; p += n;
+    S.res = p; p += 4 * n_slots;
+    S.bc = p; p += 8;
+    S.facc = p; p += rk4 ? 3 * n : 0;
+    S.vacc = p;
+    return S;
+}
+
+__device__ inline void wave_sync() { __syncthreads(); }   // one wave per workgroup
+
+// Forces on every free atom at the positions X (into S.frc) and the tip's
+// energy (into S.bc[0]): the interaction list evaluated as one integrator
+// stage does (integrators.cpp:31-33).  ts_only: only the tip-surface terms,
+// for the tip, as System::evalTipSurfaceForces does (system.cpp:41-51).
+__device__ void evaluate(const FlexArgs& a, const State& S, const double* X, int lane, bool ts_only) {
+    const int ne = ts_only ? a.n_ts : a.n_eval;
+    for (int k = lane; k < ne; k += 64) eval_term(a.terms[a.eval_term[k]], X, S.res + 4 * a.eval_slot[k]);
+    wave_sync();
+    for (int i = ts_only ? (lane == 1 ? 1 : a.n_atoms) : lane; i < a.n_atoms; i += 64) {
+        if (a.fixed[i] == 1) {
+            st3(S.frc, i, {0, 0, 0});
+            continue;
+        }
+        V3 acc = {0, 0, 0};
+        double eacc = 0;
+        const int q1 = a.csr_off[i + 1];
+        for (int q = a.csr_off[i]; q < q1; ++q) {
+            const int s = a.csr[q];
+            if (ts_only && s >= a.ts_slots) break;
+            acc = add(acc, {S.res[4 * s], S.res[4 * s + 1], S.res[4 * s + 2]});
+            eacc += S.res[4 * s + 3];
+        }
+        st3(S.frc, i, acc);
+        if (i == 1) S.bc[0] = eacc;
+        if (ts_only) break;
+    }
+    wave_sync();
+}
+
+template <bool IN_LDS>
+__global__ void __launch_bounds__(64) k_scan_flex(FlexArgs a) {
+    extern __shared__ double lds_flex[];
+    __shared__ int col_shared;
+    const int lane = threadIdx.x;
+    const int n = a.n_atoms;
+    const bool rk4 = a.minimiser == MAFM_MIN_FIRE && a.integrator == MAFM_INT_RK4;
+    double* base = IN_LDS ? lds_flex : a.work + (size_t)blockIdx.x * block_doubles(n, a.n_slots, rk4);
+    const State S = layout(base, n, a.n_slots, rk4);
+    const int n_cols = a.nx * a.ny;
+    for (;;) {
+        if (lane == 0) col_shared = atomicAdd(a.next_col, 1);
+        wave_sync();
+        const int col = col_shared;
+        wave_sync();
+        if (col >= n_cols) break;
+        const int ci = col / a.ny, cj = col - ci * a.ny;
+        const double x = ci * a.dx, y = cj * a.dy;   // simulation.cpp:80-83
+        // System copy (simulation.cpp:93): initial positions, zero velocities
+        for (int i = lane; i < n; i += 64) {
+            st3(S.pos, i, ld3(a.pos0, i));
+            st3(S.vel, i, {0, 0, 0});
+        }
+        wave_sync();
+        if (lane == 0) {
+            // setDummyXY / setDummyZ (system.cpp:139-162, system.hpp:47-50)
+            double px = x, py = y;
+            if (a.tip_pbc) wrap_xy(a.cell, a.cell_inv, a.offset, x, y, px, py);
+            st3(S.pos, 0, {px, py, a.zhigh});
+            st3(S.pos, 1, {px, py, a.zhigh - a.tip_dummy_d});
+        }
+        wave_sync();
+        double e_tip = 0;   // energies_[1] of the System copy: 0 until its first step
+        for (int k = 0; k < a.nz; ++k) {
+            // FIREMinimisation / SDMinimisation (minimiser.cpp:39-129)
+            double alpha = 0.1, dt = a.dt;
+            int nnn = 0, n_tot = 1;
+            for (; n_tot < a.maxsteps; ++n_tot) {
+                const double prev_e = e_tip;
+                if (a.minimiser == MAFM_MIN_SD) {   // evaluate, test, move
+                    evaluate(a, S, S.pos, lane, false);
+                    e_tip = S.bc[0];
+                    if (converged(a, ld3(S.frc, 1), e_tip - prev_e)) break;
+                    for (int i = lane; i < n; i += 64)
+                        if (a.fixed[i] != 1) st3(S.pos, i, add(ld3(S.pos, i), scl(a.dt, ld3(S.frc, i))));
+                    wave_sync();
+                    continue;
+                }
+                if (a.integrator == MAFM_INT_EULER) {   // eulerStep
+                    evaluate(a, S, S.pos, lane, false);
+                    for (int i = lane; i < n; i += 64)
+                        if (a.fixed[i] != 1) {
+                            st3(S.pos, i, add(ld3(S.pos, i), scl(dt, ld3(S.vel, i))));
+                            st3(S.vel, i, add(ld3(S.vel, i), scl(dt, ld3(S.frc, i))));   // masses are 1
+                        }
+                    e_tip = S.bc[0];
+                } else if (a.integrator == MAFM_INT_MIDPOINT) {   // midpointStep
+                    evaluate(a, S, S.pos, lane, false);
+                    for (int i = lane; i < n; i += 64) {
+                        if (a.fixed[i] != 1) {
+                            st3(S.pev, i, add(ld3(S.pos, i), scl(dt / 2, ld3(S.vel, i))));
+                            st3(S.vst, i, add(ld3(S.vel, i), scl(dt / 2, ld3(S.frc, i))));
+                        } else {
+                            st3(S.pev, i, ld3(S.pos, i));
+                        }
+                    }
+                    wave_sync();
+                    evaluate(a, S, S.pev, lane, false);
+                    for (int i = lane; i < n; i += 64)
+                        if (a.fixed[i] != 1) {
+                            st3(S.pos, i, add(ld3(S.pos, i), scl(dt, ld3(S.vst, i))));
+                            st3(S.vel, i, add(ld3(S.vel, i), scl(dt, ld3(S.frc, i))));
+                        }
+                    e_tip = S.bc[0];
+                } else {   // rk4Step: stages at pos, pos + dt/2 v1, pos + dt/2 v2, pos + dt v3
+                    double e_acc = 0;
+                    for (int s = 0; s < 4; ++s) {
+                        evaluate(a, S, s == 0 ? S.pos : S.pev, lane, false);
+                        const double es = S.bc[0];
+                        e_acc = s == 0 ? es : e_acc + (s == 3 ? es : 2 * es);
+                        for (int i = lane; i < n; i += 64) {
+                            const V3 fs = ld3(S.frc, i), vs = s == 0 ? ld3(S.vel, i) : ld3(S.vst, i);
+                            if (s == 0) {
+                                st3(S.facc, i, fs);
+                                st3(S.vacc, i, vs);
+                            } else if (s < 3) {
+                                st3(S.facc, i, add(ld3(S.facc, i), scl(2, fs)));
+                                st3(S.vacc, i, add(ld3(S.vacc, i), scl(2, vs)));
+                            } else {
+                                st3(S.facc, i, add(ld3(S.facc, i), fs));
+                                st3(S.vacc, i, add(ld3(S.vacc, i), vs));
+                            }
+                            if (s < 3) {
+                                const double h = s < 2 ? dt / 2 : dt;
+                                if (a.fixed[i] != 1) {
+                                    st3(S.pev, i, add(ld3(S.pos, i), scl(h, vs)));
+                                    st3(S.vst, i, add(ld3(S.vel, i), scl(h, fs)));
+                                } else {
+                                    st3(S.pev, i, ld3(S.pos, i));
+                                    st3(S.vst, i, ld3(S.vel, i));
+                                }
+                            }
+                        }
+                        wave_sync();
+                    }
+                    for (int i = lane; i < n; i += 64) {
+                        if (a.fixed[i] != 1) {
+                            st3(S.pos, i, add(ld3(S.pos, i), scl(dt / 6, ld3(S.vacc, i))));
+                            st3(S.vel, i, add(ld3(S.vel, i), scl(dt / 6, ld3(S.facc, i))));
+                        }
+                        st3(S.frc, i, dvs(ld3(S.facc, i), 6));
+                    }
+                    e_tip = e_acc / 6;
+                }
+                wave_sync();
+                if (converged(a, ld3(S.frc, 1), e_tip - prev_e)) break;
+                // FIRE (minimiser.cpp:107-126): P over every atom in index order
+                for (int i = lane; i < n; i += 64) S.pdot[i] = dot(ld3(S.vel, i), ld3(S.frc, i));
+                wave_sync();
+                if (lane == 0) {
+                    double p = 0;
+                    for (int i = 0; i < n; ++i) p += S.pdot[i];
+                    S.bc[1] = p;
+                }
+                wave_sync();
+                const double p = S.bc[1];
+                for (int i = lane; i < n; i += 64) {
+                    if (a.fixed[i] == 1) continue;
+                    const V3 v = ld3(S.vel, i), f = ld3(S.frc, i);
+                    const double fl = len3(f);
+                    const V3 fh = fl > 1e-10 ? V3{f.x / fl, f.y / fl, f.z / fl} : V3{0, 0, 0};   // normalized()
+                    st3(S.vel, i, add(scl(1 - alpha, v), scr(scl(alpha, fh), len3(v))));
+                }
+                wave_sync();
+                if (p < 0) {
+                    for (int i = lane; i < n; i += 64) st3(S.vel, i, {0, 0, 0});
+                    wave_sync();
+                    nnn = 0;
+                    dt *= 0.5;
+                    alpha = 0.1;
+                } else {
+                    if (nnn > 5) {
+                        dt = fmin(dt * 1.1, 10 * a.dt);
+                        alpha *= 0.99;
+                    }
+                    ++nnn;
+                }
+            }
+            // makeXYZFile of the state column (simulation.cpp:114-116), then
+            // getOutput (system.cpp:28-51)
+            if (col == a.state_col && a.state)
+                for (int i = lane; i < n; i += 64) st3(a.state + (size_t)k * 3 * n, i, ld3(S.pos, i));
+            evaluate(a, S, S.pos, lane, true);
+            if (lane == 0) {
+                const V3 d = ld3(S.pos, 0), t = ld3(S.pos, 1), f = ld3(S.frc, 1);
+                DevRecord rec;
+                rec.x = x; rec.y = y; rec.z = d.z;
+                rec.fx = f.x; rec.fy = f.y; rec.fz = f.z;
+                rec.rx = t.x - d.x; rec.ry = t.y - d.y; rec.rz = t.z - d.z;
+                rec.e = S.bc[0];
+                rec.n_steps = n_tot;
+                rec.flags = n_tot >= a.maxsteps ? REC_UNCONVERGED : 0;
+                a.rec[(size_t)col * a.nz + k] = rec;
+                // lowerTip (system.hpp:51-54)
+                S.pos[2] -= a.dz;
+                S.pos[5] -= a.dz;
+            }
+            wave_sync();
+        }
+    }
+}
+
+}  // namespace
+
+size_t flex_block_doubles(int n_atoms, int n_slots, bool rk4) { return block_doubles(n_atoms, n_slots, rk4); }
+
+hipError_t launch_scan_flex(const FlexArgs& a, int n_blocks, bool in_lds, hipStream_t st) {
+    const bool rk4 = a.minimiser == MAFM_MIN_FIRE && a.integrator == MAFM_INT_RK4;
+    if (in_lds) {
+        const size_t lds = block_doubles(a.n_atoms, a.n_slots, rk4) * sizeof(double);
+        hipLaunchKernelGGL(k_scan_flex<true>, dim3(n_blocks), dim3(64), lds, st, a);
+    } else {
+        hipLaunchKernelGGL(k_scan_flex<false>, dim3(n_blocks), dim3(64), 0, st, a);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace mafm

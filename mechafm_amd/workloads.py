@@ -17,6 +17,12 @@ mechafm.cpp:47-49):
           Hartree potential on a 160 x 160 x 80 grid from Gaussian charges,
           256 x 256 px x 40 z.
 * ``c5``  150-atom molecule, LJ + Coulomb, 128 x 128 px x 30 z (adjoint).
+* ``fx``  flexible on: a hydrogen-terminated coronene (C24H12, edge H
+          tethered in xy, two carbons fixed) beside a separate ammonia
+          molecule, LJ + Coulomb with xyz charges, so the interaction list
+          holds bonds, angles, proper and improper dihedrals, non-bonded
+          pairs between the two molecules, the substrate wall and xy tethers.
+* ``fx-example``  the reference's example graphene job with flexible on.
 
 Every workload has a ``mini`` variant that keeps the physics and shrinks the
 pixel grid so a CPU FP64 run finishes in seconds.
@@ -50,6 +56,7 @@ class Molecule:
     types: List[str]
     pos: np.ndarray                       # (n, 3) float64
     charges: Optional[np.ndarray] = None  # (n,) float64 or None
+    fixed: Optional[np.ndarray] = None    # (n,) int: xyz sixth column (flexible on)
 
 
 @dataclass
@@ -270,14 +277,17 @@ def write_xyz(path: str, mol: Molecule, with_charges: bool) -> None:
         f.write(f"{len(mol.types)}\n\n")
         for i, t in enumerate(mol.types):
             x, y, z = mol.pos[i]
-            if with_charges:
+            if mol.fixed is not None:
+                q = mol.charges[i] if (with_charges and mol.charges is not None) else 0.0
+                f.write(f"{t:<4s} {x:.8f} {y:.8f} {z:.8f} {q:.8f} {int(mol.fixed[i])}\n")
+            elif with_charges:
                 f.write(f"{t:<4s} {x:.8f} {y:.8f} {z:.8f} {mol.charges[i]:.8f}\n")
             else:
                 f.write(f"{t:<4s} {x:.8f} {y:.8f} {z:.8f}\n")
 
 
 def write_params(path: str, atoms: Dict[str, Tuple[float, float, float, float]],
-                 harm=HARM, overwrites: Sequence[str] = ()) -> None:
+                 harm=HARM, overwrites: Sequence[str] = (), extra: Sequence[str] = ()) -> None:
     with open(path, "w") as f:
         f.write("# name | epsilon | sigma | mass | charge\n")
         for t, (e, s, m, q) in atoms.items():
@@ -285,6 +295,8 @@ def write_params(path: str, atoms: Dict[str, Tuple[float, float, float, float]],
         f.write(f"harm {harm[0]:.5f} {harm[1]:.2f}\n")
         for line in overwrites:
             f.write(f"pair_ovwrt {line}\n")
+        for line in extra:
+            f.write(line + "\n")
 
 
 def write_input(path: str, opts: Dict[str, str]) -> None:
@@ -350,10 +362,10 @@ def morse_overwrites(tip: str, types: Sequence[str], atoms=BASE_ATOMS, a: float 
 
 
 def _emit(name: str, outdir: str, mol: Molecule, opts: Dict[str, str], charges_in_xyz: bool,
-          overwrites: Sequence[str] = (), atoms=BASE_ATOMS, extra=None) -> Workload:
+          overwrites: Sequence[str] = (), atoms=BASE_ATOMS, extra=None, param_lines: Sequence[str] = ()) -> Workload:
     os.makedirs(outdir, exist_ok=True)
     write_xyz(os.path.join(outdir, opts["xyzfile"]), mol, charges_in_xyz)
-    write_params(os.path.join(outdir, opts["paramfile"]), atoms, HARM, overwrites)
+    write_params(os.path.join(outdir, opts["paramfile"]), atoms, HARM, overwrites, param_lines)
     path = os.path.join(outdir, "input.scan")
     write_input(path, opts)
     return Workload(name, path, dict(opts), mol, dict(extra or {}))
@@ -504,3 +516,101 @@ def scan_shape(opts: Dict[str, str]) -> Tuple[int, int, int]:
     dx, dy, dz = float(opts["dx"]), float(opts["dy"]), float(opts["dz"])
     zh, zl = float(opts["zhigh"]), float(opts["zlow"])
     return (math.floor(ax / dx) + 1, math.floor(ay / dy) + 1, math.floor((zh - zl) / dz) + 1)
+
+
+# --------------------------------------------------------------------------
+# flexible on
+# --------------------------------------------------------------------------
+# The flexible-topology lines of the reference's examples/parameters.dat:26-44
+# (possible bonds, harmonic constants, 10-4 substrate wall), plus N-H / C-N.
+FLEX_PARAM_LINES = (
+    "topobond C C 1.430", "topobond C H 1.095", "topobond N H 1.050", "topobond C N 1.470",
+    "bond 25.000", "angle 0.2500", "dihedral 0.2500",
+    "substrate 0.100 3.0 3.0 7.5 0.01",
+)
+
+
+def terminated_flake(n_rings: int = 2, ch: float = 1.09) -> Molecule:
+    """hex_flake(n_rings) with one H on every edge carbon (two C neighbours),
+    pointing away from the two neighbours' midpoint."""
+    c = hex_flake(n_rings)
+    xy = c.pos[:, :2]
+    types, pos = list(c.types), [p.copy() for p in c.pos]
+    for i in range(len(xy)):
+        d = np.linalg.norm(xy - xy[i], axis=1)
+        nb = np.where((d > 0.1) & (d < 1.6))[0]
+        if len(nb) == 2:
+            u = xy[i] - xy[nb].mean(axis=0)
+            u /= np.linalg.norm(u)
+            types.append("H")
+            pos.append(np.array([xy[i][0] + ch * u[0], xy[i][1] + ch * u[1], 0.0]))
+    return Molecule(types, np.array(pos))
+
+
+def flex_molecule(seed: int = 1240) -> Molecule:
+    """Coronene C24H12 with a slight seeded buckle, edge H tethered (fixed 2),
+    two inner carbons fixed (1), and a pyramidal ammonia 6.9 A from the
+    centre at z = 0.6 A, 2.3 A from the nearest flake atom (its first H
+    tethered, the rest free), with seeded zero-sum xyz charges."""
+    rng = np.random.default_rng(seed)
+    a = terminated_flake(2)
+    pos = a.pos.copy()
+    pos[:, 2] += rng.normal(0.0, 0.02, len(pos))
+    types = list(a.types)
+    fixed = [2 if t == "H" else 0 for t in types]
+    inner = np.argsort(np.linalg.norm(pos[:, :2], axis=1))[:2]
+    for i in inner:
+        fixed[i] = 1
+    n0 = np.array([6.8, 1.2, 0.6])
+    types.append("N")
+    pos = np.vstack([pos, n0])
+    fixed.append(0)
+    for k in range(3):
+        ang = math.radians(120.0 * k + 15.0)
+        h = n0 + np.array([0.95 * math.cos(ang), 0.95 * math.sin(ang), 0.34])
+        types.append("H")
+        pos = np.vstack([pos, h])
+        fixed.append(0 if k else 2)
+    q = rng.normal(0.0, 0.08, len(types))
+    q -= q.mean()
+    return Molecule(types, pos, q, np.array(fixed))
+
+
+def make_fx(outdir: str, mini: bool = True, seed: int = 1240, **over) -> Workload:
+    mol = flex_molecule(seed)
+    opts = _base_options(flexible="on", coulomb="on", tip_dummy_coulomb="on", area="0.5 0.5",
+                         zhigh="9.0", zlow="8.0", dx="0.25", dy="0.25", center="0.1 -0.05")
+    if not mini:
+        opts.update(area="8.0 8.0", zlow="6.0", dx="0.2", dy="0.2")
+    opts.update({k: str(v) for k, v in over.items()})
+    return _emit("fx-mini" if mini else "fx", outdir, mol, opts, True, param_lines=FLEX_PARAM_LINES)
+
+
+EXAMPLE_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "golden",
+                           "example_graphene")
+
+
+def make_fx_example(outdir: str, example_dir: str = EXAMPLE_DIR, **over) -> Workload:
+    """The reference's own example job (examples/input.scan, graphene.xyz,
+    parameters.dat: 72 atoms, every one tethered in xy) with flexible on and
+    any option overridden; the files are the copies kept with the tests."""
+    os.makedirs(outdir, exist_ok=True)
+    opts: Dict[str, str] = {}
+    for line in open(os.path.join(example_dir, "input.scan")):
+        t = line.split()
+        if t and not line.startswith("#"):
+            opts[t[0]] = " ".join(t[1:])
+    opts["flexible"] = "on"
+    opts.update({k: str(v) for k, v in over.items()})
+    for fn in (opts["xyzfile"], opts["paramfile"]):
+        with open(os.path.join(example_dir, fn)) as fi, open(os.path.join(outdir, fn), "w") as fo:
+            fo.write(fi.read())
+    path = os.path.join(outdir, "input.scan")
+    write_input(path, opts)
+    types, pos = [], []
+    for line in open(os.path.join(outdir, opts["xyzfile"])).read().splitlines()[2:]:
+        t = line.split()
+        if t:
+            types.append(t[0])
+            pos.append([float(v) for v in t[1:4]])
+    return Workload("fx-example", path, opts, Molecule(types, np.array(pos)))

@@ -18,8 +18,20 @@ def pytest_configure(config):
 
 
 def golden_cases():
-    """Cases with full-precision reference records (tests/golden/<case>/records.npz)."""
-    return sorted(d for d in os.listdir(GOLDEN) if os.path.exists(os.path.join(GOLDEN, d, "records.npz")))
+    """Rigid cases with full-precision reference records (tests/golden/<case>/records.npz)."""
+    return sorted(d for d in os.listdir(GOLDEN)
+                  if os.path.exists(os.path.join(GOLDEN, d, "records.npz")) and not d.startswith("fx_"))
+
+
+def flex_cases():
+    """Flexible-on cases (tests/golden/fx_*): records, the centre column's
+    relaxed atoms per height and, for some, whole-list forces."""
+    return sorted(d for d in os.listdir(GOLDEN) if d.startswith("fx_"))
+
+
+def flex_golden(case: str):
+    d = np.load(os.path.join(GOLDEN, case, "records.npz"))
+    return tuple(int(v) for v in d["shape"]), d["records"], d["state"]
 
 
 def materialize(case: str, dst: str) -> str:

@@ -135,6 +135,120 @@ def make_case(name: str) -> None:
         print(f"{name}: {rec.shape[0]} records, steps mean {rec[:, 3].mean():.1f} max {rec[:, 3].max():.0f}")
 
 
+# flexible on: name -> (maker, kwargs, perturbed configurations for the
+# whole-list force check).  Each case also keeps the centre column's relaxed
+# atoms at every height (state.npy: what System::makeXYZFile writes).
+FLEX_CASES = {
+    "fx_mini": (W.make_fx, {}, 4),
+    "fx_rk4": (W.make_fx, dict(integrator="rk4"), 0),
+    "fx_euler": (W.make_fx, dict(integrator="euler"), 0),
+    "fx_sd": (W.make_fx, dict(minimiser="SD", maxsteps="3000", area="0.25 0.25"), 0),
+    "fx_minterm_e": (W.make_fx, dict(minterm="e", etol="1e-8"), 0),
+    "fx_maxsteps": (W.make_fx, dict(maxsteps="40"), 0),
+    "fx_pbc": (W.make_fx, dict(coulomb="off", vdw_pbc="on", cell_a="16.0 0.0 0.0", cell_b="0.0 16.0 0.0",
+                               cell_c="0.0 0.0 30.0", center="7.9 8.1", area="0.5 0.5", zhigh="8.0",
+                               zlow="7.0"), 4),
+    "fx_morse": (lambda d, **k: W.make_fx(d, **k), {}, 4),
+    "fx_graphene": (W.make_fx_example, dict(area="1.0 1.0", zlow="9.0"), 4),
+}
+
+
+def morse_variant(work: str) -> None:
+    """fx_morse: Morse pair overwrites for the tip (O) against C, H and N and
+    for the C-N pairs between the two molecules."""
+    W.write_params(os.path.join(work, "parameters.dat"), W.BASE_ATOMS, W.HARM,
+                   W.morse_overwrites("O", ["C", "H", "N"]) + ["C N morse 0.118322 1.6000 3.798000"],
+                   W.FLEX_PARAM_LINES)
+
+
+def make_flex_full() -> None:
+    """fx_example_full: the reference's example job with flexible on at its
+    own size (41 x 31 columns x 41 heights, 72 free atoms): every position's
+    FIRE step count, Fz and tip energy, the centre column's relaxed atoms at
+    every height (the inputs are the example files kept with the tests), and
+    the reference's sensitivity profile: how far its own answer moves when
+    one input coordinate changes in the last printed digit."""
+    dst = os.path.join(SCALE_DIR, "fx_example_full")
+    if os.path.isdir(dst):
+        shutil.rmtree(dst)
+    os.makedirs(dst)
+    with tempfile.TemporaryDirectory() as work:
+        wl = W.make_fx_example(work)
+        rec_bin, st_bin = os.path.join(work, "records.bin"), os.path.join(work, "state.bin")
+        harness("scan", wl.input_path, rec_bin, st_bin)
+        raw = open(rec_bin, "rb").read()
+        hdr = np.frombuffer(raw[:16], dtype=np.int32)
+        rec = np.frombuffer(raw[16:], dtype=np.float64).reshape(-1, int(hdr[3]))
+        n_atoms = len(wl.molecule.types) + 2
+        state = np.fromfile(st_bin, dtype=np.float64).reshape(int(hdr[2]), n_atoms, 3)
+        # the reference's own sensitivity: the same job with one input
+        # coordinate changed in its last printed digit (+1e-5 A); per height,
+        # the fraction of positions keeping their iteration count and the
+        # 50 / 90 / 99 % quantiles and maximum of |dFz|
+        xyz = os.path.join(work, wl.options["xyzfile"])
+        lines = open(xyz).read().splitlines()
+        t = lines[6].split()
+        lines[6] = " ".join([t[0], f"{float(t[1]) + 1e-5:.5f}"] + t[2:])
+        open(xyz, "w").write("\n".join(lines) + "\n")
+        harness("scan", wl.input_path, rec_bin)
+        raw = open(rec_bin, "rb").read()
+        pert = np.frombuffer(raw[16:], dtype=np.float64).reshape(-1, int(hdr[3]))
+        nz = int(hdr[2])
+        k = np.arange(len(rec)) % nz
+        same = pert[:, 3] == rec[:, 3]
+        dfz = np.abs(pert[:, 9] - rec[:, 9])
+        sens = np.array([[same[k == kk].mean()] + list(np.quantile(dfz[k == kk], [0.5, 0.9, 0.99])) +
+                         [dfz[k == kk].max()] for kk in range(nz)])
+        np.savez_compressed(os.path.join(dst, "records.npz"), shape=hdr[:3], n_steps=rec[:, 3].astype(np.int32),
+                            fz=rec[:, 9], e=rec[:, 15], state=state, sensitivity=sens)
+    print(f"fx_example_full: {rec.shape[0]} records, {int(rec[:, 3].sum())} steps, "
+          f"{same.mean():.3f} keep their count under a 1e-5 A input change")
+
+
+def make_flex_case(name: str) -> None:
+    maker, kw, n_conf = FLEX_CASES[name]
+    dst = os.path.join(OUT, name)
+    if os.path.isdir(dst):
+        shutil.rmtree(dst)
+    with tempfile.TemporaryDirectory() as work:
+        wl = maker(work, **kw)
+        wdir = os.path.dirname(wl.input_path)
+        if name == "fx_morse":
+            morse_variant(wdir)
+        rec_bin, st_bin = os.path.join(wdir, "records.bin"), os.path.join(wdir, "state.bin")
+        harness("scan", wl.input_path, rec_bin, st_bin)
+        raw = open(rec_bin, "rb").read()
+        hdr = np.frombuffer(raw[:16], dtype=np.int32)
+        rec = np.frombuffer(raw[16:], dtype=np.float64).reshape(-1, int(hdr[3]))
+        n_atoms = len(wl.molecule.types) + 2
+        state = np.fromfile(st_bin, dtype=np.float64).reshape(int(hdr[2]), n_atoms, 3)
+        os.makedirs(dst)
+        for fn in sorted(os.listdir(wdir)):
+            if not fn.endswith(".bin"):
+                shutil.copy(os.path.join(wdir, fn), os.path.join(dst, fn))
+        np.savez_compressed(os.path.join(dst, "records.npz"), shape=hdr[:3], records=rec, state=state)
+        if n_conf:
+            # the surface as the scan starts (centred) perturbed by 0.05 A, tip 3.4 A above the centre
+            rng = np.random.default_rng(7)
+            base = state[0].copy()
+            cx, cy = (float(v) for v in wl.options.get("center", "0 0").split()) if "center" in wl.options \
+                else (0.0, 0.0)
+            confs = []
+            for c in range(n_conf):
+                p = base + rng.normal(0.0, 0.05, base.shape)
+                p[0] = (cx, cy, 9.0 + 0.1 * c)
+                p[1] = (cx + 0.1, cy - 0.05, 5.6 + 0.1 * c)
+                confs.append(p)
+            confs = np.array(confs)
+            pin, pout = os.path.join(wdir, "pos.bin"), os.path.join(wdir, "forces.bin")
+            confs.tofile(pin)
+            harness("forces", wl.input_path, pin, pout)
+            fe = np.fromfile(pout, dtype=np.float64).reshape(n_conf, 4 * n_atoms)
+            np.savez_compressed(os.path.join(dst, "forces.npz"), pos=confs,
+                                forces=fe[:, :3 * n_atoms].reshape(n_conf, n_atoms, 3), energies=fe[:, 3 * n_atoms:])
+    print(f"{name}: {rec.shape[0]} records, steps mean {rec[:, 3].mean():.1f} max {rec[:, 3].max():.0f}")
+
+
 # scale cases: name -> (generator name in mechafm_amd.workloads, kwargs, eval samples)
 SCALE = {
     # the periodic slab at full size (2000 atoms x 9 images, 160 x 160 x 80
@@ -232,9 +346,13 @@ def make_example() -> None:
 def main(names=None) -> None:
     if not os.path.exists(HARNESS):
         raise SystemExit("build the reference harness first: make -C oracle ref")
-    for name in names or list(CASES) + ["example_graphene"] + list(SCALE):
+    for name in names or list(CASES) + ["example_graphene"] + list(FLEX_CASES) + list(SCALE) + ["fx_example_full"]:
         if name == "example_graphene":
             make_example()
+        elif name == "fx_example_full":
+            make_flex_full()
+        elif name in FLEX_CASES:
+            make_flex_case(name)
         elif name in SCALE:
             make_scale_case(name)
         else:
