@@ -230,6 +230,76 @@ def adjoint_bench(tmp, args):
     print(json.dumps(out), flush=True)
 
 
+# algorithmic FP64 operations of one evaluation of each flexible term kind, counted
+# from the reference's formulas (interactions.cpp:15-51, 252-362); a libm call
+# (exp, acos, sin, atan) counts as one
+FLEX_TERM_FLOPS = {0: 24, 1: 30, 2: 16, 3: 16, 4: 19, 5: 71, 6: 137, 7: 24, 8: 12}
+FLEX_TERM_ARITY = {0: 2, 1: 2, 2: 2, 3: 2, 4: 2, 5: 3, 6: 4, 7: 1, 8: 1}
+FLEX_ATOM_STEP_FLOPS = 40      # integrator update, P term, FIRE velocity mix of one free atom
+
+
+def flexible_bench(tmp, args):
+    """The reference's example job with flexible on at its own size (72 free
+    atoms, 781 terms, 41 x 31 columns x 41 heights): every surface atom
+    relaxes with the tip (mafm_scan_flexible)."""
+    from mechafm_amd import abi, workloads as W
+    wl = W.make_fx_example(os.path.join(tmp, "fx"))
+    ctx = abi.Context(0)
+    sim = abi.Simulation(wl.input_path, ctx)
+    info = sim.info()
+    positions = info["nx"] * info["ny"] * info["nz"]
+    for _ in range(args.warmup):
+        sim.run("plain64")
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = sim.run("plain64")
+        kms.append(st["kernel_ms"])
+    wall = time.perf_counter() - t0
+    fx = sim.flex_system()
+    t = fx["terms"]
+    free = fx["fixed"] != 1
+    act = np.array([free[t["atom"][q][:FLEX_TERM_ARITY[int(t["kind"][q])]]].any() for q in range(len(t))])
+    term_flops = sum(FLEX_TERM_FLOPS[int(k)] for k in t["kind"][act])
+    slots = sum(FLEX_TERM_ARITY[int(k)] for k in t["kind"][act])
+    evals = EVALS_PER_ITER[1]
+    steps = st["total_steps"]
+    flops = steps * (evals * (term_flops + 3 * slots) + int(free.sum()) * FLEX_ATOM_STEP_FLOPS)
+    k = float(np.mean(kms)) / 1e3
+    out = {"metric": "flexible scan: relaxed tip positions/s (reference example, flexible on)",
+           "value": positions * args.steps / wall, "unit": "positions/s", "ms_per_step": wall / args.steps * 1e3,
+           "kernel_ms": k * 1e3, "dtype": "f64", "data": "the reference's examples/ job",
+           "config": {"workload": "examples/graphene.xyz, 72 free atoms, 781 terms, 41x31 px x 41 z, FIRE+midpoint",
+                      "positions": positions, "total_steps": steps, "terms": len(t),
+                      "workgroup_threads": st["block_threads"], "workgroups": st["grid_blocks"]},
+           "us_per_fire_step_per_column": k * 1e6 * st["grid_blocks"] / steps,
+           "roofline": {"bound": "valu", "achieved": flops / k / 1e12, "peak": FP64_VALU_PEAK / 1e12,
+                        "unit": "TFLOP/s", "frac": flops / k / FP64_VALU_PEAK, "traffic": None,
+                        "flops_per_launch": flops}}
+    if not args.no_cpu:
+        exe = os.path.join(ROOT, "oracle", "_ref", "mechafm-omp")
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        # every 4th pixel in x and y (dx = dy = 2.0 A): 11 x 8 columns, all heights
+        swl = W.make_fx_example(os.path.join(tmp, "fx_cpu"), dx="2.0", dy="2.0")
+        shape = W.scan_shape(swl.options)
+        n = shape[0] * shape[1] * shape[2]
+        t0 = time.perf_counter()
+        if os.path.exists(exe):
+            kind = "reference"
+            subprocess.run([exe, swl.input_path, os.path.join(tmp, "fx_cpu_out")], check=True,
+                           stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                           env=dict(os.environ, OMP_NUM_THREADS=str(threads)))
+        else:
+            kind = "port"
+            from oracle import oracle_flex as OF
+            OF.scan(OF.flex_setup(swl.input_path))
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n / el, "unit": "positions/s", "cores": threads, "kind": kind,
+                               "sample": f"{n} positions: every 4th pixel in x and y (dx = dy = 2.0 A), all "
+                                         f"heights; {el:.1f} s wall on {threads} threads"}
+    print(json.dumps(out), flush=True)
+
+
 def bar_compare(recs, prec_recs):
     """Per-position |dr| (max over x, y, z) and relative Fz error of recs
     against prec_recs, and the 1e-4 A / 1e-3 bar."""
@@ -350,6 +420,8 @@ def main():
                     help="FP32 sums: 0 blocks of 4-6 sites folded into FP64, 1 Kahan, 2 plain")
     ap.add_argument("--adjoint", action="store_true",
                     help="the adjoint workload (150 atoms, 128x128 px x 30 z): gradient timing and finite differences")
+    ap.add_argument("--flexible", action="store_true",
+                    help="time the flexible scan of the reference's example job at full size instead")
     ap.add_argument("--margins", action="store_true",
                     help="decision-margin analysis of the FP32 scan against FP64 (no timing line)")
     args = ap.parse_args()
@@ -374,6 +446,12 @@ def main():
     if args.adjoint:
         try:
             adjoint_bench(tmp, args)
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+        return
+    if args.flexible:
+        try:
+            flexible_bench(tmp, args)
         finally:
             shutil.rmtree(tmp, ignore_errors=True)
         return

@@ -1,5 +1,6 @@
 /*
- * mafm.h -- C ABI of libmafm, the MI355X (gfx950) rigid probe-particle scan.
+ * mafm.h -- C ABI of libmafm, the MI355X (gfx950) probe-particle scan (rigid
+ * and flexible surfaces).
  *
  * Plain C types only: host pointers are borrowed for the duration of a call,
  * every entry point returns mafm_status (no exit(), no exceptions cross the
@@ -145,7 +146,10 @@ enum { MAFM_PREC_PLAIN64 = 0, MAFM_PREC_FAST64 = 1, MAFM_PREC_FAST32 = 2 };
  * taken on FP32 pair sums flips against the FP64 trajectory when its
  * margin is below the FP32 error of the tested quantity; the margins show
  * where. */
-enum { MAFM_SCAN_REPRODUCIBLE = 1, MAFM_SCAN_MARGINS = 4 };
+/* STATE_IN_HBM (mafm_scan_flexible): keep each column's state in HBM even
+ * where it fits one workgroup's LDS -- the path systems too large for LDS
+ * take; leaves the LDS to other work. */
+enum { MAFM_SCAN_REPRODUCIBLE = 1, MAFM_SCAN_MARGINS = 4, MAFM_SCAN_STATE_IN_HBM = 8 };
 
 typedef struct {
     int32_t nx, ny, nz;

@@ -20,6 +20,7 @@ STATUS = {0: "OK", 1: "INVALID", 2: "HIP", 3: "OUT_OF_GRID", 4: "STATE", 5: "FFT
 PREC = {"plain64": 0, "fast64": 1, "fast32": 2}
 SCAN_REPRODUCIBLE = 1
 SCAN_MARGINS = 4
+SCAN_STATE_IN_HBM = 8
 VDW_LJ, VDW_MORSE = 0, 1
 
 

@@ -120,6 +120,7 @@ struct mafm_ctx {
     std::string err;
     int cus = 0;
     size_t lds_per_cu = 0;   // LDS bytes per compute unit
+    size_t lds_per_block = 0;   // LDS bytes one workgroup may allocate
     std::vector<HostGrid> grids;
     DevBuf<DevGrid> d_grids;
     std::vector<DevMol> mols;
@@ -487,9 +488,11 @@ mafm_status mafm_create(int device, mafm_ctx** out) {
     if (hipGetDeviceProperties(&p, device) == hipSuccess) {
         c->cus = p.multiProcessorCount;
         c->lds_per_cu = p.maxSharedMemoryPerMultiProcessor;
+        c->lds_per_block = p.sharedMemPerBlock;
     }
     if (c->cus <= 0) c->cus = 256;
     if (c->lds_per_cu < 64 * 1024) c->lds_per_cu = 64 * 1024;
+    if (c->lds_per_block < 64 * 1024) c->lds_per_block = 64 * 1024;
     *out = c;
     return MAFM_OK;
 }
@@ -890,8 +893,11 @@ mafm_status mafm_scan_flexible(mafm_ctx* c, const mafm_flex_desc* s, const mafm_
             return fail(c, MAFM_ERR_INVALID, "term %d is a tip-surface term after the first non-tip-surface term", t);
     }
     // terms acting on a free atom, their result slots, and each free atom's slots in list order
-    std::vector<int32_t> eval_term, eval_slot, csr_off(n + 1, 0), csr;
-    int n_slots = 0, ts_slots = 0;
+    // the terms acting on a free atom (in list order, each with its first
+    // result slot), each free atom's slots in list order, the tip's terms
+    std::vector<mafm_flex_term> eval;
+    std::vector<int32_t> csr_off(n + 1, 0), csr, tip_eval;
+    int n_slots = 0, ts_slots = 0, n_ts_eval = 0;
     std::vector<std::vector<int32_t>> per_atom(n);
     for (int t = 0; t < s->n_terms; ++t) {
         const mafm_flex_term& T = s->terms[t];
@@ -899,27 +905,29 @@ mafm_status mafm_scan_flexible(mafm_ctx* c, const mafm_flex_desc* s, const mafm_
         bool any_free = false;
         for (int r = 0; r < ar; ++r) any_free |= s->fixed[T.atom[r]] != 1;
         if (!any_free) continue;
-        eval_term.push_back(t);
-        eval_slot.push_back(n_slots);
+        const int k = (int)eval.size();
+        eval.push_back(T);
+        eval.back().reserved = n_slots;
         const int* order = flex_update_order(T.kind);
         for (int q = 0; q < ar; ++q) {
             const int r = order[q];
             if (s->fixed[T.atom[r]] != 1) per_atom[T.atom[r]].push_back(n_slots + r);
+            if (T.atom[r] == 1) tip_eval.push_back(k);   // energies_[1] += e
         }
         n_slots += ar;
-        if (t < n_ts) ts_slots = n_slots;
+        if (t < n_ts) {
+            ts_slots = n_slots;
+            n_ts_eval = k + 1;
+        }
     }
     for (int i = 0; i < n; ++i) {
         csr_off[i + 1] = csr_off[i] + (int32_t)per_atom[i].size();
         csr.insert(csr.end(), per_atom[i].begin(), per_atom[i].end());
     }
-    const int n_eval = (int)eval_term.size();
-    const bool rk4 = d->minimiser == MAFM_MIN_FIRE && d->integrator == MAFM_INT_RK4;
-    const size_t block_bytes = flex_block_doubles(n, n_slots, rk4) * sizeof(double);
-    const bool in_lds = block_bytes + 64 <= 64 * 1024;
-    int per_cu = in_lds ? (int)std::min<size_t>(16, c->lds_per_cu / block_bytes) : 8;
-    per_cu = std::max(per_cu, 1);
-    const int n_blocks = (int)std::min<long>(n_cols, (long)c->cus * per_cu);
+    const int n_eval = (int)eval.size();
+    int tip_ts_slots = 0, tip_ts_terms = 0;
+    for (int32_t sl : per_atom[1]) tip_ts_slots += sl < ts_slots;
+    for (int32_t k : tip_eval) tip_ts_terms += k < n_ts_eval;
     const size_t n_rec = (size_t)n_cols * d->nz;
 
     HIPCHK(c, hipSetDevice(c->device));
@@ -930,21 +938,51 @@ mafm_status mafm_scan_flexible(mafm_ctx* c, const mafm_flex_desc* s, const mafm_
     c->last_flex = false;
     c->fx_state_atoms = c->fx_state_heights = 0;
     std::vector<int32_t> ints;
-    ints.reserve(2 * n_eval + (n + 1) + csr.size() + n);
-    ints.insert(ints.end(), eval_term.begin(), eval_term.end());
-    ints.insert(ints.end(), eval_slot.begin(), eval_slot.end());
+    ints.reserve((n + 1) + csr.size() + tip_eval.size() + n);
     ints.insert(ints.end(), csr_off.begin(), csr_off.end());
     ints.insert(ints.end(), csr.begin(), csr.end());
+    ints.insert(ints.end(), tip_eval.begin(), tip_eval.end());
     ints.insert(ints.end(), s->fixed, s->fixed + n);
-    HIPCHK(c, c->d_fx_terms.reserve(std::max(s->n_terms, 1)));
+
+    FlexArgs a;
+    memset(&a, 0, sizeof a);
+    a.n_atoms = n; a.n_eval = n_eval; a.n_ts = n_ts_eval;
+    a.n_slots = n_slots; a.ts_slots = ts_slots;
+    a.n_csr = (int32_t)csr.size();
+    a.n_tip = (int32_t)tip_eval.size();
+    a.tip_ts_slots = tip_ts_slots;
+    a.tip_ts_terms = tip_ts_terms;
+    a.nx = d->nx; a.ny = d->ny; a.nz = d->nz;
+    a.dx = d->dx; a.dy = d->dy; a.dz = d->dz; a.zhigh = d->zhigh;
+    a.minimiser = d->minimiser; a.integrator = d->integrator; a.minterm = d->minterm;
+    a.maxsteps = d->maxsteps; a.etol = d->etol; a.ftol = d->ftol; a.dt = d->dt;
+    a.tip_pbc = s->tip_pbc ? 1 : 0;
+    a.state_col = s->state_column;
+    memcpy(a.cell, s->cell, sizeof a.cell);
+    if (a.tip_pbc) inv3(a.cell, a.cell_inv);
+    memcpy(a.offset, s->offset, sizeof a.offset);
+    a.tip_dummy_d = s->tip_dummy_d;
+    const bool rk4 = d->minimiser == MAFM_MIN_FIRE && d->integrator == MAFM_INT_RK4;
+    // the column state in LDS when one workgroup may hold it (gfx950: up to
+    // 160 KiB), as many columns per CU as its LDS holds (up to 8)
+    const size_t block_bytes = flex_block_lds(a) + 64;
+    const bool in_lds = !(d->options & MAFM_SCAN_STATE_IN_HBM) && block_bytes <= c->lds_per_block;
+    // up to 4 waves share a column (terms and atoms spread over 64-256 threads);
+    // two resident waves per SIMD, and as many columns per CU as its LDS holds
+    const int waves = std::max(n_eval, n) > 128 ? 4 : (std::max(n_eval, n) > 64 ? 2 : 1);
+    int per_cu = 8 / waves;
+    if (in_lds) per_cu = (int)std::max<size_t>(1, std::min<size_t>(per_cu, c->lds_per_cu / block_bytes));
+    const int n_blocks = (int)std::min<long>(n_cols, (long)c->cus * per_cu);
+
+    HIPCHK(c, c->d_fx_terms.reserve(std::max(n_eval, 1)));
     HIPCHK(c, c->d_fx_ints.reserve(ints.size()));
     HIPCHK(c, c->d_fx_pos.reserve(3 * (size_t)n));
     HIPCHK(c, c->d_rec.reserve(n_rec));
     HIPCHK(c, c->d_next.reserve(1));
     if (s->state_column >= 0) HIPCHK(c, c->d_fx_state.reserve((size_t)d->nz * n * 3));
-    if (!in_lds) HIPCHK(c, c->d_fx_work.reserve((size_t)n_blocks * flex_block_doubles(n, n_slots, rk4)));
-    if (s->n_terms > 0)
-        HIPCHK(c, hipMemcpyAsync(c->d_fx_terms.p, s->terms, s->n_terms * sizeof(mafm_flex_term),
+    if (!in_lds) HIPCHK(c, c->d_fx_work.reserve((size_t)n_blocks * flex_block_doubles(n, n_slots, n_eval, rk4)));
+    if (n_eval > 0)
+        HIPCHK(c, hipMemcpyAsync(c->d_fx_terms.p, eval.data(), n_eval * sizeof(mafm_flex_term),
                                  hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_fx_ints.p, ints.data(), ints.size() * sizeof(int32_t), hipMemcpyHostToDevice,
                              c->stream));
@@ -953,35 +991,16 @@ mafm_status mafm_scan_flexible(mafm_ctx* c, const mafm_flex_desc* s, const mafm_
     HIPCHK(c, hipMemsetAsync(c->d_next.p, 0, sizeof(int32_t), c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_counters.p, 0, 8 * sizeof(int64_t), c->stream));
 
-    FlexArgs a;
-    memset(&a, 0, sizeof a);
-    a.n_atoms = n; a.n_eval = n_eval; a.n_ts = 0;
-    for (int k = 0; k < n_eval && eval_term[k] < n_ts; ++k) a.n_ts = k + 1;
-    a.n_slots = n_slots; a.ts_slots = ts_slots;
-    a.nx = d->nx; a.ny = d->ny; a.nz = d->nz;
-    a.dx = d->dx; a.dy = d->dy; a.dz = d->dz; a.zhigh = d->zhigh;
-    a.minimiser = d->minimiser; a.integrator = d->integrator; a.minterm = d->minterm;
-    a.maxsteps = d->maxsteps; a.etol = d->etol; a.ftol = d->ftol; a.dt = d->dt;
     a.terms = c->d_fx_terms.p;
-    a.eval_term = c->d_fx_ints.p;
-    a.eval_slot = a.eval_term + n_eval;
-    a.csr_off = a.eval_slot + n_eval;
-    a.csr = a.csr_off + (n + 1);
-    a.fixed = a.csr + csr.size();
+    a.idx = c->d_fx_ints.p;
     a.pos0 = c->d_fx_pos.p;
-    a.tip_pbc = s->tip_pbc ? 1 : 0;
-    a.state_col = s->state_column;
-    memcpy(a.cell, s->cell, sizeof a.cell);
-    if (a.tip_pbc) inv3(a.cell, a.cell_inv);
-    memcpy(a.offset, s->offset, sizeof a.offset);
-    a.tip_dummy_d = s->tip_dummy_d;
     a.rec = c->d_rec.p;
     a.next_col = c->d_next.p;
     a.state = s->state_column >= 0 ? c->d_fx_state.p : nullptr;
     a.work = in_lds ? nullptr : c->d_fx_work.p;
 
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    HIPCHK(c, launch_scan_flex(a, n_blocks, in_lds, c->stream));
+    HIPCHK(c, launch_scan_flex(a, n_blocks, waves, in_lds, c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, launch_record_totals(c->d_rec.p, (long)n_rec, c->d_counters.p, c->stream));
     int64_t counters[8];
@@ -1006,9 +1025,9 @@ mafm_status mafm_scan_flexible(mafm_ctx* c, const mafm_flex_desc* s, const mafm_
         const int64_t evals = d->minimiser == MAFM_MIN_SD ? 1 : (d->integrator == 1 ? 2 : (d->integrator == 2 ? 4 : 1));
         // term evaluations: every evaluated term per integrator stage, plus the tip-surface terms of each output
         stats->pair_evals = counters[0] * evals * n_eval + (int64_t)n_rec * a.n_ts;
-        stats->lanes_per_pixel = 64;
+        stats->lanes_per_pixel = 64 * waves;
         stats->grid_blocks = n_blocks;
-        stats->block_threads = 64;
+        stats->block_threads = 64 * waves;
     }
     return MAFM_OK;
 }

@@ -7,10 +7,10 @@
 namespace mafm {
 
 // Every term is evaluated once per integrator stage into result slots, one
-// slot {fx, fy, fz, e} per atom the term acts on, holding the signed force
-// that atom receives (the reference's `forces[i] -= f` is `+= -f`, the same
-// rounding).  Each free atom then sums its slots in the interaction list's
-// order.
+// slot {fx, fy, fz} per atom the term acts on, holding the signed force that
+// atom receives (the reference's `forces[i] -= f` is `+= -f`, the same
+// rounding), plus the term's energy.  Each free atom then sums its slots in
+// the interaction list's order; the tip sums the energies of its terms.
 struct FlexArgs {
     int32_t n_atoms, n_eval, n_ts;    // eval entries [0, n_ts) are the tip-surface terms
     int32_t n_slots, ts_slots;        // slots [0, ts_slots) belong to the tip-surface terms
@@ -18,13 +18,16 @@ struct FlexArgs {
     double dx, dy, dz, zhigh;
     int32_t minimiser, integrator, minterm, maxsteps;
     double etol, ftol, dt;
+    // the terms that act on a free atom, in list order, each with its first
+    // result slot in `reserved`
     const mafm_flex_term* terms;
-    const int32_t* eval_term;         // n_eval: terms that act on a free atom, in list order
-    const int32_t* eval_slot;         // n_eval: first result slot of each
-    const int32_t* csr_off;           // n_atoms + 1 (empty ranges for fixed atoms)
-    const int32_t* csr;               // result slots of each free atom, in list order
+    // index tables, back to back: csr_off (n_atoms + 1, empty ranges for fixed
+    // atoms), csr (n_csr: the result slots of each free atom in list order),
+    // tip_eval (n_tip: the terms acting on the tip, in list order), fixed (n_atoms)
+    const int32_t* idx;
+    int32_t n_csr, n_tip;
+    int32_t tip_ts_slots, tip_ts_terms;   // the tip's slots / terms from tip-surface terms
     const double* pos0;               // n_atoms x 3
-    const int32_t* fixed;             // n_atoms
     int32_t tip_pbc;
     int32_t state_col;                // column whose positions are kept per height, -1 none
     double cell[9], cell_inv[9], offset[3];
@@ -37,8 +40,11 @@ struct FlexArgs {
 
 // Doubles of per-column state one block keeps (positions, integrator
 // stages, velocities, forces, term results).
-size_t flex_block_doubles(int n_atoms, int n_slots, bool rk4);
-// in_lds: the state lives in LDS (else in a.work, one slice per block).
-hipError_t launch_scan_flex(const FlexArgs& a, int n_blocks, bool in_lds, hipStream_t st);
+size_t flex_block_doubles(int n_atoms, int n_slots, int n_eval, bool rk4);
+// LDS bytes of one block when the state lives there (state + index tables).
+size_t flex_block_lds(const FlexArgs& a);
+// waves: wavefronts per workgroup (1-4); in_lds: the state lives in LDS (else
+// in a.work, one slice per block).
+hipError_t launch_scan_flex(const FlexArgs& a, int n_blocks, int waves, bool in_lds, hipStream_t st);
 
 }  // namespace mafm

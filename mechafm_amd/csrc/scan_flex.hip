@@ -1,23 +1,27 @@
 // Flexible scan (flexible on): every free surface atom relaxes with the tip.
 //
-// One wavefront relaxes one pixel column at a time (persistent grid, columns
-// handed out by an atomic counter).  The column's whole state lives with the
-// wave -- in LDS when it fits, else in a per-block slice of HBM: positions,
-// the integrator's evaluation point and stage velocities, velocities, forces
-// and the term results.
+// One workgroup of 1-4 wavefronts relaxes one pixel column at a time
+// (persistent grid, columns handed out by an atomic counter).  The column's
+// whole state lives with the workgroup -- in LDS when it fits (gfx950: up to
+// 160 KiB), else in a per-block slice of HBM: positions, the integrator's
+// evaluation point and stage velocities, velocities, forces, the term
+// results and the index tables.
 //
 // One integrator stage evaluates the interaction list in two phases:
-//   1. term-parallel: lane l evaluates terms l, l + 64, ... and writes one
-//      result slot {fx, fy, fz, e} per atom the term acts on, the force that
+//   1. term-parallel: thread l evaluates terms l, l + T, ... and writes one
+//      result slot {fx, fy, fz} per atom the term acts on, the force that
 //      atom receives (interactions.cpp:15-51, 252-362; `forces[i] -= f` is
-//      stored as -f, which adds with the same rounding);
-//   2. atom-parallel: lane l sums the slots of atoms l, l + 64, ... in the
+//      stored as -f, which adds with the same rounding), and the term's
+//      energy once (only the tip's energy is ever read);
+//   2. atom-parallel: thread l sums the slots of its atoms in the
 //      interaction list's order, from zero -- the order in which the
 //      reference's Interaction::eval calls accumulate forces_[i] and
 //      energies_[i] (integrators.cpp:8-111) -- so each force is the
-//      reference's to the last bit given the same term arithmetic.
-// FIRE's power P = sum_i v_i . f_i (minimiser.cpp:107-109) is summed by one
-// lane in atom order.  Fixed atoms (fixed = 1) never move and have zero
+//      reference's to the last bit given the same term arithmetic.  The tip,
+//      whose list holds every tip-surface term, is summed by the block's last
+//      thread, whose wave has no other atom when the block has several waves.
+// FIRE's power P = sum_i v_i . f_i (minimiser.cpp:107-109) is summed in atom
+// order by one thread.  Fixed atoms (fixed = 1) never move and have zero
 // velocity, so their forces (which the reference computes and multiplies by
 // that zero velocity in P) are not summed.  Built with -ffp-contract=off;
 // IEEE divisions and square roots.  The angle, dihedral and Morse terms call
@@ -48,34 +52,31 @@ __device__ inline V3 cross(V3 a, V3 b) {                                        
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
-__device__ inline void put(double* R, int slot, V3 f, double e) {
-    R[4 * slot] = f.x;
-    R[4 * slot + 1] = f.y;
-    R[4 * slot + 2] = f.z;
-    R[4 * slot + 3] = e;
+__device__ inline void put(double* R, int slot, V3 f) {
+    R[3 * slot] = f.x;
+    R[3 * slot + 1] = f.y;
+    R[3 * slot + 2] = f.z;
 }
 
-// One Interaction::eval of the list at positions X, into the term's slots R
-// (slot r = the term's atom[r]).
-__device__ void eval_term(const mafm_flex_term& t, const double* X, double* R) {
+// One Interaction::eval of the list at positions X: the force on each atom
+// into the term's slots R (slot r = the term's atom[r]), the energy into e.
+__device__ void eval_term(const mafm_flex_term& t, const double* X, double* R, double& e_out) {
     switch (t.kind) {
         case MAFM_FX_LJ:
         case MAFM_FX_MORSE: {   // interactions.cpp:15-40
             const V3 p2 = ld3(X, t.atom[1]);
             const V3 r = sub(ld3(X, t.atom[0]), add(p2, {t.shift[0], t.shift[1], t.shift[2]}));
             V3 f;
-            double e;
-            vdw(t.kind == MAFM_FX_LJ ? 0 : 1, t.p[0], t.p[1], t.p[2], r, f, e);
-            put(R, 0, f, e);
-            put(R, 1, neg(f), e);
+            vdw(t.kind == MAFM_FX_LJ ? 0 : 1, t.p[0], t.p[1], t.p[2], r, f, e_out);
+            put(R, 0, f);
+            put(R, 1, neg(f));
             return;
         }
         case MAFM_FX_COULOMB: {   // interactions.cpp:42-51
             V3 f;
-            double e;
-            coulomb(t.p[0], sub(ld3(X, t.atom[0]), ld3(X, t.atom[1])), f, e);
-            put(R, 0, f, e);
-            put(R, 1, neg(f), e);
+            coulomb(t.p[0], sub(ld3(X, t.atom[0]), ld3(X, t.atom[1])), f, e_out);
+            put(R, 0, f);
+            put(R, 1, neg(f));
             return;
         }
         case MAFM_FX_TIP_HARMONIC: {   // interactions.cpp:252-266
@@ -88,8 +89,9 @@ __device__ void eval_term(const mafm_flex_term& t, const double* X, double* R) {
                 const double s = -2 * t.p[0] * dr / rl;
                 f = {s * r.x, s * r.y, 0};
             }
-            put(R, 0, f, e);
-            put(R, 1, neg(f), e);
+            e_out = e;
+            put(R, 0, f);
+            put(R, 1, neg(f));
             return;
         }
         case MAFM_FX_BOND: {   // HarmonicInteraction::eval, interactions.cpp:296-306
@@ -98,8 +100,9 @@ __device__ void eval_term(const mafm_flex_term& t, const double* X, double* R) {
             const double dr = rl - t.p[1];
             const double e = t.p[0] * sq(dr);
             const V3 f = scl(-2 * t.p[0] * dr / rl, r);
-            put(R, 0, f, e);
-            put(R, 1, neg(f), e);
+            e_out = e;
+            put(R, 0, f);
+            put(R, 1, neg(f));
             return;
         }
         case MAFM_FX_ANGLE: {   // HarmonicAngleInteraction::eval, interactions.cpp:308-332
@@ -115,10 +118,10 @@ __device__ void eval_term(const mafm_flex_term& t, const double* X, double* R) {
             const double fm = -2 * t.p[0] * d_theta / sin_t;
             const V3 f1 = scl(-fm / r1, sub(dvs(r2v, r2), dvs(scr(r1v, cos_t), r1)));
             const V3 f2 = scl(-fm / r2, sub(dvs(r1v, r1), dvs(scr(r2v, cos_t), r2)));
-            const double e = t.p[0] * sq(d_theta);
-            put(R, 0, neg(add(f1, f2)), e);   // forces[shared] -= f1 + f2
-            put(R, 1, f1, e);
-            put(R, 2, f2, e);
+            e_out = t.p[0] * sq(d_theta);
+            put(R, 0, neg(add(f1, f2)));   // forces[shared] -= f1 + f2
+            put(R, 1, f1);
+            put(R, 2, f2);
             return;
         }
         case MAFM_FX_DIHEDRAL: {   // HarmonicDihedralInteraction::eval, interactions.cpp:334-362
@@ -131,13 +134,13 @@ __device__ void eval_term(const mafm_flex_term& t, const double* X, double* R) {
             const double sigma = atan(tan_s);
             const double d_sigma = sigma - t.p[1];
             const double fm = 2 * t.p[0] * d_sigma;
-            const double e = t.p[0] * sq(d_sigma);
-            put(R, 0, scl(-fm * r23l / (m * m), mv), e);
+            e_out = t.p[0] * sq(d_sigma);
+            put(R, 0, scl(-fm * r23l / (m * m), mv));
             put(R, 1, scl(-fm, sub(scl(dot(r43, r23) / (n * n * r23l), nv),
-                                   scl(((r23l * r23l) + dot(r12, r23)) / (m * m * r23l), mv))), e);
+                                   scl(((r23l * r23l) + dot(r12, r23)) / (m * m * r23l), mv))));
             put(R, 2, scl(-fm, add(scl(dot(r12, r23) / (m * m * r23l), mv),
-                                   scl(((r23l * r23l) - dot(r43, r23)) / (n * n * r23l), nv))), e);
-            put(R, 3, scl(fm * r23l / (n * n), nv), e);
+                                   scl(((r23l * r23l) - dot(r43, r23)) / (n * n * r23l), nv))));
+            put(R, 3, scl(fm * r23l / (n * n), nv));
             return;
         }
         case MAFM_FX_SUBSTRATE: {   // SubstrateInteraction::eval, interactions.cpp:280-294
@@ -153,7 +156,8 @@ __device__ void eval_term(const mafm_flex_term& t, const double* X, double* R) {
                 fterm += 4 * t.p[0] / dz * (sig_z10 - sig_z4) + 2 * t.p[1] * dz;
             }
             // forces[i].z += fterm: the +0 added to x and y leaves them as they are
-            put(R, 0, {0, 0, fterm}, eterm);
+            put(R, 0, {0, 0, fterm});
+            e_out = eterm;
             return;
         }
         default: {   // MAFM_FX_XY_HARMONIC, interactions.cpp:268-278: p = k, x0, y0
@@ -162,7 +166,8 @@ __device__ void eval_term(const mafm_flex_term& t, const double* X, double* R) {
             const double e = t.p[0] * rl * rl;
             V3 f = {0, 0, 0};
             if (rl > TOL) f = {-2 * t.p[0] * rx, -2 * t.p[0] * ry, 0};
-            put(R, 0, f, e);
+            e_out = e;
+            put(R, 0, f);
             return;
         }
     }
@@ -177,15 +182,16 @@ __device__ inline bool converged(const FlexArgs& a, V3 f, double ediff) {
     }
 }
 
-__host__ __device__ inline size_t block_doubles(int n, int n_slots, bool rk4) {
-    return (size_t)(16 + (rk4 ? 6 : 0)) * n + 4 * (size_t)n_slots + 8;
+__host__ __device__ inline size_t block_doubles(int n, int n_slots, int n_eval, bool rk4) {
+    return (size_t)(16 + (rk4 ? 6 : 0)) * n + 3 * (size_t)n_slots + n_eval + 8;
 }
 
 struct State {
-    double *pos, *pev, *vel, *vst, *frc, *pdot, *res, *bc, *facc, *vacc;
+    double *pos, *pev, *vel, *vst, *frc, *pdot, *res, *ener, *bc, *facc, *vacc;
+    const int32_t *csr_off, *csr, *tip_eval, *fixed;   // the index tables (LDS copy or global)
 };
 
-__device__ inline State layout(double* p, int n, int n_slots, bool rk4) {
+__device__ inline State layout(double* p, int n, int n_slots, int n_eval, bool rk4) {
     State S;
     S.pos = p; p += 3 * n;
     S.pev = p; p += 3 * n;
@@ -193,68 +199,134 @@ __device__ inline State layout(double* p, int n, int n_slots, bool rk4) {
     S.vst = p; p += 3 * n;
     S.frc = p; p += 3 * n;
     S.pdot = p; p += n;
-    S.res = p; p += 4 * n_slots;
+    S.res = p; p += 3 * n_slots;
+    S.ener = p; p += n_eval;
     S.bc = p; p += 8;
     S.facc = p; p += rk4 ? 3 * n : 0;
     S.vacc = p;
     return S;
 }
 
-__device__ inline void wave_sync() { __syncthreads(); }   // one wave per workgroup
+__device__ inline void block_sync() { __syncthreads(); }
+
+// acc += the slots res[idx[q]] for q in [q0, q1), in order; the loads of
+// four slots are issued before their adds
+__device__ inline V3 sum_slots(const double* res, const int32_t* idx, int q0, int q1) {
+    V3 acc = {0, 0, 0};
+    int q = q0;
+    for (; q + 4 <= q1; q += 4) {
+        const int s0 = idx[q], s1 = idx[q + 1], s2 = idx[q + 2], s3 = idx[q + 3];
+        const V3 r0 = ld3(res, s0), r1 = ld3(res, s1), r2 = ld3(res, s2), r3 = ld3(res, s3);
+        acc = add(acc, r0);
+        acc = add(acc, r1);
+        acc = add(acc, r2);
+        acc = add(acc, r3);
+    }
+    for (; q < q1; ++q) acc = add(acc, ld3(res, idx[q]));
+    return acc;
+}
+
+// The long ordered sums -- the tip's list, which holds every tip-surface
+// term, and FIRE's power over all atoms -- by one thread: eight values are
+// loaded before their eight additions, so the chain waits on the adds, not
+// on LDS.
+__device__ inline V3 seq_sum_slots(const double* res, const int32_t* idx, int q0, int q1) {
+    V3 acc = {0, 0, 0};
+    int q = q0;
+    for (; q + 8 <= q1; q += 8) {
+        V3 r[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) r[j] = ld3(res, idx[q + j]);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc = add(acc, r[j]);
+    }
+    for (; q < q1; ++q) acc = add(acc, ld3(res, idx[q]));
+    return acc;
+}
+
+// sum of v[idx[q]] (idx NULL: v[q]) for q in [0, n), in order
+__device__ inline double seq_sum(const double* v, const int32_t* idx, int n) {
+    double acc = 0;
+    int q = 0;
+    for (; q + 8 <= n; q += 8) {
+        double r[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) r[j] = v[idx ? idx[q + j] : q + j];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc += r[j];
+    }
+    for (; q < n; ++q) acc += v[idx ? idx[q] : q];
+    return acc;
+}
 
 // Forces on every free atom at the positions X (into S.frc) and the tip's
 // energy (into S.bc[0]): the interaction list evaluated as one integrator
 // stage does (integrators.cpp:31-33).  ts_only: only the tip-surface terms,
 // for the tip, as System::evalTipSurfaceForces does (system.cpp:41-51).
-__device__ void evaluate(const FlexArgs& a, const State& S, const double* X, int lane, bool ts_only) {
+// t / nthr: this thread and the block's threads; the block's last thread
+// sums the tip.
+__device__ void evaluate(const FlexArgs& a, const State& S, const double* X, int t, int nthr, bool ts_only) {
     const int ne = ts_only ? a.n_ts : a.n_eval;
-    for (int k = lane; k < ne; k += 64) eval_term(a.terms[a.eval_term[k]], X, S.res + 4 * a.eval_slot[k]);
-    wave_sync();
-    for (int i = ts_only ? (lane == 1 ? 1 : a.n_atoms) : lane; i < a.n_atoms; i += 64) {
-        if (a.fixed[i] == 1) {
-            st3(S.frc, i, {0, 0, 0});
-            continue;
-        }
-        V3 acc = {0, 0, 0};
-        double eacc = 0;
-        const int q1 = a.csr_off[i + 1];
-        for (int q = a.csr_off[i]; q < q1; ++q) {
-            const int s = a.csr[q];
-            if (ts_only && s >= a.ts_slots) break;
-            acc = add(acc, {S.res[4 * s], S.res[4 * s + 1], S.res[4 * s + 2]});
-            eacc += S.res[4 * s + 3];
-        }
-        st3(S.frc, i, acc);
-        if (i == 1) S.bc[0] = eacc;
-        if (ts_only) break;
+    for (int k = t; k < ne; k += nthr) {
+        const mafm_flex_term& term = a.terms[k];
+        eval_term(term, X, S.res + 3 * term.reserved, S.ener[k]);   // reserved: the term's first slot
     }
-    wave_sync();
+    block_sync();
+    if (!ts_only)
+        for (int i = t; i < a.n_atoms; i += nthr) {
+            if (i == 1) continue;
+            if (S.fixed[i] == 1) st3(S.frc, i, {0, 0, 0});
+            else st3(S.frc, i, sum_slots(S.res, S.csr, S.csr_off[i], S.csr_off[i + 1]));
+        }
+    if (t == nthr - 1) {   // the tip: its slots (the tip-surface ones first) and its terms' energies
+        const int q0 = S.csr_off[1];
+        st3(S.frc, 1, seq_sum_slots(S.res, S.csr, q0, ts_only ? q0 + a.tip_ts_slots : S.csr_off[2]));
+        // the tip's energy is read by the output and by the energy convergence test only
+        if (ts_only || a.minterm != MAFM_TERM_F) S.bc[0] = seq_sum(S.ener, S.tip_eval, ts_only ? a.tip_ts_terms : a.n_tip);
+    }
+    block_sync();
 }
 
 template <bool IN_LDS>
-__global__ void __launch_bounds__(64) k_scan_flex(FlexArgs a) {
+__global__ void __launch_bounds__(256) k_scan_flex(FlexArgs a) {
     extern __shared__ double lds_flex[];
     __shared__ int col_shared;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x;   // thread of the block (1-4 waves share a column)
+    const int nthr = blockDim.x;
     const int n = a.n_atoms;
     const bool rk4 = a.minimiser == MAFM_MIN_FIRE && a.integrator == MAFM_INT_RK4;
-    double* base = IN_LDS ? lds_flex : a.work + (size_t)blockIdx.x * block_doubles(n, a.n_slots, rk4);
-    const State S = layout(base, n, a.n_slots, rk4);
+    const size_t nd = block_doubles(n, a.n_slots, a.n_eval, rk4);
+    double* base = IN_LDS ? lds_flex : a.work + (size_t)blockIdx.x * nd;
+    State S = layout(base, n, a.n_slots, a.n_eval, rk4);
+    // the index tables: csr_off (n + 1), csr, tip_eval, fixed (n); copied to LDS
+    // behind the state when it lives there
+    const int n_idx = (n + 1) + a.n_csr + a.n_tip + n;
+    const int32_t* idx = a.idx;
+    if (IN_LDS) {
+        int32_t* l = reinterpret_cast<int32_t*>(lds_flex + nd);
+        for (int q = lane; q < n_idx; q += nthr) l[q] = a.idx[q];
+        idx = l;
+    }
+    S.csr_off = idx;
+    S.csr = idx + (n + 1);
+    S.tip_eval = S.csr + a.n_csr;
+    S.fixed = S.tip_eval + a.n_tip;
+    block_sync();
     const int n_cols = a.nx * a.ny;
     for (;;) {
         if (lane == 0) col_shared = atomicAdd(a.next_col, 1);
-        wave_sync();
+        block_sync();
         const int col = col_shared;
-        wave_sync();
+        block_sync();
         if (col >= n_cols) break;
         const int ci = col / a.ny, cj = col - ci * a.ny;
         const double x = ci * a.dx, y = cj * a.dy;   // simulation.cpp:80-83
         // System copy (simulation.cpp:93): initial positions, zero velocities
-        for (int i = lane; i < n; i += 64) {
+        for (int i = lane; i < n; i += nthr) {
             st3(S.pos, i, ld3(a.pos0, i));
             st3(S.vel, i, {0, 0, 0});
         }
-        wave_sync();
+        block_sync();
         if (lane == 0) {
             // setDummyXY / setDummyZ (system.cpp:139-162, system.hpp:47-50)
             double px = x, py = y;
@@ -262,7 +334,7 @@ __global__ void __launch_bounds__(64) k_scan_flex(FlexArgs a) {
             st3(S.pos, 0, {px, py, a.zhigh});
             st3(S.pos, 1, {px, py, a.zhigh - a.tip_dummy_d});
         }
-        wave_sync();
+        block_sync();
         double e_tip = 0;   // energies_[1] of the System copy: 0 until its first step
         for (int k = 0; k < a.nz; ++k) {
             // FIREMinimisation / SDMinimisation (minimiser.cpp:39-129)
@@ -271,36 +343,36 @@ __global__ void __launch_bounds__(64) k_scan_flex(FlexArgs a) {
             for (; n_tot < a.maxsteps; ++n_tot) {
                 const double prev_e = e_tip;
                 if (a.minimiser == MAFM_MIN_SD) {   // evaluate, test, move
-                    evaluate(a, S, S.pos, lane, false);
+                    evaluate(a, S, S.pos, lane, nthr, false);
                     e_tip = S.bc[0];
                     if (converged(a, ld3(S.frc, 1), e_tip - prev_e)) break;
-                    for (int i = lane; i < n; i += 64)
-                        if (a.fixed[i] != 1) st3(S.pos, i, add(ld3(S.pos, i), scl(a.dt, ld3(S.frc, i))));
-                    wave_sync();
+                    for (int i = lane; i < n; i += nthr)
+                        if (S.fixed[i] != 1) st3(S.pos, i, add(ld3(S.pos, i), scl(a.dt, ld3(S.frc, i))));
+                    block_sync();
                     continue;
                 }
                 if (a.integrator == MAFM_INT_EULER) {   // eulerStep
-                    evaluate(a, S, S.pos, lane, false);
-                    for (int i = lane; i < n; i += 64)
-                        if (a.fixed[i] != 1) {
+                    evaluate(a, S, S.pos, lane, nthr, false);
+                    for (int i = lane; i < n; i += nthr)
+                        if (S.fixed[i] != 1) {
                             st3(S.pos, i, add(ld3(S.pos, i), scl(dt, ld3(S.vel, i))));
                             st3(S.vel, i, add(ld3(S.vel, i), scl(dt, ld3(S.frc, i))));   // masses are 1
                         }
                     e_tip = S.bc[0];
                 } else if (a.integrator == MAFM_INT_MIDPOINT) {   // midpointStep
-                    evaluate(a, S, S.pos, lane, false);
-                    for (int i = lane; i < n; i += 64) {
-                        if (a.fixed[i] != 1) {
+                    evaluate(a, S, S.pos, lane, nthr, false);
+                    for (int i = lane; i < n; i += nthr) {
+                        if (S.fixed[i] != 1) {
                             st3(S.pev, i, add(ld3(S.pos, i), scl(dt / 2, ld3(S.vel, i))));
                             st3(S.vst, i, add(ld3(S.vel, i), scl(dt / 2, ld3(S.frc, i))));
                         } else {
                             st3(S.pev, i, ld3(S.pos, i));
                         }
                     }
-                    wave_sync();
-                    evaluate(a, S, S.pev, lane, false);
-                    for (int i = lane; i < n; i += 64)
-                        if (a.fixed[i] != 1) {
+                    block_sync();
+                    evaluate(a, S, S.pev, lane, nthr, false);
+                    for (int i = lane; i < n; i += nthr)
+                        if (S.fixed[i] != 1) {
                             st3(S.pos, i, add(ld3(S.pos, i), scl(dt, ld3(S.vst, i))));
                             st3(S.vel, i, add(ld3(S.vel, i), scl(dt, ld3(S.frc, i))));
                         }
@@ -308,10 +380,10 @@ __global__ void __launch_bounds__(64) k_scan_flex(FlexArgs a) {
                 } else {   // rk4Step: stages at pos, pos + dt/2 v1, pos + dt/2 v2, pos + dt v3
                     double e_acc = 0;
                     for (int s = 0; s < 4; ++s) {
-                        evaluate(a, S, s == 0 ? S.pos : S.pev, lane, false);
+                        evaluate(a, S, s == 0 ? S.pos : S.pev, lane, nthr, false);
                         const double es = S.bc[0];
                         e_acc = s == 0 ? es : e_acc + (s == 3 ? es : 2 * es);
-                        for (int i = lane; i < n; i += 64) {
+                        for (int i = lane; i < n; i += nthr) {
                             const V3 fs = ld3(S.frc, i), vs = s == 0 ? ld3(S.vel, i) : ld3(S.vst, i);
                             if (s == 0) {
                                 st3(S.facc, i, fs);
@@ -325,7 +397,7 @@ __global__ void __launch_bounds__(64) k_scan_flex(FlexArgs a) {
                             }
                             if (s < 3) {
                                 const double h = s < 2 ? dt / 2 : dt;
-                                if (a.fixed[i] != 1) {
+                                if (S.fixed[i] != 1) {
                                     st3(S.pev, i, add(ld3(S.pos, i), scl(h, vs)));
                                     st3(S.vst, i, add(ld3(S.vel, i), scl(h, fs)));
                                 } else {
@@ -334,10 +406,10 @@ __global__ void __launch_bounds__(64) k_scan_flex(FlexArgs a) {
                                 }
                             }
                         }
-                        wave_sync();
+                        block_sync();
                     }
-                    for (int i = lane; i < n; i += 64) {
-                        if (a.fixed[i] != 1) {
+                    for (int i = lane; i < n; i += nthr) {
+                        if (S.fixed[i] != 1) {
                             st3(S.pos, i, add(ld3(S.pos, i), scl(dt / 6, ld3(S.vacc, i))));
                             st3(S.vel, i, add(ld3(S.vel, i), scl(dt / 6, ld3(S.facc, i))));
                         }
@@ -345,29 +417,25 @@ __global__ void __launch_bounds__(64) k_scan_flex(FlexArgs a) {
                     }
                     e_tip = e_acc / 6;
                 }
-                wave_sync();
+                block_sync();
                 if (converged(a, ld3(S.frc, 1), e_tip - prev_e)) break;
                 // FIRE (minimiser.cpp:107-126): P over every atom in index order
-                for (int i = lane; i < n; i += 64) S.pdot[i] = dot(ld3(S.vel, i), ld3(S.frc, i));
-                wave_sync();
-                if (lane == 0) {
-                    double p = 0;
-                    for (int i = 0; i < n; ++i) p += S.pdot[i];
-                    S.bc[1] = p;
-                }
-                wave_sync();
+                for (int i = lane; i < n; i += nthr) S.pdot[i] = dot(ld3(S.vel, i), ld3(S.frc, i));
+                block_sync();
+                if (lane == nthr - 1) S.bc[1] = seq_sum(S.pdot, nullptr, n);
+                block_sync();
                 const double p = S.bc[1];
-                for (int i = lane; i < n; i += 64) {
-                    if (a.fixed[i] == 1) continue;
+                for (int i = lane; i < n; i += nthr) {
+                    if (S.fixed[i] == 1) continue;
                     const V3 v = ld3(S.vel, i), f = ld3(S.frc, i);
                     const double fl = len3(f);
                     const V3 fh = fl > 1e-10 ? V3{f.x / fl, f.y / fl, f.z / fl} : V3{0, 0, 0};   // normalized()
                     st3(S.vel, i, add(scl(1 - alpha, v), scr(scl(alpha, fh), len3(v))));
                 }
-                wave_sync();
+                block_sync();
                 if (p < 0) {
-                    for (int i = lane; i < n; i += 64) st3(S.vel, i, {0, 0, 0});
-                    wave_sync();
+                    for (int i = lane; i < n; i += nthr) st3(S.vel, i, {0, 0, 0});
+                    block_sync();
                     nnn = 0;
                     dt *= 0.5;
                     alpha = 0.1;
@@ -382,8 +450,8 @@ __global__ void __launch_bounds__(64) k_scan_flex(FlexArgs a) {
             // makeXYZFile of the state column (simulation.cpp:114-116), then
             // getOutput (system.cpp:28-51)
             if (col == a.state_col && a.state)
-                for (int i = lane; i < n; i += 64) st3(a.state + (size_t)k * 3 * n, i, ld3(S.pos, i));
-            evaluate(a, S, S.pos, lane, true);
+                for (int i = lane; i < n; i += nthr) st3(a.state + (size_t)k * 3 * n, i, ld3(S.pos, i));
+            evaluate(a, S, S.pos, lane, nthr, true);
             if (lane == 0) {
                 const V3 d = ld3(S.pos, 0), t = ld3(S.pos, 1), f = ld3(S.frc, 1);
                 DevRecord rec;
@@ -398,22 +466,34 @@ __global__ void __launch_bounds__(64) k_scan_flex(FlexArgs a) {
                 S.pos[2] -= a.dz;
                 S.pos[5] -= a.dz;
             }
-            wave_sync();
+            block_sync();
         }
     }
 }
 
 }  // namespace
 
-size_t flex_block_doubles(int n_atoms, int n_slots, bool rk4) { return block_doubles(n_atoms, n_slots, rk4); }
+size_t flex_block_doubles(int n_atoms, int n_slots, int n_eval, bool rk4) {
+    return block_doubles(n_atoms, n_slots, n_eval, rk4);
+}
 
-hipError_t launch_scan_flex(const FlexArgs& a, int n_blocks, bool in_lds, hipStream_t st) {
+size_t flex_block_lds(const FlexArgs& a) {
+    const bool rk4 = a.minimiser == MAFM_MIN_FIRE && a.integrator == MAFM_INT_RK4;
+    const size_t n_idx = (size_t)(a.n_atoms + 1) + a.n_csr + a.n_tip + a.n_atoms;
+    return block_doubles(a.n_atoms, a.n_slots, a.n_eval, rk4) * sizeof(double) + n_idx * sizeof(int32_t);
+}
+
+hipError_t launch_scan_flex(const FlexArgs& a, int n_blocks, int waves, bool in_lds, hipStream_t st) {
     const bool rk4 = a.minimiser == MAFM_MIN_FIRE && a.integrator == MAFM_INT_RK4;
     if (in_lds) {
-        const size_t lds = block_doubles(a.n_atoms, a.n_slots, rk4) * sizeof(double);
-        hipLaunchKernelGGL(k_scan_flex<true>, dim3(n_blocks), dim3(64), lds, st, a);
+        const size_t lds = flex_block_lds(a);
+        // above 64 KiB a workgroup asks for the larger allocation (gfx950: 160 KiB)
+        hipError_t e = hipFuncSetAttribute((const void*)k_scan_flex<true>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_scan_flex<true>, dim3(n_blocks), dim3(64 * waves), lds, st, a);
     } else {
-        hipLaunchKernelGGL(k_scan_flex<false>, dim3(n_blocks), dim3(64), 0, st, a);
+        hipLaunchKernelGGL(k_scan_flex<false>, dim3(n_blocks), dim3(64 * waves), 0, st, a);
     }
     return hipGetLastError();
 }

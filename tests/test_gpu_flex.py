@@ -83,14 +83,14 @@ def test_c_abi_on_a_perturbed_surface_matches_the_oracle(gpu_ctx, tmp_path):
     assert st["total_steps"] == steps
 
 
-def test_state_outside_lds_matches(gpu_ctx, tmp_path):
-    """The reference's example (74 atoms, 781 terms) keeps its column state in
-    HBM (over 64 KiB); fx_mini keeps it in LDS: both agree with the reference
-    (the per-case test), and a state column of -1 keeps no state."""
+def test_state_in_hbm_matches(gpu_ctx, tmp_path):
+    """The column state in HBM (the path of systems too large for one
+    workgroup's LDS, forced here by MAFM_SCAN_STATE_IN_HBM) gives the same
+    records; a state column of -1 keeps no state."""
     shape, ref, _ = flex_golden("fx_graphene")
     sim = abi.Simulation(materialize("fx_graphene", str(tmp_path)), None)
     fx = sim.flex_system()
-    desc = sim.scan_desc("plain64")
+    desc = sim.scan_desc("plain64", options=abi.SCAN_STATE_IN_HBM)
     gpu_ctx.scan_flexible(fx["n_atoms"], fx["positions"], fx["fixed"], fx["terms"], desc,
                           state_column=-1, tip_dummy_d=fx["tip_dummy_d"], cell=fx["cell"], offset=fx["offset"])
     rec = abi.records_to_table(gpu_ctx.records(0, len(ref)), shape)
