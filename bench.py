@@ -300,6 +300,35 @@ def flexible_bench(tmp, args):
     print(json.dumps(out), flush=True)
 
 
+def job_bench(abi, ctx, args, tmp, precision, launch, scan_s):
+    """The whole job: scan + the per-height files of every molecule
+    (mafm_batch_run_output: 2 launches, the first half's files written while
+    the second half scans), gzip on and off, against the scan alone (one launch) and the
+    writing alone (the same files again from the records on the host)."""
+    from mechafm_amd import workloads as W
+    out = {}
+    for gz in (True, False):
+        d = os.path.join(tmp, "job_gz" if gz else "job_txt")
+        wls = W.make_c3(os.path.join(d, "in"), gzip="on" if gz else "off")
+        sims = [abi.Simulation(w.input_path, ctx, outdir=os.path.join(d, "out", f"mol{m:02d}"))
+                for m, w in enumerate(wls)]
+        t0 = time.perf_counter()
+        st = abi.batch_run_output(sims, precision, groups=2, threads=0, **launch)
+        job = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for s_ in sims:
+            s_.write_output(threads=0)
+        write = time.perf_counter() - t0
+        size = sum(os.path.getsize(os.path.join(r, f)) for r, _, fs in os.walk(os.path.join(d, "out")) for f in fs)
+        out["gzip" if gz else "text"] = {
+            "job_s": job, "scan_s": scan_s, "write_s": write, "ratio_to_max_scan_write": job / max(scan_s, write),
+            "job_scan_kernel_ms": st["kernel_ms"], "launches": 2, "files": 40 * len(sims), "bytes": size}
+        for s_ in sims:
+            s_.close()
+        shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
 def bar_compare(recs, prec_recs):
     """Per-position |dr| (max over x, y, z) and relative Fz error of recs
     against prec_recs, and the 1e-4 A / 1e-3 bar."""
@@ -420,6 +449,7 @@ def main():
                     help="FP32 sums: 0 blocks of 4-6 sites folded into FP64, 1 Kahan, 2 plain")
     ap.add_argument("--adjoint", action="store_true",
                     help="the adjoint workload (150 atoms, 128x128 px x 30 z): gradient timing and finite differences")
+    ap.add_argument("--no-job", action="store_true", help="skip the job-level (scan + files) measurement")
     ap.add_argument("--flexible", action="store_true",
                     help="time the flexible scan of the reference's example job at full size instead")
     ap.add_argument("--margins", action="store_true",
@@ -518,6 +548,11 @@ def main():
             if f32 is not None:
                 out["fast32"].update({"speedup_vs_plain_fp64": p_s / f32["kernel_s"]},
                                      **bar_compare(f32["recs"], prec))
+        if not args.no_job and args.config == "c3":
+            st = head["stats"]
+            launch = dict(lanes_per_pixel=st["lanes_per_pixel"], waves_per_simd=st["waves_per_simd"],
+                          compensation=args.compensation, waves_per_group=args.group, options=options)
+            out["job"] = job_bench(abi, ctx, args, tmp, args.precision, launch, head["kernel_s"])
         if not args.no_cpu:
             out["cpu_baseline"] = cpu_baseline(args.config, tmp)
         print(json.dumps(out), flush=True)

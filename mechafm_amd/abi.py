@@ -174,6 +174,8 @@ SIGNATURES = {
     "mafm_sim_scan_desc": (ctypes.c_int, [_vp, _P(RunOpts), _P(ScanDesc)]),
     "mafm_sim_run": (ctypes.c_int, [_vp, _P(RunOpts), _P(ScanStats)]),
     "mafm_batch_run": (ctypes.c_int, [ctypes.c_int, _P(_vp), _P(RunOpts), _P(ScanStats)]),
+    "mafm_batch_run_output": (ctypes.c_int, [ctypes.c_int, _P(_vp), _P(RunOpts), ctypes.c_int, ctypes.c_int,
+                                             _P(ScanStats)]),
     "mafm_sim_settings": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t, _P(ctypes.c_size_t)]),
     "mafm_sim_records": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _vp]),
     "mafm_sim_write_output": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -489,6 +491,20 @@ def batch_run(sims: Sequence[Simulation], precision: str = "fast32", lanes_per_p
     o = RunOpts(PREC[precision], lanes_per_pixel, waves_per_simd, compensation, waves_per_group, options)
     st = ScanStats()
     rc = lib().mafm_batch_run(len(sims), arr, ctypes.byref(o), ctypes.byref(st))
+    if rc != MAFM_OK:
+        raise MafmError(rc, lib().mafm_sim_last_error(sims[0].h).decode())
+    return st.as_dict()
+
+
+def batch_run_output(sims: Sequence[Simulation], precision: str = "fast64", groups: int = 0, threads: int = 0,
+                     lanes_per_pixel: int = 0, waves_per_simd: int = 0, compensation: int = 0,
+                     waves_per_group: int = 0, options: int = 0) -> dict:
+    """A batch run that writes each job's files while the rest of the batch
+    scans (mafm_batch_run_output)."""
+    arr = (ctypes.c_void_p * len(sims))(*[s.h.value for s in sims])
+    o = RunOpts(PREC[precision], lanes_per_pixel, waves_per_simd, compensation, waves_per_group, options)
+    st = ScanStats()
+    rc = lib().mafm_batch_run_output(len(sims), arr, ctypes.byref(o), groups, threads, ctypes.byref(st))
     if rc != MAFM_OK:
         raise MafmError(rc, lib().mafm_sim_last_error(sims[0].h).decode())
     return st.as_dict()

@@ -18,6 +18,9 @@
 #include <cstring>
 #include <fstream>
 #include <sstream>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
 #include <thread>
 
 #include "../ctx_ref.hpp"
@@ -1192,6 +1195,9 @@ void Simulation::run(const mafm_run_opts& r, mafm_scan_stats* stats) {   // simu
 }
 
 void Simulation::requireOwnRecords() const {
+    if (host_records_)
+        fail(MAFM_ERR_STATE, "the records of a streamed batch run are on the host only; device-side "
+                             "post-processing needs the job run again");
     if (scan_generation_ == 0) fail(MAFM_ERR_STATE, "no records: this simulation has not run");
     if (scan_generation_ != ctx_scan_generation(ctx_))
         fail(MAFM_ERR_STATE, "the context has run another scan since this simulation's: its records are gone "
@@ -1199,6 +1205,7 @@ void Simulation::requireOwnRecords() const {
 }
 
 void Simulation::fetchRecords() {
+    if (host_records_) return;   // a streamed batch left them on the host
     requireOwnRecords();
     size_t n = (size_t)n_points_[0] * n_points_[1] * n_points_[2];
     records_.resize(n);
@@ -1311,9 +1318,12 @@ void Simulation::writeOutput(int threads) {
             fprintf(f, "Simulation run finished\n");
             fprintf(f, "Statistics:\n");
             fprintf(f, "    Computed %lld tip positions\n", (long long)n_pts);
-            fprintf(f, "    Needed %lld minimization steps in total\n", (long long)last_stats_.total_steps);
+            // this job's own iterations (a batched launch's statistics cover all its jobs)
+            long long steps = 0;
+            for (const mafm_record& r : records_) steps += r.n_steps;
+            fprintf(f, "    Needed %lld minimization steps in total\n", steps);
             fprintf(f, "    Which means approximately %.2f minimization steps per tip position\n",
-                    (double)last_stats_.total_steps / n_pts);
+                    (double)steps / n_pts);
             // the reference's two times (mechafm.cpp:137-150), from the job's opening
             const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - opened_).count();
             fprintf(f, "    The simulation wall time is %.2f seconds\n", wall);
@@ -1324,8 +1334,8 @@ void Simulation::writeOutput(int threads) {
 }
 
 void Simulation::writeStates() {                  // system.cpp:54-65
-    requireOwnRecords();
     if (!options_.flexible || flex_desc_.state_column < 0) return;
+    requireOwnRecords();
     if (records_.empty()) fetchRecords();
     const int n = system.n_atoms_, nz = n_points_[2];
     std::vector<double> st((size_t)nz * n * 3);
@@ -1499,8 +1509,56 @@ mafm_status mafm_sim_run(mafm_sim* s, const mafm_run_opts* o, mafm_scan_stats* s
     def.precision = MAFM_PREC_FAST64;
     return guarded(s, [&]() {
         s->sim.records_.clear();
+        s->sim.host_records_ = false;
         s->sim.run(o ? *o : def, stats);
     });
+}
+
+// The members of a batched launch: initialised rigid jobs on one context
+// with the same scan geometry and minimiser settings; returns the shared
+// scan descriptor.
+static mafm_scan_desc batch_desc(int n, mafm_sim* const* sims, const mafm_run_opts& o) {
+    mafm_ctx* ctx = sims[0]->sim.ctx_;
+    if (!ctx) throw Error(MAFM_ERR_STATE, "a batch needs a device context");
+    mafm_scan_desc d0 = sims[0]->sim.scanDesc(o);
+    for (int m = 0; m < n; ++m) {
+        if (!sims[m]) throw Error(MAFM_ERR_INVALID, "batch member is NULL");
+        Simulation& s = sims[m]->sim;
+        if (!s.initialized_) throw Error(MAFM_ERR_STATE, "batch member not initialised");
+        if (s.options_.flexible) throw Error(MAFM_ERR_INVALID, "flexible jobs run one at a time (mafm_sim_run)");
+        if (s.ctx_ != ctx) throw Error(MAFM_ERR_INVALID, "batch members must share one context");
+        mafm_scan_desc d = s.scanDesc(o);
+        if (d.nx != d0.nx || d.ny != d0.ny || d.nz != d0.nz || d.dx != d0.dx || d.dy != d0.dy ||
+            d.dz != d0.dz || d.zhigh != d0.zhigh || d.minimiser != d0.minimiser ||
+            d.integrator != d0.integrator || d.minterm != d0.minterm || d.etol != d0.etol ||
+            d.ftol != d0.ftol || d.dt != d0.dt || d.maxsteps != d0.maxsteps)
+            throw Error(MAFM_ERR_INVALID, "batch members must share the scan options");
+    }
+    return d0;
+}
+
+// One batched launch of the given members; their records stay on the device.
+static void batch_scan(int n, mafm_sim* const* sims, const mafm_scan_desc& d0, mafm_scan_stats& ss) {
+    mafm_ctx* ctx = sims[0]->sim.ctx_;
+    std::vector<mafm_system_desc> descs;
+    for (int m = 0; m < n; ++m) {
+        descs.push_back(sims[m]->sim.systemDesc());
+        sims[m]->sim.records_.clear();
+        sims[m]->sim.host_records_ = false;
+        sims[m]->sim.scan_generation_ = 0;
+    }
+    mafm_status st = mafm_set_systems(ctx, n, descs.data());
+    if (st != MAFM_OK) throw Error(st, mafm_last_error(ctx));
+    ss = mafm_scan_stats{};
+    st = mafm_scan(ctx, &d0, &ss);
+    if (st != MAFM_OK) throw Error(st, mafm_last_error(ctx));
+    const size_t per = (size_t)d0.nx * d0.ny * d0.nz;
+    for (int m = 0; m < n; ++m) {
+        sims[m]->sim.record_base_ = per * m;
+        sims[m]->sim.system_index_ = m;
+        sims[m]->sim.last_stats_ = ss;
+        sims[m]->sim.scan_generation_ = mafm::ctx_scan_generation(ctx);
+    }
 }
 
 mafm_status mafm_batch_run(int n, mafm_sim* const* sims, const mafm_run_opts* o, mafm_scan_stats* stats) {
@@ -1508,39 +1566,124 @@ mafm_status mafm_batch_run(int n, mafm_sim* const* sims, const mafm_run_opts* o,
     mafm_run_opts def{};
     def.precision = MAFM_PREC_FAST32;
     return guarded(sims[0], [&]() {
-        mafm_ctx* ctx = sims[0]->sim.ctx_;
-        std::vector<mafm_system_desc> descs;
-        mafm_scan_desc d0 = sims[0]->sim.scanDesc(o ? *o : def);
-        for (int m = 0; m < n; ++m) {
-            Simulation& s = sims[m]->sim;
-            if (!s.initialized_) throw Error(MAFM_ERR_STATE, "batch member not initialised");
-            if (s.options_.flexible) throw Error(MAFM_ERR_INVALID, "flexible jobs run one at a time (mafm_sim_run)");
-            if (s.ctx_ != ctx) throw Error(MAFM_ERR_INVALID, "batch members must share one context");
-            mafm_scan_desc d = s.scanDesc(o ? *o : def);
-            if (d.nx != d0.nx || d.ny != d0.ny || d.nz != d0.nz || d.dx != d0.dx || d.dy != d0.dy ||
-                d.dz != d0.dz || d.zhigh != d0.zhigh || d.minimiser != d0.minimiser ||
-                d.integrator != d0.integrator || d.minterm != d0.minterm || d.etol != d0.etol ||
-                d.ftol != d0.ftol || d.dt != d0.dt || d.maxsteps != d0.maxsteps)
-                throw Error(MAFM_ERR_INVALID, "batch members must share the scan options");
-            descs.push_back(s.systemDesc());
-        }
-        for (int m = 0; m < n; ++m) {
-            sims[m]->sim.records_.clear();
-            sims[m]->sim.scan_generation_ = 0;
-        }
-        mafm_status st = mafm_set_systems(ctx, n, descs.data());
-        if (st != MAFM_OK) throw Error(st, mafm_last_error(ctx));
+        const mafm_scan_desc d0 = batch_desc(n, sims, o ? *o : def);
         mafm_scan_stats ss{};
-        st = mafm_scan(ctx, &d0, &ss);
-        if (st != MAFM_OK) throw Error(st, mafm_last_error(ctx));
-        const size_t per = (size_t)d0.nx * d0.ny * d0.nz;
-        for (int m = 0; m < n; ++m) {
-            sims[m]->sim.record_base_ = per * m;
-            sims[m]->sim.system_index_ = m;
-            sims[m]->sim.last_stats_ = ss;
-            sims[m]->sim.scan_generation_ = mafm::ctx_scan_generation(ctx);
-        }
+        batch_scan(n, sims, d0, ss);
         if (stats) *stats = ss;
+    });
+}
+
+mafm_status mafm_batch_run_output(int n, mafm_sim* const* sims, const mafm_run_opts* o, int groups, int threads,
+                                  mafm_scan_stats* stats) {
+    if (n <= 0 || !sims || !sims[0]) return MAFM_ERR_INVALID;
+    mafm_run_opts def{};
+    def.precision = MAFM_PREC_FAST64;
+    return guarded(sims[0], [&]() {
+        const mafm_scan_desc d0 = batch_desc(n, sims, o ? *o : def);
+        // two launches by default: a launch needs a few columns per resident
+        // pixel group to stay busy, so more launches slow the scan
+        if (groups <= 0) groups = 2;
+        groups = std::max(1, std::min(groups, n));
+        // contiguous groups of members, balanced by sites (the scan's work)
+        std::vector<double> cost(n);
+        double total = 0;
+        for (int m = 0; m < n; ++m) {
+            const mafm_system_desc& sd = sims[m]->sim.systemDesc();
+            cost[m] = sd.force_grid >= 0 ? 1.0 : 1.0 + (double)sd.n_sites * std::max(1, (int)sd.n_images);
+            total += cost[m];
+        }
+        std::vector<int> bounds{0};
+        double acc = 0;
+        for (int m = 0; m < n; ++m) {
+            acc += cost[m];
+            if ((int)bounds.size() < groups && acc >= total * bounds.size() / groups && m + 1 < n)
+                bounds.push_back(m + 1);
+        }
+        bounds.push_back(n);
+        // the writer: a thread that writes each finished group's files while
+        // the next group scans (each job's heights on `threads` threads)
+        std::mutex mu;
+        std::condition_variable cv;
+        std::deque<std::pair<int, int>> ready;
+        bool done = false;
+        std::string werr;
+        mafm_status wst = MAFM_OK;
+        std::thread writer([&]() {
+            for (;;) {
+                std::pair<int, int> g;
+                {
+                    std::unique_lock<std::mutex> lk(mu);
+                    cv.wait(lk, [&]() { return done || !ready.empty(); });
+                    if (ready.empty()) return;
+                    g = ready.front();
+                    ready.pop_front();
+                }
+                for (int m = g.first; m < g.second; ++m) {
+                    try {
+                        sims[m]->sim.writeOutput(threads);
+                    } catch (const Error& e) {
+                        std::lock_guard<std::mutex> lk(mu);
+                        if (wst == MAFM_OK) {
+                            wst = e.status;
+                            werr = e.what();
+                        }
+                    } catch (const std::exception& e) {
+                        std::lock_guard<std::mutex> lk(mu);
+                        if (wst == MAFM_OK) {
+                            wst = MAFM_ERR_INVALID;
+                            werr = e.what();
+                        }
+                    }
+                }
+            }
+        });
+        mafm_scan_stats sum{};
+        std::string serr;
+        mafm_status sst = MAFM_OK;
+        for (size_t b = 0; b + 1 < bounds.size(); ++b) {
+            const int m0 = bounds[b], m1 = bounds[b + 1];
+            mafm_scan_stats ss{};
+            try {
+                batch_scan(m1 - m0, sims + m0, d0, ss);
+                for (int m = m0; m < m1; ++m) {   // to the host before the next group replaces them
+                    sims[m]->sim.fetchRecords();
+                    sims[m]->sim.host_records_ = true;
+                    sims[m]->sim.scan_generation_ = 0;
+                }
+            } catch (const Error& e) {
+                sst = e.status;
+                serr = e.what();
+                break;
+            }
+            sum.kernel_ms += ss.kernel_ms;
+            sum.positions += ss.positions;
+            sum.total_steps += ss.total_steps;
+            sum.unconverged += ss.unconverged;
+            sum.grid_warnings += ss.grid_warnings;
+            sum.pair_evals += ss.pair_evals;
+            sum.lanes_per_pixel = ss.lanes_per_pixel;
+            sum.waves_per_simd = ss.waves_per_simd;
+            sum.grid_blocks = std::max(sum.grid_blocks, ss.grid_blocks);
+            sum.block_threads = ss.block_threads;
+            sum.group_trips += ss.group_trips;
+            sum.active_group_trips += ss.active_group_trips;
+            sum.wide_trips += ss.wide_trips;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                ready.emplace_back(m0, m1);
+            }
+            cv.notify_one();
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            done = true;
+        }
+        cv.notify_one();
+        writer.join();
+        if (sst != MAFM_OK) throw Error(sst, serr);
+        if (wst != MAFM_OK) throw Error(wst, werr);
+        for (int m = 0; m < n; ++m) sims[m]->sim.last_stats_ = sum;
+        if (stats) *stats = sum;
     });
 }
 
@@ -1565,6 +1708,12 @@ mafm_status mafm_sim_settings(mafm_sim* s, char* buf, size_t size, size_t* neede
 mafm_status mafm_sim_records(mafm_sim* s, size_t first, size_t count, mafm_record* out) {
     if (!s) return MAFM_ERR_INVALID;
     return guarded(s, [&]() {
+        if (s->sim.host_records_) {
+            if (first + count > s->sim.records_.size() || (count && !out))
+                throw Error(MAFM_ERR_INVALID, "records outside the job's");
+            if (count) memcpy(out, s->sim.records_.data() + first, count * sizeof(mafm_record));
+            return;
+        }
         s->sim.requireOwnRecords();
         mafm_status st = mafm_records(s->sim.ctx_, s->sim.record_base_ + first, count, out);
         if (st != MAFM_OK) throw Error(st, mafm_last_error(s->sim.ctx_));

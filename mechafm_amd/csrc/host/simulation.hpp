@@ -384,6 +384,9 @@ class Simulation {
     // the context's scan count right after this simulation's scan (0: none):
     // another scan on the context replaces the records
     uint64_t scan_generation_ = 0;
+    // set by mafm_batch_run_output: records_ holds this job's records, the
+    // context no longer does
+    bool host_records_ = false;
     void requireOwnRecords() const;
     // console lines of the output writing ("+- Wrote ..."), in order
     std::string log_;
