@@ -214,6 +214,23 @@ static mafm_status new_grid(mafm_ctx* c, const mafm_grid_desc* g, HostGrid& h) {
     return MAFM_OK;
 }
 
+// A new grid's device buffers, freed on every early return unless the grid
+// was stored in the context.
+struct GridGuard {
+    HostGrid& h;
+    bool kept = false;
+    explicit GridGuard(HostGrid& g) : h(g) {}
+    GridGuard(const GridGuard&) = delete;
+    GridGuard& operator=(const GridGuard&) = delete;
+    ~GridGuard() {
+        if (kept) return;
+        if (h.fe64) (void)hipFree(h.fe64);
+        if (h.fe32) (void)hipFree(h.fe32);
+        h.fe64 = nullptr;
+        h.fe32 = nullptr;
+    }
+};
+
 static int store_grid(mafm_ctx* c, const HostGrid& h) {
     for (size_t k = 0; k < c->grids.size(); ++k)
         if (!c->grids[k].live) {
@@ -560,6 +577,7 @@ mafm_status mafm_grid_upload(mafm_ctx* c, const mafm_grid_desc* g, int* handle) 
     if (!g || !g->fe) return fail(c, MAFM_ERR_INVALID, "grid upload needs sample values");
     HIPCHK(c, hipSetDevice(c->device));
     HostGrid h;
+    GridGuard guard(h);
     mafm_status s = new_grid(c, g, h);
     if (s != MAFM_OK) return s;
     size_t np = (size_t)h.n[0] * h.n[1] * h.n[2];
@@ -567,6 +585,7 @@ mafm_status mafm_grid_upload(mafm_ctx* c, const mafm_grid_desc* g, int* handle) 
     HIPCHK(c, launch_to_f32(h.fe64, h.fe32, (long)np, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     *handle = store_grid(c, h);
+    guard.kept = true;
     return sync_grids(c);
 }
 
@@ -636,6 +655,7 @@ mafm_status mafm_grid_build(mafm_ctx* c, const mafm_system_desc* sys, const mafm
     mafm_status s = build_tables(c, 1, sys, mols, ref, seg64, seg32, cost);
     if (s != MAFM_OK) return s;
     HostGrid h;
+    GridGuard guard(h);
     s = new_grid(c, g, h);
     if (s != MAFM_OK) return s;
     if (!h.orthogonal) return fail(c, MAFM_ERR_INVALID, "grid build expects a diagonal spacing basis");
@@ -660,12 +680,9 @@ mafm_status mafm_grid_build(mafm_ctx* c, const mafm_system_desc* sys, const mafm
     HIPCHK(c, hipStreamSynchronize(c->stream));
     dm.release();
     dr.release();
-    if (counters[3] != 0) {
-        (void)hipFree(h.fe64);
-        (void)hipFree(h.fe32);
-        return fail(c, MAFM_ERR_OUT_OF_GRID, "potential-grid lookup left the grid during grid build");
-    }
+    if (counters[3] != 0) return fail(c, MAFM_ERR_OUT_OF_GRID, "potential-grid lookup left the grid during grid build");
     *handle = store_grid(c, h);
+    guard.kept = true;
     return sync_grids(c);
 }
 
@@ -1231,6 +1248,7 @@ mafm_status mafm_potential_grid(mafm_ctx* c, const mafm_potential_desc* pd, int*
     }
     gd.periodic = 1;
     memcpy(gd.basis, B, sizeof gd.basis);
+    GridGuard guard(h);
     s = new_grid(c, &gd, h);
     if (s != MAFM_OK) return s;
     double* fe = reinterpret_cast<double*>(h.fe64);
@@ -1253,11 +1271,10 @@ mafm_status mafm_potential_grid(mafm_ctx* c, const mafm_potential_desc* pd, int*
     tmp.release();
     real.release();
     if (flag) {
-        (void)hipFree(h.fe64);
-        (void)hipFree(h.fe32);
         return fail(c, MAFM_ERR_FFT_IMAG, "Imaginary part of inverse FFT exceeded threshold 1.0e-8!");
     }
     *handle = store_grid(c, h);
+    guard.kept = true;
     return sync_grids(c);
 }
 

@@ -1,7 +1,7 @@
-"""The maintainer patch of INTEGRATION.md (oracle/integration.patch: a
-`lower()` on each reference Interaction subclass and Simulation::run handing
-its pixel loop to libmafm) applied to the reference sources, compiled and
-linked against libmafm.so by the oracle/Makefile recipe.  Built only, never
+"""The maintainer patch (oracle/integration.patch: a `lower()` / `lowerFlex()`
+on each reference Interaction subclass and Simulation::run handing its pixel
+loop -- rigid or flexible -- to libmafm) applied to the reference sources,
+compiled and linked against libmafm.so by the oracle/Makefile recipe.  Built only, never
 run: this checks that the patch applies to the reference as it is and that
 the C ABI it calls links."""
 import os
@@ -26,7 +26,9 @@ def test_patched_reference_compiles_and_links():
     assert os.path.exists(exe)
     syms = subprocess.check_output(["nm", "-C", exe], text=True)
     assert "Simulation::runOnDevice()" in syms
-    for name in ("mafm_create", "mafm_set_systems", "mafm_scan", "mafm_records", "mafm_destroy"):
+    assert "Simulation::runFlexibleOnDevice()" in syms
+    for name in ("mafm_create", "mafm_set_systems", "mafm_scan", "mafm_records", "mafm_destroy",
+                 "mafm_scan_flexible", "mafm_flex_state"):
         assert f"U {name}" in syms, name
     deps = subprocess.check_output(["ldd", exe], text=True)
     assert "libmafm.so" in deps and "not found" not in deps
