@@ -302,27 +302,38 @@ def flexible_bench(tmp, args):
 
 def job_bench(abi, ctx, args, tmp, precision, launch, scan_s):
     """The whole job: scan + the per-height files of every molecule
-    (mafm_batch_run_output: 2 launches, the first half's files written while
-    the second half scans), gzip on and off, against the scan alone (one launch) and the
-    writing alone (the same files again from the records on the host)."""
+    (mafm_batch_run_output: one launch streaming records to pinned host
+    memory; each molecule's files are written as soon as its columns are in,
+    while the rest scans), gzip on and off, against the scan alone and the
+    writing alone (the same files again from the records on the host, on the
+    same writer pool: mafm_batch_write_output).  A
+    first, untimed job on the context allocates the pinned buffers."""
     from mechafm_amd import workloads as W
     out = {}
+    warm = os.path.join(tmp, "job_warm")
+    wls = W.make_c3(os.path.join(warm, "in"), gzip="off")
+    sims = [abi.Simulation(w.input_path, ctx, outdir=os.path.join(warm, "out", f"mol{m:02d}"))
+            for m, w in enumerate(wls)]
+    abi.batch_run_output(sims, precision, threads=0, **launch)
+    for s_ in sims:
+        s_.close()
+    shutil.rmtree(warm, ignore_errors=True)
     for gz in (True, False):
         d = os.path.join(tmp, "job_gz" if gz else "job_txt")
         wls = W.make_c3(os.path.join(d, "in"), gzip="on" if gz else "off")
         sims = [abi.Simulation(w.input_path, ctx, outdir=os.path.join(d, "out", f"mol{m:02d}"))
                 for m, w in enumerate(wls)]
         t0 = time.perf_counter()
-        st = abi.batch_run_output(sims, precision, groups=2, threads=0, **launch)
+        st = abi.batch_run_output(sims, precision, threads=0, **launch)
         job = time.perf_counter() - t0
         t0 = time.perf_counter()
-        for s_ in sims:
-            s_.write_output(threads=0)
+        abi.batch_write_output(sims, threads=0)
         write = time.perf_counter() - t0
         size = sum(os.path.getsize(os.path.join(r, f)) for r, _, fs in os.walk(os.path.join(d, "out")) for f in fs)
         out["gzip" if gz else "text"] = {
             "job_s": job, "scan_s": scan_s, "write_s": write, "ratio_to_max_scan_write": job / max(scan_s, write),
-            "job_scan_kernel_ms": st["kernel_ms"], "launches": 2, "files": 40 * len(sims), "bytes": size}
+            "job_scan_kernel_ms": st["kernel_ms"], "launches": 1, "files": 40 * len(sims), "bytes": size,
+            "host_threads": len(os.sched_getaffinity(0))}
         for s_ in sims:
             s_.close()
         shutil.rmtree(d, ignore_errors=True)
@@ -387,7 +398,7 @@ def roofline_of(sims, run, precision, info):
     return r
 
 
-def margins_report(abi, sims, ctx, positions, nz):
+def margins_report(abi, sims, ctx, positions, nz, tmp=None):
     """Why FP32 pair sums cannot keep the bar on every position: the FP32
     and FP64 scans with their decision margins (MAFM_SCAN_MARGINS) against
     the plain FP64 scan.  For every column whose FIRE step counts leave the
@@ -422,6 +433,21 @@ def margins_report(abi, sims, ctx, positions, nz):
     out["replay_band_sweep"] = [{"band": b, "columns_flagged": float((colmin < b).mean()),
                                  "divergent_columns_missed": int((colmin[div] >= b).sum())}
                                 for b in (1e-7, 1e-6, 1e-5, 3e-5, 1e-4)]
+    if tmp is not None:
+        # the reference's own arithmetic on inputs moved by FP32-sized noise:
+        # how far plain FP64 itself keeps the bar when every atom coordinate
+        # changes in its last written digit (1e-8 A) or by the rounding of an
+        # FP32 coordinate near 10 A (1e-6 A)
+        from mechafm_amd import workloads as W
+        out["plain64_input_sensitivity"] = []
+        for jitter in (1e-8, 1e-6):
+            wls = W.make_c3(os.path.join(tmp, f"jitter{jitter:g}"), jitter=jitter)
+            jsims = [abi.Simulation(w.input_path, ctx) for w in wls]
+            run_batch(abi, jsims, "plain64")
+            out["plain64_input_sensitivity"].append(
+                dict({"jitter_A": jitter}, **bar_compare([np.concatenate([s.records() for s in jsims])], [rp])))
+            for s in jsims:
+                s.close()
     return out
 
 
@@ -498,7 +524,8 @@ def main():
               f"{time.perf_counter() - t0:.1f} s on {dev['name']} ({dev['compute_units']} CUs)",
               file=sys.stderr, flush=True)
         if args.margins:
-            print(json.dumps({"margins_" + args.config: margins_report(abi, sims, ctx, positions, shape[2])}),
+            print(json.dumps({"margins_" + args.config: margins_report(
+                abi, sims, ctx, positions, shape[2], tmp if args.config == "c3" else None)}),
                   flush=True)
             return
 

@@ -20,6 +20,9 @@
  *                          System::getOutput (system.cpp:28-51)
  *   mafm_records           the OutputData buffer drained by writeOutput
  *                          (simulation.cpp:145-199; globals.hpp:27-32)
+ *   mafm_scan_host         mafm_scan with the records handed to the host as
+ *                          the scan runs, as the reference flushes its buffer
+ *                          to the writer inside the scan (simulation.cpp:123-138)
  *   mafm_eval_tip_surface  Interaction::eval over the tip-surface list
  *                          (interactions.hpp:68-76, interactions.cpp:15-51,
  *                          236-250) as System::evalTipSurfaceForces calls it
@@ -323,6 +326,25 @@ mafm_status mafm_scan(mafm_ctx* ctx, const mafm_scan_desc* desc, mafm_scan_stats
  * report MAFM_ERR_STATE / an empty range until the next mafm_scan). */
 mafm_status mafm_scan_noalloc(mafm_ctx* ctx, const mafm_scan_desc* desc, mafm_record* records, size_t capacity,
                               mafm_scan_stats* stats);
+/* Streamed scan: the records the reference's scan loop hands to its writer
+ * while it runs (simulation.cpp:123-138, flushed every bufsize columns).
+ * Launches mafm_scan's kernels and returns at once; the kernels store the
+ * records straight into pinned host memory owned by the context
+ * (*records, in mafm_records' order), and column_done[system*nx*ny + i*ny
+ * + j] counts the heights of that column whose records are there (nz once
+ * the column is finished; heights are stored from the top, k = 0 first), so
+ * the caller may read records as soon as the count covers them (read the
+ * count with acquire ordering).  mafm_scan_host_query reports whether the launch
+ * has finished without waiting; mafm_scan_host_wait ends it and fills the
+ * statistics (MAFM_ERR_OUT_OF_GRID as mafm_scan).  Until then every other
+ * scan on the context is refused with MAFM_ERR_STATE.  The host memory stays
+ * valid until the next streamed scan or mafm_destroy; the context holds no
+ * device records afterwards (mafm_records / mafm_delta_f refuse).
+ * MAFM_SCAN_MARGINS is refused. */
+mafm_status mafm_scan_host(mafm_ctx* ctx, const mafm_scan_desc* desc, const mafm_record** records,
+                           const int32_t** column_done);
+mafm_status mafm_scan_host_query(mafm_ctx* ctx, int* finished);
+mafm_status mafm_scan_host_wait(mafm_ctx* ctx, mafm_scan_stats* stats);
 /* records of all systems: index ((system*nx + i)*ny + j)*nz + k */
 mafm_status mafm_records(mafm_ctx* ctx, size_t first, size_t count, mafm_record* out);
 /* Decision margins of the last scan (FAST kernels run with

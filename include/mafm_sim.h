@@ -18,6 +18,8 @@
  *                          the reference runs one molecule per process)
  *   mafm_batch_run_output  the same, writing the files during the scan
  *                          (simulation.cpp:123-138 flushes during the run)
+ *   mafm_batch_write_output  writeOutput for many jobs on one thread pool
+ *                          (simulation.cpp:145-199)
  *   mafm_sim_flex_system   the flexible interaction list of buildInteractions
  *                          (simulation.cpp:201-216, 503-693)
  *   mafm_sim_write_output  openUniverse / writeOutput / closeUniverse
@@ -75,19 +77,22 @@ mafm_status mafm_sim_flex_system(mafm_sim* sim, mafm_flex_desc* desc);
 mafm_status mafm_sim_scan_desc(mafm_sim* sim, const mafm_run_opts* opts, mafm_scan_desc* desc);
 mafm_status mafm_sim_run(mafm_sim* sim, const mafm_run_opts* opts, mafm_scan_stats* stats);
 mafm_status mafm_batch_run(int n, mafm_sim* const* sims, const mafm_run_opts* opts, mafm_scan_stats* stats);
-/* A batch run that writes every job's output files while the rest of the
- * batch scans (the reference writes its buffer out during the scan,
- * simulation.cpp:123-138): the jobs are scanned in `groups` launches
- * (<= 0: 8) of contiguous members balanced by sites; each finished group's
- * records come to the host and a writer thread formats and writes its files
- * (each job's heights on `threads` threads, <= 0: every hardware thread)
- * while the next group scans.  groups <= 0: 2 (a launch needs a few columns
- * per resident pixel group to stay busy: more, smaller launches overlap
- * more of the writing but slow the scan).  Afterwards each job holds its records on the
- * host (mafm_sim_records serves them; delta f, which reads the device
- * records, needs another run).  stats: summed over the launches. */
-mafm_status mafm_batch_run_output(int n, mafm_sim* const* sims, const mafm_run_opts* opts, int groups, int threads,
+/* A batch run that writes every job's output files while the batch scans
+ * (the reference writes its buffer out during the scan,
+ * simulation.cpp:123-138): one launch (mafm_scan_host) stores the records
+ * in pinned host memory as pixel columns finish; as soon as every column of
+ * a job is there, a writer thread copies the job's records and formats and
+ * writes its files (each job's heights on `threads` threads, <= 0: every
+ * hardware thread) while the rest of the batch scans.  Afterwards each job
+ * holds its records on the host (mafm_sim_records serves them; delta f,
+ * which reads the device records, needs another run). */
+mafm_status mafm_batch_run_output(int n, mafm_sim* const* sims, const mafm_run_opts* opts, int threads,
                                   mafm_scan_stats* stats);
+/* the per-height files (and statistics.txt) of many rigid jobs that have
+ * run, written on one pool of `threads` threads (<= 0: every hardware
+ * thread), chunk by chunk of pixel columns, gzip deflated in parallel within
+ * a file: mafm_sim_write_output for a batch (simulation.cpp:145-199) */
+mafm_status mafm_batch_write_output(int n, mafm_sim* const* sims, int threads);
 mafm_status mafm_sim_records(mafm_sim* sim, size_t first, size_t count, mafm_record* out);
 /* per-height files scan-%06.3f.dat[.gz] with the reference's line format
  * (simulation.cpp:145-199); threads <= 0 uses every hardware thread */

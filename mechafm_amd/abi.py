@@ -155,6 +155,9 @@ SIGNATURES = {
     "mafm_set_systems": (ctypes.c_int, [_vp, ctypes.c_int, _P(SystemDesc)]),
     "mafm_scan": (ctypes.c_int, [_vp, _P(ScanDesc), _P(ScanStats)]),
     "mafm_scan_noalloc": (ctypes.c_int, [_vp, _P(ScanDesc), _vp, ctypes.c_size_t, _P(ScanStats)]),
+    "mafm_scan_host": (ctypes.c_int, [_vp, _P(ScanDesc), _P(_vp), _P(_P(ctypes.c_int32))]),
+    "mafm_scan_host_query": (ctypes.c_int, [_vp, _P(ctypes.c_int)]),
+    "mafm_scan_host_wait": (ctypes.c_int, [_vp, _P(ScanStats)]),
     "mafm_records": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _vp]),
     "mafm_margins": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _P(ctypes.c_float)]),
     "mafm_scan_flexible": (ctypes.c_int, [_vp, _P(FlexDesc), _P(ScanDesc), _P(ScanStats)]),
@@ -174,8 +177,8 @@ SIGNATURES = {
     "mafm_sim_scan_desc": (ctypes.c_int, [_vp, _P(RunOpts), _P(ScanDesc)]),
     "mafm_sim_run": (ctypes.c_int, [_vp, _P(RunOpts), _P(ScanStats)]),
     "mafm_batch_run": (ctypes.c_int, [ctypes.c_int, _P(_vp), _P(RunOpts), _P(ScanStats)]),
-    "mafm_batch_run_output": (ctypes.c_int, [ctypes.c_int, _P(_vp), _P(RunOpts), ctypes.c_int, ctypes.c_int,
-                                             _P(ScanStats)]),
+    "mafm_batch_run_output": (ctypes.c_int, [ctypes.c_int, _P(_vp), _P(RunOpts), ctypes.c_int, _P(ScanStats)]),
+    "mafm_batch_write_output": (ctypes.c_int, [ctypes.c_int, _P(_vp), ctypes.c_int]),
     "mafm_sim_settings": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t, _P(ctypes.c_size_t)]),
     "mafm_sim_records": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _vp]),
     "mafm_sim_write_output": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -294,6 +297,31 @@ class Context:
     def scan(self, desc: ScanDesc) -> dict:
         st = ScanStats()
         self.check(lib().mafm_scan(self.h, ctypes.byref(desc), ctypes.byref(st)))
+        return st.as_dict()
+
+    def scan_host(self, desc: ScanDesc, n_systems: int):
+        """Launch a streamed scan of the n_systems set by set_systems
+        (mafm_scan_host); returns numpy views of the pinned host records and
+        the per-column flags, valid until the next streamed scan on this
+        context.  End it with scan_host_wait()."""
+        rec = ctypes.c_void_p()
+        done = _P(ctypes.c_int32)()
+        self.check(lib().mafm_scan_host(self.h, ctypes.byref(desc), ctypes.byref(rec), ctypes.byref(done)))
+        n_cols = desc.nx * desc.ny * n_systems
+        n_rec = n_cols * desc.nz
+        recs = np.frombuffer((ctypes.c_char * (n_rec * RECORD_DTYPE.itemsize)).from_address(rec.value),
+                             dtype=RECORD_DTYPE)
+        flags = np.ctypeslib.as_array(done, shape=(n_cols,))
+        return recs, flags
+
+    def scan_host_query(self) -> bool:
+        fin = ctypes.c_int()
+        self.check(lib().mafm_scan_host_query(self.h, ctypes.byref(fin)))
+        return bool(fin.value)
+
+    def scan_host_wait(self) -> dict:
+        st = ScanStats()
+        self.check(lib().mafm_scan_host_wait(self.h, ctypes.byref(st)))
         return st.as_dict()
 
     def records(self, first: int, count: int) -> np.ndarray:
@@ -496,18 +524,27 @@ def batch_run(sims: Sequence[Simulation], precision: str = "fast32", lanes_per_p
     return st.as_dict()
 
 
-def batch_run_output(sims: Sequence[Simulation], precision: str = "fast64", groups: int = 0, threads: int = 0,
+def batch_run_output(sims: Sequence[Simulation], precision: str = "fast64", threads: int = 0,
                      lanes_per_pixel: int = 0, waves_per_simd: int = 0, compensation: int = 0,
                      waves_per_group: int = 0, options: int = 0) -> dict:
-    """A batch run that writes each job's files while the rest of the batch
-    scans (mafm_batch_run_output)."""
+    """A batch run that writes each job's files as soon as its columns are
+    scanned, while the rest of the batch scans (mafm_batch_run_output)."""
     arr = (ctypes.c_void_p * len(sims))(*[s.h.value for s in sims])
     o = RunOpts(PREC[precision], lanes_per_pixel, waves_per_simd, compensation, waves_per_group, options)
     st = ScanStats()
-    rc = lib().mafm_batch_run_output(len(sims), arr, ctypes.byref(o), groups, threads, ctypes.byref(st))
+    rc = lib().mafm_batch_run_output(len(sims), arr, ctypes.byref(o), threads, ctypes.byref(st))
     if rc != MAFM_OK:
         raise MafmError(rc, lib().mafm_sim_last_error(sims[0].h).decode())
     return st.as_dict()
+
+
+def batch_write_output(sims: Sequence[Simulation], threads: int = 0) -> None:
+    """The per-height files of many jobs that have run, on one thread pool
+    (mafm_batch_write_output)."""
+    arr = (ctypes.c_void_p * len(sims))(*[s.h.value for s in sims])
+    rc = lib().mafm_batch_write_output(len(sims), arr, threads)
+    if rc != MAFM_OK:
+        raise MafmError(rc, lib().mafm_sim_last_error(sims[0].h).decode())
 
 
 def records_to_table(rec: np.ndarray, shape) -> np.ndarray:

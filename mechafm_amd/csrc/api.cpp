@@ -8,9 +8,11 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <array>
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mafm.h"
@@ -111,6 +113,50 @@ struct DevBuf {
     }
 };
 
+// Pinned, coherent host memory mapped into the device's address space: the
+// kernels store into it through `dp` over PCIe while the host reads it
+// through `p`.  Grows only.
+template <typename T>
+struct HostBuf {
+    T* p = nullptr;
+    T* dp = nullptr;
+    size_t cap = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    ~HostBuf() { release(); }
+    hipError_t reserve(size_t n) {
+        if (n <= cap && p) return hipSuccess;
+        release();
+        void* h = nullptr;
+        hipError_t e = hipHostMalloc(&h, std::max<size_t>(n, 1) * sizeof(T), hipHostMallocMapped | hipHostMallocCoherent);
+        if (e != hipSuccess) return e;
+        void* d = nullptr;
+        e = hipHostGetDevicePointer(&d, h, 0);
+        if (e != hipSuccess) {
+            (void)hipHostFree(h);
+            return e;
+        }
+        p = static_cast<T*>(h);
+        dp = static_cast<T*>(d);
+        cap = std::max<size_t>(n, 1);
+        return hipSuccess;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = dp = nullptr;
+        cap = 0;
+    }
+};
+
+// What finishing a launched scan needs (mafm_scan_host_wait).
+struct PendingScan {
+    mafm_scan_desc d{};
+    bool fast = false;
+    int G = 1, wps = 0, n_blocks = 0, W = 1;
+    size_t n_rec = 0;
+};
+
 }  // namespace
 
 struct mafm_ctx {
@@ -135,6 +181,11 @@ struct mafm_ctx {
     DevBuf<int64_t> d_counters;
     DevBuf<double> d_tmp;
     DevBuf<float4> d_margins;      // per-position decision margins (MAFM_SCAN_MARGINS)
+    // streamed scans (mafm_scan_host): records and per-column flags in pinned host memory
+    HostBuf<DevRecord> h_rec;
+    HostBuf<int32_t> h_done;
+    bool host_pending = false;     // launched, not yet ended by mafm_scan_host_wait
+    PendingScan pending;
     // flexible scan: term list, index tables, initial positions, kept states, per-block work
     DevBuf<mafm_flex_term> d_fx_terms;
     DevBuf<int32_t> d_fx_ints;
@@ -534,6 +585,8 @@ static void destroy_now(mafm_ctx* c) {
     c->d_counters.release();
     c->d_tmp.release();
     c->d_margins.release();
+    c->h_rec.release();
+    c->h_done.release();
     c->d_fx_terms.release();
     c->d_fx_ints.release();
     c->d_fx_pos.release();
@@ -722,9 +775,71 @@ static mafm_status check_scan_desc(mafm_ctx* c, const mafm_scan_desc* d) {
     return MAFM_OK;
 }
 
+// Totals of a streamed scan's records on the host (what k_record_totals
+// computes on the device for the other scans): [0] minimiser iterations,
+// [1] unconverged positions, [2] grid-clamp warnings, [3] out-of-grid lookups.
+static void host_record_totals(const DevRecord* r, size_t n, int64_t* out) {
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < ((size_t)1 << 16)) nt = 1;
+    std::vector<std::array<int64_t, 4>> part(nt);
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nt; ++t)
+        pool.emplace_back([&, t]() {
+            int64_t s = 0, u = 0, w = 0, o = 0;
+            for (size_t i = n * t / nt, e = n * (t + 1) / nt; i < e; ++i) {
+                s += (uint32_t)r[i].n_steps;
+                u += (r[i].flags & REC_UNCONVERGED) != 0;
+                w += (r[i].flags & REC_GRID_CLAMP) != 0;
+                o += (r[i].flags & REC_OUT_OF_GRID) != 0;
+            }
+            part[t] = {s, u, w, o};
+        });
+    for (auto& th : pool) th.join();
+    for (auto& p : part)
+        for (int k = 0; k < 4; ++k) out[k] += p[k];
+}
+
+// The statistics of a finished scan from its counters and event times.
+static mafm_status scan_stats(mafm_ctx* c, const PendingScan& ps, const int64_t* counters, mafm_scan_stats* stats) {
+    const mafm_scan_desc* d = &ps.d;
+    const int n_mols = (int)c->mols.size();
+    const size_t per_mol = (size_t)d->nx * d->ny * d->nz;
+    const long cols_per_mol = (long)d->nx * d->ny;
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (stats) {
+        memset(stats, 0, sizeof *stats);
+        stats->kernel_ms = ms;
+        stats->positions = (int64_t)ps.n_rec;
+        stats->total_steps = counters[0];
+        stats->unconverged = counters[1];
+        stats->grid_warnings = counters[2];
+        int64_t evals_per_step = d->minimiser == MAFM_MIN_SD ? 1 : (d->integrator == 1 ? 2 : (d->integrator == 2 ? 4 : 1));
+        int64_t pe = 0;
+        for (int m = 0; m < n_mols; ++m) {
+            int64_t sites = c->mols[m].force_grid >= 0 ? 0 : (int64_t)c->mols[m].n_sites * c->mols[m].n_images;
+            pe += sites;
+        }
+        // algorithmic pair evaluations: (steps * evals + one getOutput eval) per position
+        stats->pair_evals = n_mols ? (counters[0] * evals_per_step + (int64_t)per_mol * n_mols) * (pe / n_mols) : 0;
+        stats->lanes_per_pixel = ps.fast ? ps.G : 1;
+        stats->group_trips = counters[4];
+        stats->active_group_trips = counters[5];
+        stats->wide_trips = counters[6];
+        stats->waves_per_simd = ps.fast ? ps.wps : 0;
+        stats->grid_blocks = ps.fast ? ps.n_blocks : (int)((cols_per_mol * n_mols + 255) / 256);
+        stats->block_threads = ps.fast ? 64 * ps.W : 256;
+    }
+    if (counters[3] != 0)
+        return fail(c, MAFM_ERR_OUT_OF_GRID, "%lld positions needed a force-grid point past the grid",
+                    (long long)counters[3]);
+    return MAFM_OK;
+}
+
 static mafm_status scan_impl(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* stats, DevRecord* out, size_t cap,
-                             ScanScratch* ws) {
+                             ScanScratch* ws, bool to_host = false) {
     if (!c || !d) return MAFM_ERR_INVALID;
+    if (c->host_pending) return fail(c, MAFM_ERR_STATE, "a streamed scan is still running (mafm_scan_host_wait ends it)");
     if (!c->have_systems) return fail(c, MAFM_ERR_STATE, "mafm_scan before mafm_set_systems");
     mafm_status sd = check_scan_desc(c, d);
     if (sd != MAFM_OK) return sd;
@@ -749,6 +864,8 @@ static mafm_status scan_impl(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_sta
         if (d->options & MAFM_SCAN_MARGINS)
             return fail(c, MAFM_ERR_INVALID, "the allocation-free scan records no decision margins");
     }
+    if (to_host && (d->options & MAFM_SCAN_MARGINS))
+        return fail(c, MAFM_ERR_INVALID, "a streamed scan records no decision margins");
     HIPCHK(c, hipSetDevice(c->device));
     for (int m = 0; m < n_mols; ++m) c->mols[m].rec_off = (int64_t)(per_mol * m);
     HIPCHK(c, hipMemcpyAsync(c->d_mols.p, c->mols.data(), n_mols * sizeof(DevMol), hipMemcpyHostToDevice, c->stream));
@@ -757,7 +874,16 @@ static mafm_status scan_impl(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_sta
     c->have_margins = false;
     c->last_flex = false;
     c->fx_state_atoms = c->fx_state_heights = 0;
-    if (out) {
+    if (to_host) {
+        // the records of this scan live in pinned host memory only
+        HIPCHK(c, c->h_rec.reserve(n_rec));
+        HIPCHK(c, c->h_done.reserve((size_t)cols_per_mol * n_mols));
+        HIPCHK(c, hipStreamSynchronize(c->stream));   // no earlier work still reads the flags
+        memset(c->h_done.p, 0, (size_t)cols_per_mol * n_mols * sizeof(int32_t));
+        c->have_scan = false;
+        c->n_records = 0;
+        out = c->h_rec.dp;
+    } else if (out) {
         c->have_scan = false;
         c->n_records = 0;
     } else {
@@ -774,6 +900,7 @@ static mafm_status scan_impl(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_sta
     a.minimiser = d->minimiser; a.integrator = d->integrator; a.minterm = d->minterm;
     a.maxsteps = d->maxsteps; a.etol = d->etol; a.ftol = d->ftol; a.dt = d->dt;
     a.rec = out;
+    if (to_host) a.col_done = c->h_done.dp;
     // last groups of a wave: regroup over 64 lanes once that halves the sites per lane
     a.wide_tail = (d->options & MAFM_SCAN_REPRODUCIBLE) ? 0 : 2;
     a.inv_2ftol = d->ftol > 0 ? 1.0 / (2.0 * d->ftol) : 0.0;
@@ -808,44 +935,66 @@ static mafm_status scan_impl(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_sta
                                    plan.lds_sites, plan.scalar_sites, plan.mid_family, c->stream));
     }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    PendingScan ps;
+    ps.d = *d;
+    ps.fast = fast;
+    ps.G = plan.G;
+    ps.wps = plan.wps;
+    ps.n_blocks = plan.n_blocks;
+    ps.W = plan.W;
+    ps.n_rec = n_rec;
+    if (to_host) {
+        // mafm_scan_host_wait finishes it
+        c->pending = ps;
+        c->host_pending = true;
+        return MAFM_OK;
+    }
     // the FAST kernels leave the step / convergence / grid totals to the records
     if (fast) HIPCHK(c, launch_record_totals(out, (long)n_rec, a.counters, c->stream));
     int64_t counters[8];
     HIPCHK(c, hipMemcpyAsync(counters, a.counters, sizeof counters, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    float ms = 0;
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    if (stats) {
-        memset(stats, 0, sizeof *stats);
-        stats->kernel_ms = ms;
-        stats->positions = (int64_t)n_rec;
-        stats->total_steps = counters[0];
-        stats->unconverged = counters[1];
-        stats->grid_warnings = counters[2];
-        int64_t evals_per_step = d->minimiser == MAFM_MIN_SD ? 1 : (d->integrator == 1 ? 2 : (d->integrator == 2 ? 4 : 1));
-        int64_t pe = 0;
-        for (int m = 0; m < n_mols; ++m) {
-            int64_t sites = c->mols[m].force_grid >= 0 ? 0 : (int64_t)c->mols[m].n_sites * c->mols[m].n_images;
-            pe += sites;
-        }
-        // algorithmic pair evaluations: (steps * evals + one getOutput eval) per position
-        stats->pair_evals = n_mols ? (counters[0] * evals_per_step + (int64_t)per_mol * n_mols) * (pe / n_mols) : 0;
-        stats->lanes_per_pixel = fast ? plan.G : 1;
-        stats->group_trips = counters[4];
-        stats->active_group_trips = counters[5];
-        stats->wide_trips = counters[6];
-        stats->waves_per_simd = fast ? plan.wps : 0;
-        stats->grid_blocks = fast ? plan.n_blocks : (int)((cols_per_mol * n_mols + 255) / 256);
-        stats->block_threads = fast ? 64 * plan.W : 256;
-    }
-    if (counters[3] != 0)
-        return fail(c, MAFM_ERR_OUT_OF_GRID, "%lld positions needed a force-grid point past the grid",
-                    (long long)counters[3]);
-    return MAFM_OK;
+    return scan_stats(c, ps, counters, stats);
 }
 
 mafm_status mafm_scan(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_stats* stats) {
     return scan_impl(c, d, stats, nullptr, 0, nullptr);
+}
+
+mafm_status mafm_scan_host(mafm_ctx* c, const mafm_scan_desc* d, const mafm_record** records,
+                           const int32_t** column_done) {
+    if (!c || !d || !records || !column_done) return MAFM_ERR_INVALID;
+    mafm_status s = scan_impl(c, d, nullptr, nullptr, 0, nullptr, true);
+    if (s != MAFM_OK) return s;
+    static_assert(sizeof(mafm_record) == sizeof(DevRecord), "record layout");
+    *records = reinterpret_cast<const mafm_record*>(c->h_rec.p);
+    *column_done = c->h_done.p;
+    return MAFM_OK;
+}
+
+mafm_status mafm_scan_host_query(mafm_ctx* c, int* finished) {
+    if (!c || !finished) return MAFM_ERR_INVALID;
+    if (!c->host_pending) return fail(c, MAFM_ERR_STATE, "no streamed scan is running");
+    hipError_t e = hipEventQuery(c->ev1);
+    if (e == hipErrorNotReady) {
+        *finished = 0;
+        return MAFM_OK;
+    }
+    HIPCHK(c, e);
+    *finished = 1;
+    return MAFM_OK;
+}
+
+mafm_status mafm_scan_host_wait(mafm_ctx* c, mafm_scan_stats* stats) {
+    if (!c) return MAFM_ERR_INVALID;
+    if (!c->host_pending) return fail(c, MAFM_ERR_STATE, "no streamed scan is running");
+    c->host_pending = false;
+    HIPCHK(c, hipSetDevice(c->device));
+    int64_t counters[8];
+    HIPCHK(c, hipMemcpyAsync(counters, c->d_counters.p, sizeof counters, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->pending.fast) host_record_totals(c->h_rec.p, c->pending.n_rec, counters);
+    return scan_stats(c, c->pending, counters, stats);
 }
 
 mafm_status mafm_scan_noalloc(mafm_ctx* c, const mafm_scan_desc* d, mafm_record* records, size_t capacity,

@@ -108,7 +108,18 @@ struct ScanArgs {
     // and FIRE's power P = v.f over |v|; per position the height's minima.
     float4* margins;          // null: not recorded
     double inv_2ftol;         // 1 / (2 ftol)
+    // Streamed scans: records go to pinned host memory, and after a pixel
+    // column's record of height k is stored, the column's counter (index
+    // molecule * nx * ny + column) becomes k + 1 with a system-scope release,
+    // so the host may read the column's first k + 1 records as soon as it
+    // sees the count.  Null: no counters.
+    int32_t* col_done;
 };
+
+// Heights [0, heights) of column `gcol` are in host memory (ScanArgs::col_done).
+__device__ inline void column_progress(const ScanArgs& a, long gcol, int heights) {
+    __hip_atomic_store(a.col_done + gcol, heights, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 enum { REC_UNCONVERGED = 1, REC_GRID_CLAMP = 2, REC_OUT_OF_GRID = 4 };
 

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <fstream>
 #include <sstream>
+#include <chrono>
 #include <condition_variable>
 #include <deque>
 #include <mutex>
@@ -1245,93 +1246,312 @@ static bool write_gzip(const std::string& path, const std::string& data) {
 
 void Simulation::writeOutput(int threads) {
     if (records_.empty()) fetchRecords();
-    const InputOptions& o = options_;
-    const int nx = n_points_[0], ny = n_points_[1], nz = n_points_[2];
+    const int nz = n_points_[2];
     std::vector<std::string> errors(nz);
-    // one file per height, records formatted as the reference prints them
-    // (simulation.cpp:180-195): "%d %d %d %6.3f %6.3f %6.3f %8.4f %8.4f %8.4f
-    // %6.3f %6.3f %6.3f %6.3f %8.4f %8.4f %d"
-    auto write_one = [&](int k) {
-        char name[512];
-        double z = o.zhigh - k * o.dz;
-        snprintf(name, sizeof name, "%sscan-%06.3f.dat", o.outputfolder.c_str(), z);
-        std::string body;
-        body.resize((size_t)nx * ny * 160 + 64);
-        char* p = &body[0];
-        for (int i = 0; i < nx; ++i)
-            for (int j = 0; j < ny; ++j) {
-                const mafm_record& r = records_[((size_t)i * ny + j) * nz + k];
-                const double rl = std::sqrt(std::pow(r.rx, 2) + std::pow(r.ry, 2) + std::pow(r.rz, 2));
-                const double ang = std::atan2(std::sqrt(std::pow(r.rx, 2) + std::pow(r.ry, 2)), r.rz) * (180.0 / PI);
-                p = fmt_int(p, k); *p++ = ' ';
-                p = fmt_int(p, i); *p++ = ' ';
-                p = fmt_int(p, j); *p++ = ' ';
-                p = fmt_fixed(p, r.x, 3, 6); *p++ = ' ';
-                p = fmt_fixed(p, r.y, 3, 6); *p++ = ' ';
-                p = fmt_fixed(p, r.z, 3, 6); *p++ = ' ';
-                p = fmt_fixed(p, r.fx, 4, 8); *p++ = ' ';
-                p = fmt_fixed(p, r.fy, 4, 8); *p++ = ' ';
-                p = fmt_fixed(p, r.fz, 4, 8); *p++ = ' ';
-                p = fmt_fixed(p, r.rx, 3, 6); *p++ = ' ';
-                p = fmt_fixed(p, r.ry, 3, 6); *p++ = ' ';
-                p = fmt_fixed(p, r.rz, 3, 6); *p++ = ' ';
-                p = fmt_fixed(p, rl, 3, 6); *p++ = ' ';
-                p = fmt_fixed(p, ang, 4, 8); *p++ = ' ';
-                p = fmt_fixed(p, r.e, 4, 8); *p++ = ' ';
-                p = fmt_int(p, r.n_steps); *p++ = '\n';
-                if ((size_t)(p - &body[0]) + 4096 > body.size()) {   // non-finite or huge values: grow
-                    const size_t used = (size_t)(p - &body[0]);
-                    body.resize(body.size() * 2);
-                    p = &body[0] + used;
-                }
-            }
-        body.resize((size_t)(p - &body[0]));
-        if (o.gzip) {
-            const std::string gz = std::string(name) + ".gz";
-            if (!write_gzip(gz, body)) errors[k] = "cannot write " + gz;
-            return;
-        }
-        FILE* f = fopen(name, "wb");
-        if (!f) {
-            errors[k] = std::string("cannot open output ") + name;
-            return;
-        }
-        if (fwrite(body.data(), 1, body.size(), f) != body.size()) errors[k] = std::string("short write to ") + name;
-        if (fclose(f) != 0) errors[k] = std::string("cannot close ") + name;
-    };
     if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
     threads = std::max(1, std::min(threads, nz));
     std::atomic<int> next{0};
     std::vector<std::thread> pool;
     for (int t = 0; t < threads; ++t)
         pool.emplace_back([&]() {
-            for (int k = next++; k < nz; k = next++) write_one(k);
+            for (int k = next++; k < nz; k = next++) errors[k] = writeHeight(k, records_.data());
         });
     for (auto& th : pool) th.join();
     for (const auto& e : errors)
         if (!e.empty()) fail(MAFM_ERR_INVALID, "%s", e.c_str());
-    if (o.statistics) {                           // finalize (mechafm.cpp:129-152)
-        int64_t n_pts = (int64_t)nx * ny * nz;
-        std::string path = o.outputfolder + "statistics.txt";
-        FILE* f = fopen(path.c_str(), "w");
-        if (f) {
-            fprintf(f, "Simulation run finished\n");
-            fprintf(f, "Statistics:\n");
-            fprintf(f, "    Computed %lld tip positions\n", (long long)n_pts);
-            // this job's own iterations (a batched launch's statistics cover all its jobs)
-            long long steps = 0;
-            for (const mafm_record& r : records_) steps += r.n_steps;
-            fprintf(f, "    Needed %lld minimization steps in total\n", steps);
-            fprintf(f, "    Which means approximately %.2f minimization steps per tip position\n",
-                    (double)steps / n_pts);
-            // the reference's two times (mechafm.cpp:137-150), from the job's opening
-            const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - opened_).count();
-            fprintf(f, "    The simulation wall time is %.2f seconds\n", wall);
-            fprintf(f, "    The entire simulation took %.2f seconds\n", wall);
-            fclose(f);
+    writeStatistics(records_.data());
+}
+
+// The lines of height k for pixel columns [c0, c1) (column i*ny + j),
+// appended to `out` as the reference prints them (simulation.cpp:180-195):
+// "%d %d %d %6.3f %6.3f %6.3f %8.4f %8.4f %8.4f %6.3f %6.3f %6.3f %6.3f %8.4f
+// %8.4f %d"
+void Simulation::formatLines(int k, const mafm_record* recs, size_t c0, size_t c1, std::string& out) const {
+    const int ny = n_points_[1], nz = n_points_[2];
+    size_t used = out.size();
+    out.resize(used + (c1 - c0) * 160 + 4096);
+    char* p = &out[0] + used;
+    for (size_t col = c0; col < c1; ++col) {
+        const int i = (int)(col / ny), j = (int)(col % ny);
+        const mafm_record& r = recs[col * nz + k];
+        const double rl = std::sqrt(std::pow(r.rx, 2) + std::pow(r.ry, 2) + std::pow(r.rz, 2));
+        const double ang = std::atan2(std::sqrt(std::pow(r.rx, 2) + std::pow(r.ry, 2)), r.rz) * (180.0 / PI);
+        p = fmt_int(p, k); *p++ = ' ';
+        p = fmt_int(p, i); *p++ = ' ';
+        p = fmt_int(p, j); *p++ = ' ';
+        p = fmt_fixed<3>(p, r.x, 6); *p++ = ' ';
+        p = fmt_fixed<3>(p, r.y, 6); *p++ = ' ';
+        p = fmt_fixed<3>(p, r.z, 6); *p++ = ' ';
+        p = fmt_fixed<4>(p, r.fx, 8); *p++ = ' ';
+        p = fmt_fixed<4>(p, r.fy, 8); *p++ = ' ';
+        p = fmt_fixed<4>(p, r.fz, 8); *p++ = ' ';
+        p = fmt_fixed<3>(p, r.rx, 6); *p++ = ' ';
+        p = fmt_fixed<3>(p, r.ry, 6); *p++ = ' ';
+        p = fmt_fixed<3>(p, r.rz, 6); *p++ = ' ';
+        p = fmt_fixed<3>(p, rl, 6); *p++ = ' ';
+        p = fmt_fixed<4>(p, ang, 8); *p++ = ' ';
+        p = fmt_fixed<4>(p, r.e, 8); *p++ = ' ';
+        p = fmt_int(p, r.n_steps); *p++ = '\n';
+        if ((size_t)(p - &out[0]) + 4096 > out.size()) {   // non-finite or huge values: grow
+            used = (size_t)(p - &out[0]);
+            out.resize(out.size() * 2);
+            p = &out[0] + used;
         }
     }
+    out.resize((size_t)(p - &out[0]));
 }
+
+std::string Simulation::heightPath(int k) const {
+    char name[512];
+    snprintf(name, sizeof name, "%sscan-%06.3f.dat", options_.outputfolder.c_str(), options_.zhigh - k * options_.dz);
+    return options_.gzip ? std::string(name) + ".gz" : std::string(name);
+}
+
+// One file per height, lines in (i, j) order.
+std::string Simulation::writeHeight(int k, const mafm_record* recs) const {
+    std::string body;
+    formatLines(k, recs, 0, (size_t)n_points_[0] * n_points_[1], body);
+    const std::string name = heightPath(k);
+    if (options_.gzip) return write_gzip(name, body) ? std::string() : "cannot write " + name;
+    FILE* f = fopen(name.c_str(), "wb");
+    if (!f) return "cannot open output " + name;
+    std::string err;
+    if (fwrite(body.data(), 1, body.size(), f) != body.size()) err = "short write to " + name;
+    if (fclose(f) != 0 && err.empty()) err = "cannot close " + name;
+    return err;
+}
+
+void Simulation::writeStatistics(const mafm_record* recs) const {
+    const InputOptions& o = options_;
+    if (!o.statistics) return;                    // finalize (mechafm.cpp:129-152)
+    const int64_t n_pts = (int64_t)n_points_[0] * n_points_[1] * n_points_[2];
+    std::string path = o.outputfolder + "statistics.txt";
+    FILE* f = fopen(path.c_str(), "w");
+    if (!f) return;
+    fprintf(f, "Simulation run finished\n");
+    fprintf(f, "Statistics:\n");
+    fprintf(f, "    Computed %lld tip positions\n", (long long)n_pts);
+    // this job's own iterations (a batched launch's statistics cover all its jobs)
+    long long steps = 0;
+    for (int64_t q = 0; q < n_pts; ++q) steps += recs[q].n_steps;
+    fprintf(f, "    Needed %lld minimization steps in total\n", steps);
+    fprintf(f, "    Which means approximately %.2f minimization steps per tip position\n", (double)steps / n_pts);
+    // the reference's two times (mechafm.cpp:137-150), from the job's opening
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - opened_).count();
+    fprintf(f, "    The simulation wall time is %.2f seconds\n", wall);
+    fprintf(f, "    The entire simulation took %.2f seconds\n", wall);
+    fclose(f);
+}
+
+// ---------------------------------------------------------------- batch writer
+// Writes the per-height files of many jobs on one pool of threads, a chunk of
+// pixel columns at a time.  The lines of one chunk at one height are
+// formatted as soon as the chunk's records of that height are on the host
+// (ready()); with
+// gzip on, each such segment is deflated on its own as raw deflate ending on
+// a byte boundary (Z_SYNC_FLUSH; the file's last segment Z_FINISH), so the
+// segments of a file concatenate into one deflate stream -- the layout of
+// pigz -- behind one gzip header, with the CRC-32 combined from the
+// segments'.  A file is written by whichever thread completes its last
+// segment.
+class BatchWriter {
+  public:
+    // recs[m]: job m's records in mafm_records' order; copy: also copy them
+    // into the job's host records once the whole job is in
+    BatchWriter(std::vector<Simulation*> sims, std::vector<const mafm_record*> recs, int threads, bool copy)
+        : sims_(std::move(sims)), recs_(std::move(recs)), copy_(copy) {
+        const int n = (int)sims_.size();
+        jobs_.resize(n);
+        for (int m = 0; m < n; ++m) {
+            Job& j = jobs_[m];
+            const Simulation& s = *sims_[m];
+            j.cols = (size_t)s.n_points_[0] * s.n_points_[1];
+            j.nz = s.n_points_[2];
+            j.chunk = std::max<size_t>(256, (j.cols + 63) / 64);
+            j.n_chunks = (int)((j.cols + j.chunk - 1) / j.chunk);
+            j.queued.assign(j.n_chunks, 0);
+            j.files.reset(new File[j.nz]);
+            for (int k = 0; k < j.nz; ++k) {
+                j.files[k].seg.resize(j.n_chunks);
+                j.files[k].crc.resize(j.n_chunks);
+                j.files[k].len.resize(j.n_chunks);
+            }
+        }
+        if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+        threads = std::max(1, std::min(threads, 64));
+        for (int t = 0; t < threads; ++t) pool_.emplace_back([this]() { work(); });
+    }
+    ~BatchWriter() { close(); }
+
+    int chunks(int m) const { return jobs_[m].n_chunks; }
+    // pixel columns [first, last) of chunk c of job m
+    size_t chunk_first(int m, int c) const { return (size_t)c * jobs_[m].chunk; }
+    size_t chunk_last(int m, int c) const { return std::min(jobs_[m].cols, (size_t)(c + 1) * jobs_[m].chunk); }
+    // heights of chunk c of job m handed to the writers so far
+    int queued(int m, int c) const { return jobs_[m].queued[c]; }
+    // every height of every chunk of job m has been handed over
+    bool complete(int m) const { return jobs_[m].full == jobs_[m].n_chunks; }
+
+    // the records of heights [0, heights) of every column of chunk c of job
+    // m are on the host (called from one thread)
+    void ready(int m, int c, int heights) {
+        Job& j = jobs_[m];
+        heights = std::min(heights, j.nz);
+        if (heights <= j.queued[c]) return;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (int k = j.queued[c]; k < heights; ++k) tasks_.push_back({m, c, k});
+            if (heights == j.nz && ++j.full == j.n_chunks) tasks_.push_back({m, -1, -1});   // copy, statistics
+        }
+        j.queued[c] = heights;
+        cv_.notify_all();
+    }
+
+    // waits for every queued task; throws the first error
+    void finish() {
+        close();
+        if (!err_.empty()) throw Error(MAFM_ERR_INVALID, err_);
+    }
+
+  private:
+    struct File {
+        std::vector<std::string> seg;
+        std::vector<uint32_t> crc;
+        std::vector<size_t> len;
+        std::atomic<int> done{0};
+    };
+    struct Job {
+        size_t cols = 0, chunk = 1;
+        int nz = 0, n_chunks = 0, full = 0;   // full: chunks with every height handed over
+        std::vector<int> queued;
+        std::unique_ptr<File[]> files;
+    };
+    struct Task {
+        int m, c, k;
+    };
+
+    void close() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            closing_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : pool_) t.join();
+        pool_.clear();
+    }
+
+    void error(const std::string& e) {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (err_.empty()) err_ = e;
+    }
+
+    void work() {
+        for (;;) {
+            Task t;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&]() { return closing_ || !tasks_.empty(); });
+                if (tasks_.empty()) return;
+                t = tasks_.front();
+                tasks_.pop_front();
+            }
+            try {
+                run(t);
+            } catch (const std::exception& e) {
+                error(e.what());
+            }
+        }
+    }
+
+    void run(const Task& t) {
+        Job& j = jobs_[t.m];
+        Simulation& sim = *sims_[t.m];
+        const mafm_record* r = recs_[t.m];
+        if (t.c < 0) {
+            if (copy_) {
+                sim.records_.assign(r, r + j.cols * j.nz);
+                sim.host_records_ = true;
+            }
+            sim.writeStatistics(r);
+            return;
+        }
+        File& f = j.files[t.k];
+        const size_t c0 = (size_t)t.c * j.chunk, c1 = std::min(j.cols, c0 + j.chunk);
+        std::string text;
+        sim.formatLines(t.k, r, c0, c1, text);
+        f.len[t.c] = text.size();
+        if (sim.options_.gzip) {
+            f.crc[t.c] = (uint32_t)crc32(0L, (const Bytef*)text.data(), (uInt)text.size());
+            f.seg[t.c] = deflate_segment(text, t.c + 1 == j.n_chunks);
+        } else {
+            f.seg[t.c] = std::move(text);
+        }
+        if (f.done.fetch_add(1) + 1 == j.n_chunks) write_file(sim, t.k, f);
+    }
+
+    static std::string deflate_segment(const std::string& in, bool last) {
+        z_stream zs{};
+        if (deflateInit2(&zs, 6, Z_DEFLATED, -15, 8, Z_DEFAULT_STRATEGY) != Z_OK)
+            throw std::runtime_error("deflateInit2 failed");
+        std::string out;
+        out.resize(deflateBound(&zs, (uLong)in.size()) + 16);
+        zs.next_in = (Bytef*)in.data();
+        zs.avail_in = (uInt)in.size();
+        zs.next_out = (Bytef*)&out[0];
+        zs.avail_out = (uInt)out.size();
+        const int rc = deflate(&zs, last ? Z_FINISH : Z_SYNC_FLUSH);
+        const size_t have = out.size() - zs.avail_out;
+        deflateEnd(&zs);
+        if ((last && rc != Z_STREAM_END) || (!last && (rc != Z_OK || zs.avail_in != 0)))
+            throw std::runtime_error("deflate failed");
+        out.resize(have);
+        return out;
+    }
+
+    void write_file(const Simulation& sim, int k, File& f) {
+        const std::string name = sim.heightPath(k);
+        FILE* fp = fopen(name.c_str(), "wb");
+        if (!fp) {
+            error("cannot open output " + name);
+            return;
+        }
+        bool ok = true;
+        const bool gz = sim.options_.gzip;
+        if (gz) {   // gzip header: deflate, no name, mtime 0, Unix (what zlib's gzip wrapper writes)
+            static const unsigned char hdr[10] = {0x1f, 0x8b, 8, 0, 0, 0, 0, 0, 0, 3};
+            ok = fwrite(hdr, 1, sizeof hdr, fp) == sizeof hdr;
+        }
+        uLong crc = 0;
+        uint64_t total = 0;
+        for (size_t c = 0; c < f.seg.size() && ok; ++c) {
+            ok = fwrite(f.seg[c].data(), 1, f.seg[c].size(), fp) == f.seg[c].size();
+            crc = c == 0 ? f.crc[0] : crc32_combine(crc, f.crc[c], (z_off_t)f.len[c]);
+            total += f.len[c];
+            std::string().swap(f.seg[c]);
+        }
+        if (gz && ok) {   // trailer: CRC-32 and length mod 2^32, little-endian
+            unsigned char tr[8];
+            for (int b = 0; b < 4; ++b) {
+                tr[b] = (unsigned char)(crc >> (8 * b));
+                tr[4 + b] = (unsigned char)(total >> (8 * b));
+            }
+            ok = fwrite(tr, 1, 8, fp) == 8;
+        }
+        if (fclose(fp) != 0) ok = false;
+        if (!ok) error("cannot write " + name);
+    }
+
+    std::vector<Simulation*> sims_;
+    std::vector<const mafm_record*> recs_;
+    bool copy_;
+    std::vector<Job> jobs_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<Task> tasks_;
+    bool closing_ = false;
+    std::string err_;
+    std::vector<std::thread> pool_;
+};
 
 void Simulation::writeStates() {                  // system.cpp:54-65
     if (!options_.flexible || flex_desc_.state_column < 0) return;
@@ -1573,117 +1793,132 @@ mafm_status mafm_batch_run(int n, mafm_sim* const* sims, const mafm_run_opts* o,
     });
 }
 
-mafm_status mafm_batch_run_output(int n, mafm_sim* const* sims, const mafm_run_opts* o, int groups, int threads,
+mafm_status mafm_batch_run_output(int n, mafm_sim* const* sims, const mafm_run_opts* o, int threads,
                                   mafm_scan_stats* stats) {
     if (n <= 0 || !sims || !sims[0]) return MAFM_ERR_INVALID;
     mafm_run_opts def{};
     def.precision = MAFM_PREC_FAST64;
     return guarded(sims[0], [&]() {
         const mafm_scan_desc d0 = batch_desc(n, sims, o ? *o : def);
-        // two launches by default: a launch needs a few columns per resident
-        // pixel group to stay busy, so more launches slow the scan
-        if (groups <= 0) groups = 2;
-        groups = std::max(1, std::min(groups, n));
-        // contiguous groups of members, balanced by sites (the scan's work)
-        std::vector<double> cost(n);
-        double total = 0;
+        mafm_ctx* ctx = sims[0]->sim.ctx_;
+        std::vector<mafm_system_desc> descs;
         for (int m = 0; m < n; ++m) {
-            const mafm_system_desc& sd = sims[m]->sim.systemDesc();
-            cost[m] = sd.force_grid >= 0 ? 1.0 : 1.0 + (double)sd.n_sites * std::max(1, (int)sd.n_images);
-            total += cost[m];
+            descs.push_back(sims[m]->sim.systemDesc());
+            sims[m]->sim.records_.clear();
+            sims[m]->sim.host_records_ = false;
+            sims[m]->sim.scan_generation_ = 0;
         }
-        std::vector<int> bounds{0};
-        double acc = 0;
+        mafm_status st = mafm_set_systems(ctx, n, descs.data());
+        if (st != MAFM_OK) throw Error(st, mafm_last_error(ctx));
+        const mafm_record* recs = nullptr;
+        const int32_t* done = nullptr;
+        st = mafm_scan_host(ctx, &d0, &recs, &done);
+        if (st != MAFM_OK) throw Error(st, mafm_last_error(ctx));
+        struct EndScan {   // the launch is ended on every way out
+            mafm_ctx* c;
+            bool armed = true;
+            ~EndScan() {
+                mafm_scan_stats ignored;
+                if (armed) (void)mafm_scan_host_wait(c, &ignored);
+            }
+        } end_scan{ctx};
+        const size_t cols = (size_t)d0.nx * d0.ny, per = cols * d0.nz;
+
+        // the writers format each chunk of columns, height by height, as soon
+        // as it is in host memory, straight from the pinned records, while the
+        // rest scans
+        std::vector<Simulation*> jobs;
+        std::vector<const mafm_record*> rp;
         for (int m = 0; m < n; ++m) {
-            acc += cost[m];
-            if ((int)bounds.size() < groups && acc >= total * bounds.size() / groups && m + 1 < n)
-                bounds.push_back(m + 1);
+            jobs.push_back(&sims[m]->sim);
+            rp.push_back(recs + per * m);
         }
-        bounds.push_back(n);
-        // the writer: a thread that writes each finished group's files while
-        // the next group scans (each job's heights on `threads` threads)
-        std::mutex mu;
-        std::condition_variable cv;
-        std::deque<std::pair<int, int>> ready;
-        bool done = false;
-        std::string werr;
-        mafm_status wst = MAFM_OK;
-        std::thread writer([&]() {
-            for (;;) {
-                std::pair<int, int> g;
-                {
-                    std::unique_lock<std::mutex> lk(mu);
-                    cv.wait(lk, [&]() { return done || !ready.empty(); });
-                    if (ready.empty()) return;
-                    g = ready.front();
-                    ready.pop_front();
-                }
-                for (int m = g.first; m < g.second; ++m) {
-                    try {
-                        sims[m]->sim.writeOutput(threads);
-                    } catch (const Error& e) {
-                        std::lock_guard<std::mutex> lk(mu);
-                        if (wst == MAFM_OK) {
-                            wst = e.status;
-                            werr = e.what();
-                        }
-                    } catch (const std::exception& e) {
-                        std::lock_guard<std::mutex> lk(mu);
-                        if (wst == MAFM_OK) {
-                            wst = MAFM_ERR_INVALID;
-                            werr = e.what();
-                        }
+        mafm::BatchWriter writer(jobs, rp, threads, true);
+        // poll the column counters (acquire: a column's records of heights
+        // [0, count) are visible once the count is); each chunk of columns
+        // goes to the writers height by height, as its slowest column passes
+        const int nz = d0.nz;
+        std::vector<char> job_done(n, 0);
+        int complete = 0;
+        bool finished = false;
+        mafm_status sst = MAFM_OK;
+        std::string serr;
+        while (complete < n) {
+            bool progress = false;
+            for (int m = 0; m < n; ++m) {
+                if (job_done[m]) continue;
+                const int32_t* f = done + cols * m;
+                for (int c = 0; c < writer.chunks(m); ++c) {
+                    const int q = writer.queued(m, c);
+                    if (q == nz) continue;
+                    int h = nz;
+                    for (size_t col = writer.chunk_first(m, c), e = writer.chunk_last(m, c); col < e && h > q; ++col)
+                        h = std::min(h, __atomic_load_n(f + col, __ATOMIC_ACQUIRE));
+                    if (h > q) {
+                        writer.ready(m, c, h);
+                        progress = true;
                     }
                 }
-            }
-        });
-        mafm_scan_stats sum{};
-        std::string serr;
-        mafm_status sst = MAFM_OK;
-        for (size_t b = 0; b + 1 < bounds.size(); ++b) {
-            const int m0 = bounds[b], m1 = bounds[b + 1];
-            mafm_scan_stats ss{};
-            try {
-                batch_scan(m1 - m0, sims + m0, d0, ss);
-                for (int m = m0; m < m1; ++m) {   // to the host before the next group replaces them
-                    sims[m]->sim.fetchRecords();
-                    sims[m]->sim.host_records_ = true;
-                    sims[m]->sim.scan_generation_ = 0;
+                if (writer.complete(m)) {
+                    job_done[m] = 1;
+                    ++complete;
                 }
-            } catch (const Error& e) {
-                sst = e.status;
-                serr = e.what();
+            }
+            if (complete == n || progress) continue;
+            if (finished) {
+                sst = MAFM_ERR_HIP;
+                serr = "the streamed scan finished with columns unmarked";
                 break;
             }
-            sum.kernel_ms += ss.kernel_ms;
-            sum.positions += ss.positions;
-            sum.total_steps += ss.total_steps;
-            sum.unconverged += ss.unconverged;
-            sum.grid_warnings += ss.grid_warnings;
-            sum.pair_evals += ss.pair_evals;
-            sum.lanes_per_pixel = ss.lanes_per_pixel;
-            sum.waves_per_simd = ss.waves_per_simd;
-            sum.grid_blocks = std::max(sum.grid_blocks, ss.grid_blocks);
-            sum.block_threads = ss.block_threads;
-            sum.group_trips += ss.group_trips;
-            sum.active_group_trips += ss.active_group_trips;
-            sum.wide_trips += ss.wide_trips;
-            {
-                std::lock_guard<std::mutex> lk(mu);
-                ready.emplace_back(m0, m1);
+            int fin = 0;
+            st = mafm_scan_host_query(ctx, &fin);
+            if (st != MAFM_OK) {
+                sst = st;
+                serr = mafm_last_error(ctx);
+                break;
             }
-            cv.notify_one();
+            if (fin) finished = true;   // one more pass reads the last counters
+            else std::this_thread::sleep_for(std::chrono::microseconds(200));
         }
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            done = true;
+        mafm_scan_stats ss{};
+        end_scan.armed = false;
+        const mafm_status est = mafm_scan_host_wait(ctx, &ss);
+        const std::string eerr = est != MAFM_OK ? mafm_last_error(ctx) : "";
+        std::string werr;
+        try {
+            writer.finish();
+        } catch (const Error& e) {
+            werr = e.what();
         }
-        cv.notify_one();
-        writer.join();
         if (sst != MAFM_OK) throw Error(sst, serr);
-        if (wst != MAFM_OK) throw Error(wst, werr);
-        for (int m = 0; m < n; ++m) sims[m]->sim.last_stats_ = sum;
-        if (stats) *stats = sum;
+        if (est != MAFM_OK) throw Error(est, eerr);
+        if (!werr.empty()) throw Error(MAFM_ERR_INVALID, werr);
+        for (int m = 0; m < n; ++m) {
+            sims[m]->sim.record_base_ = per * m;
+            sims[m]->sim.system_index_ = m;
+            sims[m]->sim.last_stats_ = ss;
+        }
+        if (stats) *stats = ss;
+    });
+}
+
+mafm_status mafm_batch_write_output(int n, mafm_sim* const* sims, int threads) {
+    if (n <= 0 || !sims || !sims[0]) return MAFM_ERR_INVALID;
+    return guarded(sims[0], [&]() {
+        std::vector<Simulation*> jobs;
+        std::vector<const mafm_record*> rp;
+        for (int m = 0; m < n; ++m) {
+            if (!sims[m]) throw Error(MAFM_ERR_INVALID, "batch member is NULL");
+            Simulation& s = sims[m]->sim;
+            if (s.options_.flexible) throw Error(MAFM_ERR_INVALID, "flexible jobs write with mafm_sim_write_output");
+            if (s.records_.empty()) s.fetchRecords();
+            jobs.push_back(&s);
+            rp.push_back(s.records_.data());
+        }
+        mafm::BatchWriter writer(jobs, rp, threads, false);
+        for (int m = 0; m < n; ++m)
+            for (int c = 0; c < writer.chunks(m); ++c) writer.ready(m, c, jobs[m]->n_points_[2]);
+        writer.finish();
     });
 }
 

@@ -356,6 +356,15 @@ class Simulation {
     void run(const mafm_run_opts& opts, mafm_scan_stats* stats);
     void fetchRecords();
     void writeOutput(int threads);
+    // the file of height k from this job's records `recs` (records_ or
+    // pinned host memory); returns an error text, empty on success
+    std::string writeHeight(int k, const mafm_record* recs) const;
+    // the lines of height k for pixel columns [c0, c1), appended to out
+    void formatLines(int k, const mafm_record* recs, size_t c0, size_t c1, std::string& out) const;
+    // scan-%06.3f.dat[.gz] of height k in the output folder
+    std::string heightPath(int k) const;
+    // statistics.txt (mechafm.cpp:129-152) when the options ask for it
+    void writeStatistics(const mafm_record* recs) const;
     // Delta f images (simple-vafm-image.py:76-204) from this job's records,
     // one deltaf-%06.3f.dat per image named by its lowest height
     void writeDeltaF(const mafm_deltaf_desc& d);

@@ -901,6 +901,7 @@ __device__ __attribute__((always_inline)) inline void scan_fast_body(const ScanA
                 const long ri = M->rec_off + ((long)col * a.nz + k);
                 a.rec[ri] = rec;
                 if (a.margins) a.margins[ri] = make_float4(mgf, mge, mgp, 0.f);
+                if (a.col_done) column_progress(a, (long)m * n_cols + col, k + 1);
             }
             dmz -= a.dz;
             tz -= a.dz;

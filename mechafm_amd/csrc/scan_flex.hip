@@ -484,7 +484,6 @@ size_t flex_block_lds(const FlexArgs& a) {
 }
 
 hipError_t launch_scan_flex(const FlexArgs& a, int n_blocks, int waves, bool in_lds, hipStream_t st) {
-    const bool rk4 = a.minimiser == MAFM_MIN_FIRE && a.integrator == MAFM_INT_RK4;
     if (in_lds) {
         const size_t lds = flex_block_lds(a);
         // above 64 KiB a workgroup asks for the larger allocation (gfx950: 160 KiB)

@@ -305,6 +305,7 @@ __global__ void __launch_bounds__(256) k_scan_plain(ScanArgs a) {
         r.n_steps = n;
         r.flags = c.flags | (n >= a.maxsteps ? REC_UNCONVERGED : 0);
         a.rec[m.rec_off + (local * a.nz + k)] = r;
+        if (a.col_done) column_progress(a, col, k + 1);
         steps += n;
         unconv += n >= a.maxsteps;
         warn += (c.flags & REC_GRID_CLAMP) != 0;

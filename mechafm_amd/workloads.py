@@ -397,8 +397,16 @@ def c3_molecules(seed: int = 1237, count: int = 64, nmin: int = 20, nmax: int = 
     return [planar_molecule(int(n), rng) for n in sizes]
 
 
-def make_c3(outdir: str, mini: bool = False, seed: int = 1237, count: int = 64, **over) -> List[Workload]:
+def make_c3(outdir: str, mini: bool = False, seed: int = 1237, count: int = 64, jitter: float = 0.0,
+            **over) -> List[Workload]:
+    """The 64-molecule batch.  jitter > 0 moves every atom by a seeded
+    uniform offset in [-jitter, jitter] A per coordinate (the xyz files keep
+    8 decimals, so 1e-8 is the smallest change they can carry)."""
     mols = c3_molecules(seed, count)
+    if jitter > 0:
+        rng = np.random.default_rng(seed + 7919)
+        for mol in mols:
+            mol.pos = mol.pos + rng.uniform(-jitter, jitter, size=mol.pos.shape)
     out = []
     for m, mol in enumerate(mols):
         opts = _base_options(coulomb="on", area="6.35 6.35", zlow="6.1")

@@ -59,6 +59,9 @@ def test_create_without_device(lib):
 def test_null_arguments_rejected(lib):
     assert lib.mafm_scan(None, None, None) == 1           # MAFM_ERR_INVALID
     assert lib.mafm_sim_open(None, None, None, None) == 1
+    assert lib.mafm_scan_host(None, None, None, None) == 1
+    assert lib.mafm_scan_host_query(None, None) == 1
+    assert lib.mafm_scan_host_wait(None, None) == 1
 
 
 STRUCTS = [("mafm_site", "Site"), ("mafm_tip_dummy", "TipDummy"), ("mafm_system_desc", "SystemDesc"),

@@ -1,13 +1,21 @@
-"""Output written while the batch scans (mafm_batch_run_output).
+"""Output written while the batch scans.
 
-The 64 molecules of the batched workload (2 x 2 pixels each, all 40
-heights) are scanned in groups while a writer thread writes the finished
-groups' per-height files.  Every file, and every record the jobs then serve
-from the host, equals what a plain batch run followed by mafm_sim_write_output
-gives; statistics.txt counts each job's own iterations."""
+* mafm_scan_host (the streamed scan) through the C ABI: the records it
+  stores in pinned host memory equal mafm_scan's bit for bit (reproducible
+  mode) in every precision, every column's height count is nz once the scan
+  ends, and the context refuses another scan while one is streaming.
+* mafm_batch_run_output: the 64 molecules of the batched workload (2 x 2
+  pixels each, all 40 heights) are scanned in one launch while a pool of
+  writers formats every chunk of columns, height by height, as it lands.
+  Every file (gzip: one member each), and every record the jobs then serve
+  from the host, equals what a plain batch run followed by
+  mafm_sim_write_output gives; statistics.txt counts each job's own
+  iterations.  mafm_batch_write_output writes the same files again."""
 import gzip
 import os
 import re
+import time
+import zlib
 
 import numpy as np
 import pytest
@@ -18,12 +26,57 @@ from mechafm_amd import abi
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("precision", ["fast64", "fast32", "plain64"])
+def test_streamed_scan_equals_scan(gpu_ctx, tmp_path, precision):
+    from mechafm_amd import workloads as W
+    wls = W.make_c3(str(tmp_path), mini=True, count=6)
+    sims = [abi.Simulation(w.input_path, gpu_ctx) for w in wls]
+    descs = [s.system() for s in sims]
+    d = sims[0].scan_desc(precision, lanes_per_pixel=2, options=abi.SCAN_REPRODUCIBLE)
+    n = len(descs) * d.nx * d.ny * d.nz
+    gpu_ctx.set_systems(descs)
+    want_st = gpu_ctx.scan(d)
+    want = gpu_ctx.records(0, n)
+    for rep in range(2):   # the pinned buffers are reused by the second call
+        recs, flags = gpu_ctx.scan_host(d, len(descs))
+        with pytest.raises(abi.MafmError, match="still running"):
+            gpu_ctx.scan(d)
+        t0 = time.time()
+        while not gpu_ctx.scan_host_query():   # the launch ends by itself
+            assert time.time() - t0 < 60
+            time.sleep(0.001)
+        st = gpu_ctx.scan_host_wait()
+        print(f"{precision} streamed scan {rep + 1}: {n} positions, {st['kernel_ms']:.2f} ms")
+        with pytest.raises(abi.MafmError, match="no streamed scan"):
+            gpu_ctx.scan_host_query()
+        assert flags.shape == (len(descs) * d.nx * d.ny,) and (flags == d.nz).all()
+        assert (recs == want).all()
+        for key in ("positions", "total_steps", "unconverged", "grid_warnings"):
+            assert st[key] == want_st[key], key
+        # the context holds no device records after a streamed scan
+        with pytest.raises(abi.MafmError):
+            gpu_ctx.records(0, 1)
+        with pytest.raises(abi.MafmError, match="no streamed scan"):
+            gpu_ctx.scan_host_wait()
+    for s in sims:
+        s.close()
+
+
+def _one_member(raw):
+    """The text of a gzip file that must be a single gzip member (one
+    header, one deflate stream, one CRC-32 / length trailer)."""
+    d = zlib.decompressobj(wbits=31)
+    text = d.decompress(raw) + d.flush()
+    assert d.eof and d.unused_data == b"", "not a single gzip member"
+    return text
+
+
 def _files(folder):
     out = {}
     for fn in sorted(os.listdir(folder)):
         if fn.startswith("scan-"):
             raw = open(os.path.join(folder, fn), "rb").read()
-            out[fn.replace(".gz", "")] = gzip.decompress(raw) if fn.endswith(".gz") else raw
+            out[fn.replace(".gz", "")] = _one_member(raw) if fn.endswith(".gz") else raw
     return out
 
 
@@ -46,7 +99,7 @@ def test_streamed_output_equals_write_after_scan(gpu_ctx, tmp_path, gzip_on):
     for s in ref_sims:
         s.write_output(threads=4)
     sims = _jobs(gpu_ctx, inputs, str(tmp_path / "out"), gzip_on)
-    st = abi.batch_run_output(sims, "fast64", groups=5, threads=4, options=abi.SCAN_REPRODUCIBLE)
+    st = abi.batch_run_output(sims, "fast64", threads=4, options=abi.SCAN_REPRODUCIBLE)
     assert st["positions"] == sum(len(r) for r in ref_recs)
     assert st["total_steps"] == sum(int(r["n_steps"].sum()) for r in ref_recs)
     for m, (s, rr) in enumerate(zip(sims, ref_recs)):
@@ -55,9 +108,15 @@ def test_streamed_output_equals_write_after_scan(gpu_ctx, tmp_path, gzip_on):
         assert len(a) == 40 and a == b
         stats = open(tmp_path / "out" / f"mol{m:02d}" / "statistics.txt").read().splitlines()
         assert stats[3] == f"    Needed {int(rr['n_steps'].sum())} minimization steps in total"
-    # the files can be written again from the host records
+    # the files can be written again from the host records, one job or a batch
     sims[1].write_output(threads=2)
     assert _files(str(tmp_path / "out" / "mol01")) == _files(str(tmp_path / "ref" / "mol01"))
+    for m in (0, 5):
+        for fn in os.listdir(tmp_path / "out" / f"mol{m:02d}"):
+            os.remove(tmp_path / "out" / f"mol{m:02d}" / fn)
+    abi.batch_write_output(sims, threads=3)
+    for m in (0, 5, 63):
+        assert _files(str(tmp_path / "out" / f"mol{m:02d}")) == _files(str(tmp_path / "ref" / f"mol{m:02d}"))
     # the records now live on the host: device-side post-processing asks for another run
     with pytest.raises(abi.MafmError, match="on the host only"):
         sims[0].write_deltaf()
@@ -71,6 +130,6 @@ def test_streamed_output_reports_a_bad_folder(gpu_ctx, tmp_path):
         if os.access(tmp_path / "out" / "mol01", os.W_OK):
             pytest.skip("running with permissions that ignore the read-only folder")
         with pytest.raises(abi.MafmError, match="cannot open output"):
-            abi.batch_run_output(sims, "fast64", groups=3)
+            abi.batch_run_output(sims, "fast64")
     finally:
         os.chmod(tmp_path / "out" / "mol01", 0o700)
