@@ -491,9 +491,9 @@ __device__ void grid_lookup(const DevGrid& g, double px, double py, double pz, d
     int i0[3], i1[3];
     for (int d = 0; d < 3; ++d) {
         int a0 = gp[d], a1 = gp[d] + 1;
-        if (g.periodic) {
-            a0 -= (int)floor((double)a0 / g.n[d]) * g.n[d];
-            a1 -= (int)floor((double)a1 / g.n[d]) * g.n[d];
+        if (g.periodic) {   // pbcGridPoint; the division only for points outside the cell
+            if (a0 < 0 || a0 >= g.n[d]) a0 -= (int)floor((double)a0 / g.n[d]) * g.n[d];
+            if (a1 < 0 || a1 >= g.n[d]) a1 -= (int)floor((double)a1 / g.n[d]) * g.n[d];
         } else if (a1 >= g.n[d]) {
             flags |= REC_OUT_OF_GRID;
             a1 = g.n[d] - 1;
@@ -503,20 +503,31 @@ __device__ void grid_lookup(const DevGrid& g, double px, double py, double pz, d
         i1[d] = a1 < 0 ? 0 : (a1 >= g.n[d] ? g.n[d] - 1 : a1);
     }
     const double w[3] = {q[0] - gp[0], q[1] - gp[1], q[2] - gp[2]};
+    // corners are summed in the order c = 0..7 (x from bit 2, y from bit 1,
+    // z from bit 0)
     if (F32) {
+        // one index for corner (i0, i0, i0) and one stride per axis (0 where
+        // the upper corner is clamped onto the lower, negative where it
+        // wraps): the eight addresses cost seven additions, and the eight
+        // gathers are issued together
+        const long sy = g.n[2], sx = (long)g.n[1] * g.n[2];
+        const long base = i0[0] * sx + i0[1] * sy + i0[2];
+        const long dx = (i1[0] - i0[0]) * sx, dy = (i1[1] - i0[1]) * sy, dz = i1[2] - i0[2];
+        const long off[8] = {0, dz, dy, dy + dz, dx, dx + dz, dx + dy, dx + dy + dz};
         const float wf[3] = {(float)w[0], (float)w[1], (float)w[2]};
+        const float4* pt = g.fe32 + base;
+        float4 v[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) v[c] = pt[off[c]];
         float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
         for (int c = 0; c < 8; ++c) {
-            const int ix = (c & 4) ? i1[0] : i0[0];
-            const int iy = (c & 2) ? i1[1] : i0[1];
-            const int iz = (c & 1) ? i1[2] : i0[2];
             const float wt = ((c & 4) ? wf[0] : 1.f - wf[0]) * ((c & 2) ? wf[1] : 1.f - wf[1]) *
                              ((c & 1) ? wf[2] : 1.f - wf[2]);
-            const float4 v = g.fe32[((long)ix * g.n[1] + iy) * g.n[2] + iz];
-            acc[0] = fmaf(wt, v.x, acc[0]);
-            acc[1] = fmaf(wt, v.y, acc[1]);
-            acc[2] = fmaf(wt, v.z, acc[2]);
-            acc[3] = fmaf(wt, v.w, acc[3]);
+            acc[0] = fmaf(wt, v[c].x, acc[0]);
+            acc[1] = fmaf(wt, v[c].y, acc[1]);
+            acc[2] = fmaf(wt, v[c].z, acc[2]);
+            acc[3] = fmaf(wt, v[c].w, acc[3]);
         }
         for (int k = 0; k < 4; ++k) out[k] += (double)acc[k];
     } else {
@@ -526,8 +537,7 @@ __device__ void grid_lookup(const DevGrid& g, double px, double py, double pz, d
             const int iy = (c & 2) ? i1[1] : i0[1];
             const int iz = (c & 1) ? i1[2] : i0[2];
             const double wt = ((c & 4) ? w[0] : 1 - w[0]) * ((c & 2) ? w[1] : 1 - w[1]) * ((c & 1) ? w[2] : 1 - w[2]);
-            const long idx = ((long)ix * g.n[1] + iy) * g.n[2] + iz;
-            const double4 v = g.fe64[idx];
+            const double4 v = g.fe64[((long)ix * g.n[1] + iy) * g.n[2] + iz];
             acc[0] = fma(wt, v.x, acc[0]);
             acc[1] = fma(wt, v.y, acc[1]);
             acc[2] = fma(wt, v.z, acc[2]);
