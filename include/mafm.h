@@ -38,6 +38,11 @@
  *                          (simulation.cpp:201-216, 503-693) relaxed per
  *                          position; mafm_flex_state keeps what
  *                          System::makeXYZFile writes (system.cpp:54-65)
+ *   mafm_system_eval /     Interaction::eval over a System's list, eulerStep /
+ *   mafm_system_step /     midpointStep / rk4Step and FIREMinimisation /
+ *   mafm_system_minimise   SDMinimisation (interactions.hpp:68-76,
+ *                          integrators.hpp:13-17, minimiser.hpp:20-23) on many
+ *                          System states at once
  *   mafm_adjoint           no counterpart (gradients of an image loss)
  *   mafm_deltaf_weights /  the frequency-shift post-processing of
  *   mafm_delta_f           simple-vafm-image.py:76-204 (Giessibl weighting of
@@ -369,6 +374,52 @@ mafm_status mafm_scan_flexible(mafm_ctx* ctx, const mafm_flex_desc* sys, const m
 /* Relaxed atom positions of the state column of the last flexible scan,
  * n_atoms x 3 doubles per height for heights [first, first + count). */
 mafm_status mafm_flex_state(mafm_ctx* ctx, int32_t first, int32_t count, double* out);
+
+/* ---- System-level operators --------------------------------------------
+ * The reference's per-System operators, on the device, for n_states
+ * independent copies of one System at once (one workgroup each).  A System
+ * is its interaction list (Simulation::interactions_, in list order, every
+ * term kept -- also those between fixed atoms) and its fixed flags
+ * (System::fixed_, 1 = never moves); the state arrays are n_states x
+ * n_atoms x 3 doubles (positions_, velocities_, forces_) and n_states x
+ * n_atoms (energies_), read and written in place.  Atom 1 is the tip, whose
+ * force and energy the convergence tests read.  Masses are 1, as
+ * System::initialize sets them (system.cpp:18). */
+typedef struct {
+    int32_t n_atoms;
+    int32_t n_terms;
+    const mafm_flex_term* terms;   /* reserved is ignored */
+    const int32_t* fixed;          /* n_atoms */
+} mafm_interaction_list;
+
+typedef struct {                   /* the InputOptions fields minimiser.cpp reads */
+    int32_t minimiser;             /* MAFM_MIN_* */
+    int32_t integrator;            /* MAFM_INT_* (FIRE) */
+    int32_t minterm;               /* MAFM_TERM_* */
+    int32_t maxsteps;
+    double etol, ftol, dt;
+} mafm_minimiser_opts;
+
+/* Interaction::eval of every term in list order (interactions.hpp:68-76,
+ * interactions.cpp:15-362): each term's force and energy are added onto
+ * forces / energies as they are given (zero them first for the plain
+ * values), as System code calling eval over interactions_ does. */
+mafm_status mafm_system_eval(mafm_ctx* ctx, const mafm_interaction_list* list, int64_t n_states,
+                             const double* positions, double* forces, double* energies);
+/* eulerStep / midpointStep / rk4Step (integrators.hpp:13-17,
+ * integrators.cpp:8-111): Euler adds the list's forces and energies onto
+ * forces / energies as given (FIREMinimisation zeroes them first); midpoint
+ * and RK4 overwrite them with the last stage's / the averaged values. */
+mafm_status mafm_system_step(mafm_ctx* ctx, const mafm_interaction_list* list, int32_t integrator, double dt,
+                             int64_t n_states, double* positions, double* velocities, double* forces,
+                             double* energies);
+/* FIREMinimisation / SDMinimisation (minimiser.hpp:20-23,
+ * minimiser.cpp:39-129) from each given state: velocities carry in and out,
+ * energies[1] given is the tip energy the first convergence test compares
+ * with; steps[s] is the function's return value. */
+mafm_status mafm_system_minimise(mafm_ctx* ctx, const mafm_interaction_list* list, const mafm_minimiser_opts* opts,
+                                 int64_t n_states, double* positions, double* velocities, double* forces,
+                                 double* energies, int32_t* steps);
 mafm_status mafm_adjoint(mafm_ctx* ctx, int system, const mafm_adjoint_desc* desc, mafm_adjoint_out* out);
 
 /* ---- frequency shift (simple-vafm-image.py:76-204) ----

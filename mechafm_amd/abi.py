@@ -123,6 +123,17 @@ class FlexDesc(ctypes.Structure):
                 ("tip_dummy_d", ctypes.c_double), ("cell", ctypes.c_double * 9), ("offset", ctypes.c_double * 3)]
 
 
+class InteractionList(ctypes.Structure):
+    _fields_ = [("n_atoms", ctypes.c_int32), ("n_terms", ctypes.c_int32), ("terms", ctypes.c_void_p),
+                ("fixed", ctypes.POINTER(ctypes.c_int32))]
+
+
+class MinimiserOpts(ctypes.Structure):
+    _fields_ = [("minimiser", ctypes.c_int32), ("integrator", ctypes.c_int32), ("minterm", ctypes.c_int32),
+                ("maxsteps", ctypes.c_int32), ("etol", ctypes.c_double), ("ftol", ctypes.c_double),
+                ("dt", ctypes.c_double)]
+
+
 class RunOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("lanes_per_pixel", ctypes.c_int32),
                 ("waves_per_simd", ctypes.c_int32), ("compensation", ctypes.c_int32),
@@ -161,6 +172,14 @@ SIGNATURES = {
     "mafm_records": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _vp]),
     "mafm_margins": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _P(ctypes.c_float)]),
     "mafm_scan_flexible": (ctypes.c_int, [_vp, _P(FlexDesc), _P(ScanDesc), _P(ScanStats)]),
+    "mafm_system_eval": (ctypes.c_int, [_vp, _P(InteractionList), ctypes.c_int64, _P(ctypes.c_double),
+                                        _P(ctypes.c_double), _P(ctypes.c_double)]),
+    "mafm_system_step": (ctypes.c_int, [_vp, _P(InteractionList), ctypes.c_int32, ctypes.c_double, ctypes.c_int64,
+                                        _P(ctypes.c_double), _P(ctypes.c_double), _P(ctypes.c_double),
+                                        _P(ctypes.c_double)]),
+    "mafm_system_minimise": (ctypes.c_int, [_vp, _P(InteractionList), _P(MinimiserOpts), ctypes.c_int64,
+                                            _P(ctypes.c_double), _P(ctypes.c_double), _P(ctypes.c_double),
+                                            _P(ctypes.c_double), _P(ctypes.c_int32)]),
     "mafm_flex_state": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, _P(ctypes.c_double)]),
     "mafm_eval_tip_surface": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_size_t, _P(ctypes.c_double),
                                              _P(ctypes.c_double)]),
@@ -359,6 +378,60 @@ class Context:
         zl = np.zeros(w["n_images"])
         self.check(lib().mafm_delta_f(self.h, system, ctypes.byref(d), _dp(df), _dp(zl)))
         return df, zl
+
+    # ---- System-level operators (mafm_system_eval / _step / _minimise) ----
+    @staticmethod
+    def _system_list(terms: np.ndarray, fixed: np.ndarray):
+        tm = np.ascontiguousarray(terms, dtype=FLEX_TERM_DTYPE)
+        fx = np.ascontiguousarray(fixed, dtype=np.int32)
+        L = InteractionList(len(fx), len(tm), tm.ctypes.data_as(ctypes.c_void_p), fx.ctypes.data_as(_P(ctypes.c_int32)))
+        return L, (tm, fx)
+
+    @staticmethod
+    def _state(a, shape):
+        out = np.ascontiguousarray(a if a is not None else np.zeros(shape), dtype=np.float64)
+        if out.shape != shape:
+            raise ValueError(f"state array of shape {out.shape}, expected {shape}")
+        return out.copy() if out is a else out
+
+    def system_eval(self, terms, fixed, positions, forces=None, energies=None):
+        """Interaction::eval of every term onto forces / energies (zeros when
+        not given) for positions (n_states, n_atoms, 3); returns both."""
+        L, keep = self._system_list(terms, fixed)
+        pos = self._state(positions, np.shape(positions))
+        ns, n = pos.shape[0], pos.shape[1]
+        frc, ene = self._state(forces, (ns, n, 3)), self._state(energies, (ns, n))
+        self.check(lib().mafm_system_eval(self.h, ctypes.byref(L), ns, _dp(pos), _dp(frc), _dp(ene)))
+        return frc, ene
+
+    def system_step(self, terms, fixed, integrator: int, dt: float, positions, velocities, forces=None,
+                    energies=None):
+        """One eulerStep / midpointStep / rk4Step per state; returns new
+        (positions, velocities, forces, energies)."""
+        L, keep = self._system_list(terms, fixed)
+        pos = self._state(positions, np.shape(positions))
+        ns, n = pos.shape[0], pos.shape[1]
+        vel = self._state(velocities, (ns, n, 3))
+        frc, ene = self._state(forces, (ns, n, 3)), self._state(energies, (ns, n))
+        self.check(lib().mafm_system_step(self.h, ctypes.byref(L), integrator, dt, ns, _dp(pos), _dp(vel), _dp(frc),
+                                          _dp(ene)))
+        return pos, vel, frc, ene
+
+    def system_minimise(self, terms, fixed, positions, velocities, energies=None, minimiser: int = 1,
+                        integrator: int = 1, minterm: int = 1, maxsteps: int = 5000, etol: float = 0.01,
+                        ftol: float = 0.01, dt: float = 0.01):
+        """FIREMinimisation / SDMinimisation from each state; returns
+        (positions, velocities, forces, energies, steps)."""
+        L, keep = self._system_list(terms, fixed)
+        pos = self._state(positions, np.shape(positions))
+        ns, n = pos.shape[0], pos.shape[1]
+        vel = self._state(velocities, (ns, n, 3))
+        frc, ene = np.zeros((ns, n, 3)), self._state(energies, (ns, n))
+        steps = np.zeros(ns, dtype=np.int32)
+        o = MinimiserOpts(minimiser, integrator, minterm, maxsteps, etol, ftol, dt)
+        self.check(lib().mafm_system_minimise(self.h, ctypes.byref(L), ctypes.byref(o), ns, _dp(pos), _dp(vel),
+                                              _dp(frc), _dp(ene), steps.ctypes.data_as(_P(ctypes.c_int32))))
+        return pos, vel, frc, ene, steps
 
     def scan_flexible(self, n_atoms: int, positions: np.ndarray, fixed: np.ndarray, terms: np.ndarray,
                       desc: ScanDesc, tip_pbc: bool = False, state_column: int = -1, tip_dummy_d: float = 0.0,

@@ -190,6 +190,7 @@ struct mafm_ctx {
     DevBuf<mafm_flex_term> d_fx_terms;
     DevBuf<int32_t> d_fx_ints;
     DevBuf<double> d_fx_pos, d_fx_state, d_fx_work;
+    DevBuf<double> d_sys;          // System operators: positions, velocities, forces, energies, steps
     bool last_flex = false;        // the records come from mafm_scan_flexible
     int fx_state_atoms = 0, fx_state_heights = 0;   // kept state of the last flexible scan (0: none)
     bool have_margins = false;
@@ -592,6 +593,7 @@ static void destroy_now(mafm_ctx* c) {
     c->d_fx_pos.release();
     c->d_fx_state.release();
     c->d_fx_work.release();
+    c->d_sys.release();
     for (auto& kv : c->twiddles) kv.second.release();
     c->d_flag.release();
     (void)hipEventDestroy(c->ev0);
@@ -1196,6 +1198,146 @@ mafm_status mafm_scan_flexible(mafm_ctx* c, const mafm_flex_desc* s, const mafm_
         stats->block_threads = 64 * waves;
     }
     return MAFM_OK;
+}
+
+// ---------------------------------------------------------------- System operators
+// The tables of one interaction list for k_system_op: every term with its
+// first result slot, each atom's slots in the order Interaction::eval adds
+// to forces_[i], each atom's terms in the order it adds to energies_[i].
+static mafm_status system_op(mafm_ctx* c, const mafm_interaction_list* L, int op, const mafm_minimiser_opts* o,
+                             int integrator, double dt, int64_t n_states, double* pos, double* vel, double* frc,
+                             double* ener, int32_t* steps) {
+    if (!c || !L) return MAFM_ERR_INVALID;
+    if (n_states < 0) return fail(c, MAFM_ERR_INVALID, "n_states is %lld", (long long)n_states);
+    if (n_states == 0) return MAFM_OK;
+    const int n = L->n_atoms;
+    if (n < 2 || n > (1 << 20)) return fail(c, MAFM_ERR_INVALID, "n_atoms is %d (the tip is atom 1)", n);
+    if (!L->fixed || L->n_terms < 0 || (L->n_terms > 0 && !L->terms))
+        return fail(c, MAFM_ERR_INVALID, "interaction list: missing fixed flags or terms");
+    if (!pos || !frc || !ener || (op != SYS_EVAL && !vel) || (op == SYS_MINIMISE && !steps))
+        return fail(c, MAFM_ERR_INVALID, "missing state array");
+    if (op == SYS_STEP && (integrator < 0 || integrator > 2)) return fail(c, MAFM_ERR_INVALID, "integrator");
+    if (op == SYS_MINIMISE) {
+        if (!o) return fail(c, MAFM_ERR_INVALID, "minimiser options are NULL");
+        if (o->minimiser != MAFM_MIN_SD && o->minimiser != MAFM_MIN_FIRE)
+            return fail(c, MAFM_ERR_INVALID, "minimiser");
+        if (o->integrator < 0 || o->integrator > 2) return fail(c, MAFM_ERR_INVALID, "integrator");
+        if (o->minterm < 0 || o->minterm > 2) return fail(c, MAFM_ERR_INVALID, "minterm");
+    }
+    std::vector<mafm_flex_term> terms(L->terms, L->terms + L->n_terms);
+    std::vector<std::vector<int32_t>> fslots(n), eterms(n);
+    int n_slots = 0;
+    for (int k = 0; k < L->n_terms; ++k) {
+        mafm_flex_term& T = terms[k];
+        const int ar = flex_arity(T.kind);
+        if (ar == 0) return fail(c, MAFM_ERR_INVALID, "term %d: unknown kind %d", k, T.kind);
+        for (int r = 0; r < ar; ++r)
+            if (T.atom[r] < 0 || T.atom[r] >= n)
+                return fail(c, MAFM_ERR_INVALID, "term %d: atom %d outside 0..%d", k, T.atom[r], n - 1);
+        T.reserved = n_slots;
+        const int* order = flex_update_order(T.kind);
+        for (int q = 0; q < ar; ++q) {
+            const int r = order[q];
+            fslots[T.atom[r]].push_back(n_slots + r);
+            eterms[T.atom[r]].push_back(k);
+        }
+        n_slots += ar;
+    }
+    std::vector<int32_t> ints;
+    auto csr = [&](const std::vector<std::vector<int32_t>>& lists) {
+        size_t start = ints.size(), total = 0;
+        ints.resize(start + n + 1);
+        for (int i = 0; i < n; ++i) {
+            ints[start + i] = (int32_t)total;
+            total += lists[i].size();
+        }
+        ints[start + n] = (int32_t)total;
+        for (const auto& l : lists) ints.insert(ints.end(), l.begin(), l.end());
+        return (int32_t)total;
+    };
+    SysArgs a;
+    memset(&a, 0, sizeof a);
+    a.n_fcsr = csr(fslots);
+    a.n_ecsr = csr(eterms);
+    ints.insert(ints.end(), L->fixed, L->fixed + n);
+    a.n_atoms = n;
+    a.n_terms = L->n_terms;
+    a.n_slots = n_slots;
+    a.op = op;
+    a.integrator = integrator;
+    a.dt = dt;
+    if (o) {
+        a.minimiser = o->minimiser;
+        a.integrator = op == SYS_MINIMISE ? o->integrator : integrator;
+        a.minterm = o->minterm;
+        a.maxsteps = o->maxsteps;
+        a.etol = o->etol;
+        a.ftol = o->ftol;
+        a.dt = op == SYS_MINIMISE ? o->dt : dt;
+    }
+    a.n_states = n_states;
+    const size_t lds = sys_block_bytes(a) + 64;
+    const bool in_lds = lds <= c->lds_per_block;
+    const int waves = std::max(L->n_terms, n) > 128 ? 4 : (std::max(L->n_terms, n) > 64 ? 2 : 1);
+    const size_t n3 = (size_t)n_states * n * 3, n1 = (size_t)n_states * n;
+    const int64_t blocks = std::min<int64_t>(n_states, 1 << 20);
+    const size_t nd = (sys_block_bytes(a) - ints.size() * sizeof(int32_t)) / sizeof(double);
+
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, c->d_fx_terms.reserve(std::max(L->n_terms, 1)));
+    HIPCHK(c, c->d_fx_ints.reserve(ints.size()));
+    HIPCHK(c, c->d_sys.reserve(3 * n3 + n1 + (n_states + 1) / 2 + 1));
+    if (!in_lds) HIPCHK(c, c->d_fx_work.reserve((size_t)blocks * nd));
+    double* d_pos = c->d_sys.p;
+    double* d_vel = d_pos + n3;
+    double* d_frc = d_vel + n3;
+    double* d_ener = d_frc + n3;
+    int32_t* d_steps = reinterpret_cast<int32_t*>(d_ener + n1);
+    if (L->n_terms > 0)
+        HIPCHK(c, hipMemcpyAsync(c->d_fx_terms.p, terms.data(), terms.size() * sizeof(mafm_flex_term),
+                                 hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_fx_ints.p, ints.data(), ints.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_pos, pos, n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_frc, frc, n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_ener, ener, n1 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (op != SYS_EVAL) HIPCHK(c, hipMemcpyAsync(d_vel, vel, n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    a.terms = c->d_fx_terms.p;
+    a.idx = c->d_fx_ints.p;
+    a.pos = d_pos;
+    a.vel = d_vel;
+    a.frc = d_frc;
+    a.ener = d_ener;
+    a.steps = d_steps;
+    a.work = in_lds ? nullptr : c->d_fx_work.p;
+    HIPCHK(c, launch_system_op(a, waves, in_lds, c->stream));
+    HIPCHK(c, hipMemcpyAsync(pos, d_pos, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(frc, d_frc, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(ener, d_ener, n1 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (op != SYS_EVAL) HIPCHK(c, hipMemcpyAsync(vel, d_vel, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (op == SYS_MINIMISE)
+        HIPCHK(c, hipMemcpyAsync(steps, d_steps, n_states * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MAFM_OK;
+}
+
+mafm_status mafm_system_eval(mafm_ctx* c, const mafm_interaction_list* list, int64_t n_states, const double* positions,
+                             double* forces, double* energies) {
+    return system_op(c, list, SYS_EVAL, nullptr, 0, 0.0, n_states, const_cast<double*>(positions), nullptr, forces,
+                     energies, nullptr);
+}
+
+mafm_status mafm_system_step(mafm_ctx* c, const mafm_interaction_list* list, int32_t integrator, double dt,
+                             int64_t n_states, double* positions, double* velocities, double* forces,
+                             double* energies) {
+    return system_op(c, list, SYS_STEP, nullptr, integrator, dt, n_states, positions, velocities, forces, energies,
+                     nullptr);
+}
+
+mafm_status mafm_system_minimise(mafm_ctx* c, const mafm_interaction_list* list, const mafm_minimiser_opts* opts,
+                                 int64_t n_states, double* positions, double* velocities, double* forces,
+                                 double* energies, int32_t* steps) {
+    return system_op(c, list, SYS_MINIMISE, opts, 0, 0.0, n_states, positions, velocities, forces, energies, steps);
 }
 
 mafm_status mafm_flex_state(mafm_ctx* c, int32_t first, int32_t count, double* out) {

@@ -38,6 +38,33 @@ struct FlexArgs {
     double* work;                     // per-block state when it does not fit LDS
 };
 
+// The reference's System-level operators on n_states independent copies of
+// one system (one workgroup each): the whole interaction list evaluated into
+// every atom's force and energy, one integrator step, or a whole FIRE / SD
+// minimisation from the given state.
+enum { SYS_EVAL = 0, SYS_STEP = 1, SYS_MINIMISE = 2 };
+struct SysArgs {
+    int32_t n_atoms, n_terms, n_slots, op;
+    int32_t minimiser, integrator, minterm, maxsteps;
+    double etol, ftol, dt;
+    int64_t n_states;
+    // every term, in list order, its first result slot in `reserved`
+    const mafm_flex_term* terms;
+    // back to back: f_off (n_atoms + 1), f_csr (n_fcsr: each atom's result
+    // slots in list order), e_off (n_atoms + 1), e_csr (n_ecsr: each atom's
+    // terms in list order, for its energy), fixed (n_atoms)
+    const int32_t* idx;
+    int32_t n_fcsr, n_ecsr;
+    double* pos;       // n_states x n_atoms x 3, in and out
+    double* vel;       // n_states x n_atoms x 3, in and out (step, minimise)
+    double* frc;       // n_states x n_atoms x 3: in (eval and the Euler step add to it), out
+    double* ener;      // n_states x n_atoms: as frc; minimise reads energies[1] as the last tip energy
+    int32_t* steps;    // n_states (minimise)
+    double* work;      // per-block state when it does not fit LDS
+};
+size_t sys_block_bytes(const SysArgs& a);
+hipError_t launch_system_op(const SysArgs& a, int waves, bool in_lds, hipStream_t st);
+
 // Doubles of per-column state one block keeps (positions, integrator
 // stages, velocities, forces, term results).
 size_t flex_block_doubles(int n_atoms, int n_slots, int n_eval, bool rk4);

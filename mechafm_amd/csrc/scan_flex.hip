@@ -27,6 +27,8 @@
 // IEEE divisions and square roots.  The angle, dihedral and Morse terms call
 // the device acos / sin / atan / exp, which may differ from glibc's in the
 // last place.
+#include <algorithm>
+
 #include "flex.hpp"
 #include "ref_terms.hpp"
 
@@ -175,6 +177,16 @@ __device__ void eval_term(const mafm_flex_term& t, const double* X, double* R, d
 
 // checkConvergence (minimiser.cpp:15-37)
 __device__ inline bool converged(const FlexArgs& a, V3 f, double ediff) {
+    switch (a.minterm) {
+        case MAFM_TERM_E: return fabs(ediff) < a.etol;
+        case MAFM_TERM_F: return len3(f) < a.ftol;
+        default: return fabs(ediff) < a.etol && len3(f) < a.ftol;
+    }
+}
+
+// checkConvergence (minimiser.cpp:15-37) with a System operator's options
+template <typename A>
+__device__ inline bool converged_sys(const A& a, V3 f, double ediff) {
     switch (a.minterm) {
         case MAFM_TERM_E: return fabs(ediff) < a.etol;
         case MAFM_TERM_F: return len3(f) < a.ftol;
@@ -471,6 +483,243 @@ __global__ void __launch_bounds__(256) k_scan_flex(FlexArgs a) {
     }
 }
 
+// ---------------------------------------------------------------- System operators
+// State of one system copy: positions, velocities, forces and energies of
+// every atom, the integrator's evaluation point, stage velocities and
+// accumulators, FIRE's per-atom powers, and the term results.
+__host__ __device__ inline size_t sys_doubles(int n, int n_slots, int n_terms) {
+    return (size_t)24 * n + 3 * (size_t)n_slots + n_terms + 8;
+}
+
+struct SysState {
+    double *pos, *vel, *frc, *ener, *pev, *vst, *facc, *vacc, *eacc, *pdot, *res, *tener, *bc;
+    const int32_t *f_off, *f_csr, *e_off, *e_csr, *fixed;
+};
+
+__device__ inline SysState sys_layout(double* p, int n, int n_slots, int n_terms) {
+    SysState S;
+    S.pos = p; p += 3 * n;
+    S.vel = p; p += 3 * n;
+    S.frc = p; p += 3 * n;
+    S.ener = p; p += n;
+    S.pev = p; p += 3 * n;
+    S.vst = p; p += 3 * n;
+    S.facc = p; p += 3 * n;
+    S.vacc = p; p += 3 * n;
+    S.eacc = p; p += n;
+    S.pdot = p; p += n;
+    S.res = p; p += 3 * n_slots;
+    S.tener = p; p += n_terms;
+    S.bc = p;
+    return S;
+}
+
+// Every term's Interaction::eval at positions X, added in list order onto
+// F / E (interactions.cpp:15-51, 252-362): F[i] += each result slot of atom
+// i, E[i] += the energy of each term acting on i.
+__device__ void eval_list(const SysArgs& a, const SysState& S, const double* X, double* F, double* E, int t,
+                          int nthr) {
+    for (int k = t; k < a.n_terms; k += nthr) {
+        const mafm_flex_term& term = a.terms[k];
+        eval_term(term, X, S.res + 3 * term.reserved, S.tener[k]);
+    }
+    block_sync();
+    for (int i = t; i < a.n_atoms; i += nthr) {
+        V3 f = ld3(F, i);
+        for (int q = S.f_off[i]; q < S.f_off[i + 1]; ++q) f = add(f, ld3(S.res, S.f_csr[q]));
+        st3(F, i, f);
+        double e = E[i];
+        for (int q = S.e_off[i]; q < S.e_off[i + 1]; ++q) e += S.tener[S.e_csr[q]];
+        E[i] = e;
+    }
+    block_sync();
+}
+
+__device__ inline void zero_fe(const SysArgs& a, double* F, double* E, int t, int nthr) {
+    for (int i = t; i < a.n_atoms; i += nthr) {
+        st3(F, i, {0, 0, 0});
+        E[i] = 0;
+    }
+}
+
+// eulerStep / midpointStep / rk4Step (integrators.cpp:8-111), masses 1.
+// Euler adds the list onto S.frc / S.ener as they are; the other two leave
+// their final-stage (midpoint) or averaged (RK4) forces and energies there.
+__device__ void sys_step(const SysArgs& a, const SysState& S, int integrator, double dt, int t, int nthr) {
+    const int n = a.n_atoms;
+    if (integrator == MAFM_INT_EULER) {
+        eval_list(a, S, S.pos, S.frc, S.ener, t, nthr);
+        for (int i = t; i < n; i += nthr)
+            if (S.fixed[i] != 1) {
+                st3(S.pos, i, add(ld3(S.pos, i), scl(dt, ld3(S.vel, i))));
+                st3(S.vel, i, add(ld3(S.vel, i), scl(dt, ld3(S.frc, i))));
+            }
+        block_sync();
+        return;
+    }
+    if (integrator == MAFM_INT_MIDPOINT) {
+        zero_fe(a, S.facc, S.eacc, t, nthr);   // f1, e1
+        block_sync();
+        eval_list(a, S, S.pos, S.facc, S.eacc, t, nthr);
+        for (int i = t; i < n; i += nthr) {
+            const bool free_ = S.fixed[i] != 1;
+            st3(S.pev, i, free_ ? add(ld3(S.pos, i), scl(dt / 2, ld3(S.vel, i))) : ld3(S.pos, i));
+            st3(S.vst, i, free_ ? add(ld3(S.vel, i), scl(dt / 2, ld3(S.facc, i))) : ld3(S.vel, i));
+        }
+        zero_fe(a, S.frc, S.ener, t, nthr);    // f2, e2 become the system's
+        block_sync();
+        eval_list(a, S, S.pev, S.frc, S.ener, t, nthr);
+        for (int i = t; i < n; i += nthr)
+            if (S.fixed[i] != 1) {
+                st3(S.pos, i, add(ld3(S.pos, i), scl(dt, ld3(S.vst, i))));
+                st3(S.vel, i, add(ld3(S.vel, i), scl(dt, ld3(S.frc, i))));
+            }
+        block_sync();
+        return;
+    }
+    // RK4: stages at pos, pos + dt/2 v1, pos + dt/2 v2, pos + dt v3; the sums
+    // f1 + 2 f2 + 2 f3 + f4 (and v, e alike) accumulated left to right
+    for (int s = 0; s < 4; ++s) {
+        zero_fe(a, S.frc, S.ener, t, nthr);
+        block_sync();
+        eval_list(a, S, s == 0 ? S.pos : S.pev, S.frc, S.ener, t, nthr);
+        for (int i = t; i < n; i += nthr) {
+            const V3 fs = ld3(S.frc, i), vs = s == 0 ? ld3(S.vel, i) : ld3(S.vst, i);
+            const double es = S.ener[i];
+            if (s == 0) {
+                st3(S.facc, i, fs);
+                st3(S.vacc, i, vs);
+                S.eacc[i] = es;
+            } else if (s < 3) {
+                st3(S.facc, i, add(ld3(S.facc, i), scl(2, fs)));
+                st3(S.vacc, i, add(ld3(S.vacc, i), scl(2, vs)));
+                S.eacc[i] = S.eacc[i] + 2 * es;
+            } else {
+                st3(S.facc, i, add(ld3(S.facc, i), fs));
+                st3(S.vacc, i, add(ld3(S.vacc, i), vs));
+                S.eacc[i] = S.eacc[i] + es;
+            }
+            if (s < 3) {
+                const double h = s < 2 ? dt / 2 : dt;
+                if (S.fixed[i] != 1) {
+                    st3(S.pev, i, add(ld3(S.pos, i), scl(h, vs)));
+                    st3(S.vst, i, add(ld3(S.vel, i), scl(h, fs)));
+                } else {
+                    st3(S.pev, i, ld3(S.pos, i));
+                    st3(S.vst, i, ld3(S.vel, i));
+                }
+            }
+        }
+        block_sync();
+    }
+    for (int i = t; i < n; i += nthr) {
+        if (S.fixed[i] != 1) {
+            st3(S.pos, i, add(ld3(S.pos, i), scl(dt / 6, ld3(S.vacc, i))));
+            st3(S.vel, i, add(ld3(S.vel, i), scl(dt / 6, ld3(S.facc, i))));
+        }
+        st3(S.frc, i, dvs(ld3(S.facc, i), 6));
+        S.ener[i] = S.eacc[i] / 6;
+    }
+    block_sync();
+}
+
+template <bool IN_LDS>
+__global__ void __launch_bounds__(256) k_system_op(SysArgs a) {
+    extern __shared__ double lds_sys[];
+    const int t = threadIdx.x, nthr = blockDim.x;
+    const int n = a.n_atoms;
+    const size_t nd = sys_doubles(n, a.n_slots, a.n_terms);
+    double* base = IN_LDS ? lds_sys : a.work + (size_t)blockIdx.x * nd;
+    SysState S = sys_layout(base, n, a.n_slots, a.n_terms);
+    const int n_idx = 2 * (n + 1) + a.n_fcsr + a.n_ecsr + n;
+    const int32_t* idx = a.idx;
+    if (IN_LDS) {
+        int32_t* l = reinterpret_cast<int32_t*>(lds_sys + nd);
+        for (int q = t; q < n_idx; q += nthr) l[q] = a.idx[q];
+        idx = l;
+    }
+    S.f_off = idx;
+    S.f_csr = S.f_off + (n + 1);
+    S.e_off = S.f_csr + a.n_fcsr;
+    S.e_csr = S.e_off + (n + 1);
+    S.fixed = S.e_csr + a.n_ecsr;
+    for (int64_t st = blockIdx.x; st < a.n_states; st += gridDim.x) {
+        const size_t o3 = (size_t)st * 3 * n, o1 = (size_t)st * n;
+        for (int i = t; i < n; i += nthr) {
+            st3(S.pos, i, ld3(a.pos + o3, i));
+            st3(S.frc, i, ld3(a.frc + o3, i));
+            S.ener[i] = a.ener[o1 + i];
+            st3(S.vel, i, a.op == SYS_EVAL ? V3{0, 0, 0} : ld3(a.vel + o3, i));
+        }
+        block_sync();
+        int n_tot = 0;
+        if (a.op == SYS_EVAL) {
+            eval_list(a, S, S.pos, S.frc, S.ener, t, nthr);
+        } else if (a.op == SYS_STEP) {
+            sys_step(a, S, a.integrator, a.dt, t, nthr);
+        } else if (a.minimiser == MAFM_MIN_SD) {   // SDMinimisation (minimiser.cpp:39-65)
+            for (n_tot = 1; n_tot < a.maxsteps; ++n_tot) {
+                const double prev = S.ener[1];
+                block_sync();
+                zero_fe(a, S.frc, S.ener, t, nthr);
+                block_sync();
+                eval_list(a, S, S.pos, S.frc, S.ener, t, nthr);
+                if (converged_sys(a, ld3(S.frc, 1), S.ener[1] - prev)) break;
+                for (int i = t; i < n; i += nthr)
+                    if (S.fixed[i] != 1) st3(S.pos, i, add(ld3(S.pos, i), scl(a.dt, ld3(S.frc, i))));
+                block_sync();
+            }
+        } else {                                   // FIREMinimisation (minimiser.cpp:67-129)
+            double alpha = 0.1, dt = a.dt;
+            int nnn = 0;
+            for (n_tot = 1; n_tot < a.maxsteps; ++n_tot) {
+                const double prev = S.ener[1];
+                block_sync();
+                zero_fe(a, S.frc, S.ener, t, nthr);
+                block_sync();
+                sys_step(a, S, a.integrator, dt, t, nthr);
+                if (converged_sys(a, ld3(S.frc, 1), S.ener[1] - prev)) break;
+                // P over every atom in index order, then the velocity mix of the free ones
+                for (int i = t; i < n; i += nthr) S.pdot[i] = dot(ld3(S.vel, i), ld3(S.frc, i));
+                block_sync();
+                if (t == nthr - 1) S.bc[1] = seq_sum(S.pdot, nullptr, n);
+                block_sync();
+                const double p = S.bc[1];
+                for (int i = t; i < n; i += nthr) {
+                    if (S.fixed[i] == 1) continue;
+                    const V3 v = ld3(S.vel, i), f = ld3(S.frc, i);
+                    const double fl = len3(f);
+                    const V3 fh = fl > 1e-10 ? V3{f.x / fl, f.y / fl, f.z / fl} : V3{0, 0, 0};   // normalized()
+                    st3(S.vel, i, add(scl(1 - alpha, v), scr(scl(alpha, fh), len3(v))));
+                }
+                block_sync();
+                if (p < 0) {
+                    for (int i = t; i < n; i += nthr) st3(S.vel, i, {0, 0, 0});
+                    nnn = 0;
+                    dt *= 0.5;
+                    alpha = 0.1;
+                } else {
+                    if (nnn > 5) {
+                        dt = fmin(dt * 1.1, 10 * a.dt);
+                        alpha *= 0.99;
+                    }
+                    ++nnn;
+                }
+                block_sync();
+            }
+        }
+        block_sync();
+        for (int i = t; i < n; i += nthr) {
+            st3(a.pos + o3, i, ld3(S.pos, i));
+            st3(a.frc + o3, i, ld3(S.frc, i));
+            a.ener[o1 + i] = S.ener[i];
+            if (a.op != SYS_EVAL) st3(a.vel + o3, i, ld3(S.vel, i));
+        }
+        if (t == 0 && a.op == SYS_MINIMISE) a.steps[st] = n_tot;
+        block_sync();
+    }
+}
+
 }  // namespace
 
 size_t flex_block_doubles(int n_atoms, int n_slots, int n_eval, bool rk4) {
@@ -481,6 +730,25 @@ size_t flex_block_lds(const FlexArgs& a) {
     const bool rk4 = a.minimiser == MAFM_MIN_FIRE && a.integrator == MAFM_INT_RK4;
     const size_t n_idx = (size_t)(a.n_atoms + 1) + a.n_csr + a.n_tip + a.n_atoms;
     return block_doubles(a.n_atoms, a.n_slots, a.n_eval, rk4) * sizeof(double) + n_idx * sizeof(int32_t);
+}
+
+size_t sys_block_bytes(const SysArgs& a) {
+    const size_t n_idx = 2 * (size_t)(a.n_atoms + 1) + a.n_fcsr + a.n_ecsr + a.n_atoms;
+    return sys_doubles(a.n_atoms, a.n_slots, a.n_terms) * sizeof(double) + n_idx * sizeof(int32_t);
+}
+
+hipError_t launch_system_op(const SysArgs& a, int waves, bool in_lds, hipStream_t st) {
+    const unsigned blocks = (unsigned)std::min<int64_t>(a.n_states, 1 << 20);
+    if (in_lds) {
+        const size_t lds = sys_block_bytes(a);
+        hipError_t e = hipFuncSetAttribute((const void*)k_system_op<true>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_system_op<true>, dim3(blocks), dim3(64 * waves), lds, st, a);
+    } else {
+        hipLaunchKernelGGL(k_system_op<false>, dim3(blocks), dim3(64 * waves), 0, st, a);
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_scan_flex(const FlexArgs& a, int n_blocks, int waves, bool in_lds, hipStream_t st) {

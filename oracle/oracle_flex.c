@@ -402,6 +402,54 @@ static void inv3(const double* m, double* o) {     /* Matrix3::inverse (matrices
  * per position at ((col - col_first) * nz + k) in ref_harness's layout
  * (i j k n x y z fx fy fz rx ry rz r angle e); state (optional): the atoms
  * of column state_col after each height's minimisation, nz x n x 3. */
+/* The System-level operators on one state (interactions.cpp:15-362,
+ * integrators.cpp:8-111, minimiser.cpp:39-129 as the functions above restate
+ * them): op 0 adds every term of the list onto forces / energies as given;
+ * op 1 is one eulerStep / midpointStep / rk4Step (`integrator`, `dt`); op 2
+ * is SDMinimisation / FIREMinimisation with o's minimiser, integrator,
+ * minterm, maxsteps, etol, ftol and dt from the state, energies[1] being
+ * the tip energy the first test compares with.  pos / vel / forces (n x 3)
+ * and energies (n) are read and written; returns the minimiser's step
+ * count for op 2, else 0. */
+int oracle_flex_system(int n, const fx_term* T, int nt, const int* fixed, int op, const fx_opts* o, int integrator,
+                       double dt, double* pos, double* vel, double* forces, double* energies) {
+    fx_sys s;
+    memset(&s, 0, sizeof s);
+    s.n = n; s.T = T; s.nt = nt; s.fixed = fixed;
+    v3* buf = (v3*)calloc((size_t)13 * n, sizeof(v3));
+    double* ebuf = (double*)calloc((size_t)5 * n, sizeof(double));
+    s.pos = buf; s.vel = buf + n; s.frc = buf + 2 * n;
+    s.p2 = buf + 3 * n; s.p3 = buf + 4 * n; s.p4 = buf + 5 * n;
+    s.v2 = buf + 6 * n; s.v3_ = buf + 7 * n; s.v4 = buf + 8 * n;
+    s.f1 = buf + 9 * n; s.f2 = buf + 10 * n; s.f3 = buf + 11 * n; s.f4 = buf + 12 * n;
+    s.en = ebuf; s.e1 = ebuf + n; s.e2 = ebuf + 2 * n; s.e3 = ebuf + 3 * n; s.e4 = ebuf + 4 * n;
+    for (int a = 0; a < n; ++a) {
+        s.pos[a] = V(pos[3 * a], pos[3 * a + 1], pos[3 * a + 2]);
+        s.vel[a] = vel ? V(vel[3 * a], vel[3 * a + 1], vel[3 * a + 2]) : V(0, 0, 0);
+        s.frc[a] = V(forces[3 * a], forces[3 * a + 1], forces[3 * a + 2]);
+        s.en[a] = energies[a];
+    }
+    int ret = 0;
+    if (op == 0) {
+        for (int k = 0; k < nt; ++k) eval_term(&T[k], s.pos, s.frc, s.en);
+    } else if (op == 1) {
+        if (integrator == 0) euler_step(&s, dt);
+        else if (integrator == 1) midpoint_step(&s, dt);
+        else rk4_step(&s, dt);
+    } else {
+        ret = o->minimiser == 0 ? sd_min(&s, o) : fire_min(&s, o);
+    }
+    for (int a = 0; a < n; ++a) {
+        pos[3 * a] = s.pos[a].x; pos[3 * a + 1] = s.pos[a].y; pos[3 * a + 2] = s.pos[a].z;
+        if (vel) { vel[3 * a] = s.vel[a].x; vel[3 * a + 1] = s.vel[a].y; vel[3 * a + 2] = s.vel[a].z; }
+        forces[3 * a] = s.frc[a].x; forces[3 * a + 1] = s.frc[a].y; forces[3 * a + 2] = s.frc[a].z;
+        energies[a] = s.en[a];
+    }
+    free(buf);
+    free(ebuf);
+    return ret;
+}
+
 int oracle_flex_scan(int n, const fx_term* T, int nt, const double* pos0, const int* fixed, const fx_opts* o,
                      int col_first, int col_count, double* rec, int state_col, double* state, int64_t* steps) {
     int64_t total = 0;

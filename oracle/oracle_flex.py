@@ -273,6 +273,28 @@ def evaluate(terms: np.ndarray, pos: np.ndarray, only_tip_surface: bool = False)
     return F, E
 
 
+def system_op(terms: np.ndarray, fixed: np.ndarray, op: str, pos: np.ndarray, vel: Optional[np.ndarray],
+              forces: np.ndarray, energies: np.ndarray, integrator: int = 1, dt: float = 0.0,
+              minimiser: int = 1, minterm: int = 1, maxsteps: int = 5000, etol: float = 0.01,
+              ftol: float = 0.01) -> int:
+    """One System-level operator on one state, in place (oracle_flex.c:
+    oracle_flex_system): "eval" adds the list onto forces / energies, "step"
+    is one eulerStep / midpointStep / rk4Step (integrator 0 / 1 / 2),
+    "minimise" SDMinimisation (minimiser 0) or FIREMinimisation (1) with the
+    given options.  Returns the minimiser's step count ("minimise")."""
+    terms, tp = _terms_ptr(terms)
+    fixed = np.ascontiguousarray(fixed, dtype=np.int32)
+    for a in (pos, forces, energies) + ((vel,) if vel is not None else ()):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    so = FxOpts()
+    so.minimiser, so.integrator, so.minterm, so.maxsteps = minimiser, integrator, minterm, maxsteps
+    so.etol, so.ftol, so.dt = etol, ftol, dt
+    code = {"eval": 0, "step": 1, "minimise": 2}[op]
+    return O.lib().oracle_flex_system(len(pos), tp, len(terms), fixed.ctypes.data_as(ctypes.c_void_p), code,
+                                      ctypes.byref(so), integrator, ctypes.c_double(dt), O._dp(pos),
+                                      O._dp(vel) if vel is not None else None, O._dp(forces), O._dp(energies))
+
+
 def _fx_opts(fs: FlexSetup, shape) -> FxOpts:
     o, s = fs.base.opts, fs.base.system
     so = FxOpts()
