@@ -30,6 +30,7 @@
  *                          (simulation.cpp:420-479)
  *   mafm_grid_upload /     ForceGrid set-up and swapForceValues /
  *   mafm_grid_download     swapEnergyValues (force_grid.hpp:20-30)
+ *   mafm_grid_interpolate  ForceGrid::interpolate (force_grid.hpp:33)
  *   mafm_potential_grid    ElectrostaticPotentialInteraction's constructor
  *                          (interactions.cpp:53-234)
  *   mafm_fft3d             fft_data_grid / ffti_data_grid (fft.hpp:25, 34)
@@ -307,6 +308,12 @@ mafm_status mafm_grid_build(mafm_ctx* ctx, const mafm_system_desc* sys, const ma
                             int* handle);
 mafm_status mafm_grid_download(mafm_ctx* ctx, int handle, double* fe);
 mafm_status mafm_grid_free(mafm_ctx* ctx, int handle);
+/* ForceGrid::interpolate (force_grid.hpp:33, force_grid.cpp:88-155) of grid
+ * `handle` at n positions (n x 3): fe = n x 4 (fx fy fz e), the reference's
+ * trilinear weights and operation order.  MAFM_ERR_OUT_OF_GRID (values
+ * still written, clamped) when a position needs a point past a non-periodic
+ * grid, where the reference stops (force_grid.cpp:236-240). */
+mafm_status mafm_grid_interpolate(mafm_ctx* ctx, int handle, size_t n, const double* positions, double* fe);
 /* Energy and force of the tip's Gaussian charge in the potential, by FFT
  * convolution and spectral gradient, stored as a periodic grid. */
 mafm_status mafm_potential_grid(mafm_ctx* ctx, const mafm_potential_desc* pot, int* handle);

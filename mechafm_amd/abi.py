@@ -170,6 +170,8 @@ SIGNATURES = {
     "mafm_scan_host_query": (ctypes.c_int, [_vp, _P(ctypes.c_int)]),
     "mafm_scan_host_wait": (ctypes.c_int, [_vp, _P(ScanStats)]),
     "mafm_records": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _vp]),
+    "mafm_grid_interpolate": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_size_t, _P(ctypes.c_double),
+                                             _P(ctypes.c_double)]),
     "mafm_margins": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _P(ctypes.c_float)]),
     "mafm_scan_flexible": (ctypes.c_int, [_vp, _P(FlexDesc), _P(ScanDesc), _P(ScanStats)]),
     "mafm_system_eval": (ctypes.c_int, [_vp, _P(InteractionList), ctypes.c_int64, _P(ctypes.c_double),
@@ -278,6 +280,16 @@ class Context:
         h = ctypes.c_int(-1)
         self.check(lib().mafm_grid_upload(self.h, ctypes.byref(g), ctypes.byref(h)))
         return h.value
+
+    def grid_free(self, handle: int) -> None:
+        self.check(lib().mafm_grid_free(self.h, handle))
+
+    def grid_interpolate(self, handle: int, pos: np.ndarray) -> np.ndarray:
+        """ForceGrid::interpolate at pos (n, 3): (n, 4) fx fy fz e."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        out = np.zeros((len(pos), 4))
+        self.check(lib().mafm_grid_interpolate(self.h, handle, len(pos), _dp(pos), _dp(out)))
+        return out
 
     def grid_download(self, handle: int, n) -> np.ndarray:
         out = np.zeros((int(n[0]) * int(n[1]) * int(n[2]), 4))

@@ -23,6 +23,7 @@
 namespace mafm {
 hipError_t launch_scan_plain(const ScanArgs& a, hipStream_t st);
 hipError_t launch_eval_plain(const ScanArgs& a, int mol, long n, const double* pos, double* out, hipStream_t st);
+hipError_t launch_grid_interp(const ScanArgs& a, int grid, long n, const double* pos, double* out, hipStream_t st);
 hipError_t launch_grid_build(const ScanArgs& a, int mol, const int* n, const double* s, const double* o,
                              double4* out, hipStream_t st);
 hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_blocks, int W, int lds_sites,
@@ -652,6 +653,29 @@ mafm_status mafm_grid_download(mafm_ctx* c, int handle, double* fe) {
     size_t np = (size_t)h.n[0] * h.n[1] * h.n[2];
     HIPCHK(c, hipMemcpyAsync(fe, h.fe64, np * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MAFM_OK;
+}
+
+mafm_status mafm_grid_interpolate(mafm_ctx* c, int handle, size_t n, const double* pos, double* fe) {
+    if (!c || (n && (!pos || !fe))) return MAFM_ERR_INVALID;
+    if (handle < 0 || handle >= (int)c->grids.size() || !c->grids[handle].live)
+        return fail(c, MAFM_ERR_INVALID, "grid handle %d is not live", handle);
+    if (n == 0) return MAFM_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, c->d_tmp.reserve(7 * n));
+    double* dpos = c->d_tmp.p;
+    double* dout = c->d_tmp.p + 3 * n;
+    HIPCHK(c, hipMemcpyAsync(dpos, pos, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_counters.p, 0, 8 * sizeof(int64_t), c->stream));
+    ScanArgs a = base_args(c);
+    HIPCHK(c, launch_grid_interp(a, handle, (long)n, dpos, dout, c->stream));
+    int64_t counters[8];
+    HIPCHK(c, hipMemcpyAsync(fe, dout, 4 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(counters, c->d_counters.p, sizeof counters, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (counters[3] != 0)
+        return fail(c, MAFM_ERR_OUT_OF_GRID, "%lld positions needed a grid point past the grid",
+                    (long long)counters[3]);
     return MAFM_OK;
 }
 

@@ -332,6 +332,20 @@ __global__ void __launch_bounds__(256) k_eval_plain(ScanArgs a, int mol, long n,
     if (c.flags & REC_OUT_OF_GRID) atomicAdd((unsigned long long*)&a.counters[3], 1ull);
 }
 
+// ForceGrid::interpolate (force_grid.cpp:88-155) of grid `grid` at
+// arbitrary positions; out-of-grid lookups counted in counters[3].
+__global__ void __launch_bounds__(256) k_grid_interp(ScanArgs a, int grid, long n, const double* pos, double* out) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    Ctx c = {&a, nullptr, 0};
+    const V3 t = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+    V3 f;
+    double e;
+    grid_interp(c, a.grids[grid], t, f, e);
+    out[4 * p] = f.x; out[4 * p + 1] = f.y; out[4 * p + 2] = f.z; out[4 * p + 3] = e;
+    if (c.flags & REC_OUT_OF_GRID) atomicAdd((unsigned long long*)&a.counters[3], 1ull);
+}
+
 // Grid sampling of buildTipGridInteractions (simulation.cpp:450-479): the tip
 // sits on the dummy, every tip-surface term of molecule `mol` is evaluated at
 // (i*s + o) (xy wrapped by setDummyXY when the system is periodic).  One lane
@@ -372,6 +386,13 @@ hipError_t launch_eval_plain(const ScanArgs& a, int mol, long n, const double* p
     int blocks = (int)((n + 255) / 256);
     if (blocks == 0) return hipSuccess;
     hipLaunchKernelGGL(k_eval_plain, dim3(blocks), dim3(256), 0, st, a, mol, n, pos, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_interp(const ScanArgs& a, int grid, long n, const double* pos, double* out, hipStream_t st) {
+    const int blocks = (int)((n + 255) / 256);
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_grid_interp, dim3(blocks), dim3(256), 0, st, a, grid, n, pos, out);
     return hipGetLastError();
 }
 
