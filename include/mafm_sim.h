@@ -18,6 +18,8 @@
  *                          the reference runs one molecule per process)
  *   mafm_batch_run_output  the same, writing the files during the scan
  *                          (simulation.cpp:123-138 flushes during the run)
+ *   mafm_sim_run_output    Simulation::run with the files written and the
+ *                          progress reported during the scan (simulation.cpp:123-138)
  *   mafm_batch_write_output  writeOutput for many jobs on one thread pool
  *                          (simulation.cpp:145-199)
  *   mafm_sim_flex_system   the flexible interaction list of buildInteractions
@@ -88,6 +90,16 @@ mafm_status mafm_batch_run(int n, mafm_sim* const* sims, const mafm_run_opts* op
  * which reads the device records, needs another run). */
 mafm_status mafm_batch_run_output(int n, mafm_sim* const* sims, const mafm_run_opts* opts, int threads,
                                   mafm_scan_stats* stats);
+/* progress of a streamed run: the fraction (0..1) of the positions whose
+ * records are in, called from the calling thread */
+typedef void (*mafm_progress_fn)(double fraction, void* user);
+/* One rigid job scanned with its files written as the records land (the
+ * reference's Simulation::run, writing its buffer and reporting progress
+ * from inside the scan, simulation.cpp:123-138): mafm_batch_run_output for
+ * one job, with progress (may be NULL) called as heights of columns come
+ * in and once with 1.0 at the end. */
+mafm_status mafm_sim_run_output(mafm_sim* sim, const mafm_run_opts* opts, int threads, mafm_progress_fn progress,
+                                void* user, mafm_scan_stats* stats);
 /* the per-height files (and statistics.txt) of many rigid jobs that have
  * run, written on one pool of `threads` threads (<= 0: every hardware
  * thread), chunk by chunk of pixel columns, gzip deflated in parallel within

@@ -200,6 +200,7 @@ SIGNATURES = {
     "mafm_batch_run": (ctypes.c_int, [ctypes.c_int, _P(_vp), _P(RunOpts), _P(ScanStats)]),
     "mafm_batch_run_output": (ctypes.c_int, [ctypes.c_int, _P(_vp), _P(RunOpts), ctypes.c_int, _P(ScanStats)]),
     "mafm_batch_write_output": (ctypes.c_int, [ctypes.c_int, _P(_vp), ctypes.c_int]),
+    "mafm_sim_run_output": (ctypes.c_int, [_vp, _P(RunOpts), ctypes.c_int, _vp, _vp, _P(ScanStats)]),
     "mafm_sim_settings": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t, _P(ctypes.c_size_t)]),
     "mafm_sim_records": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _vp]),
     "mafm_sim_write_output": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -575,6 +576,18 @@ class Simulation:
         out = np.zeros(n, dtype=RECORD_DTYPE)
         self.check(lib().mafm_sim_records(self.h, 0, n, out.ctypes.data_as(ctypes.c_void_p)))
         return out
+
+    def run_output(self, precision: str = "fast64", threads: int = 0, progress=None, lanes_per_pixel: int = 0,
+                   waves_per_simd: int = 0, options: int = 0) -> dict:
+        """Scan with the files written as the records land
+        (mafm_sim_run_output); progress(fraction) is called as they come."""
+        o = RunOpts(PREC[precision], lanes_per_pixel, waves_per_simd, 0, 0, options)
+        st = ScanStats()
+        cb = ctypes.CFUNCTYPE(None, ctypes.c_double, ctypes.c_void_p)(
+            (lambda f, _u: progress(f)) if progress else (lambda f, _u: None))
+        self.check(lib().mafm_sim_run_output(self.h, ctypes.byref(o), threads, ctypes.cast(cb, ctypes.c_void_p),
+                                             None, ctypes.byref(st)))
+        return st.as_dict()
 
     def write_output(self, threads: int = 8) -> None:
         self.check(lib().mafm_sim_write_output(self.h, threads))

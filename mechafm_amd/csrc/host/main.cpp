@@ -75,9 +75,26 @@ int main(int argc, char** argv) {
     printf("+- 3D data grid is: %d x %d x %d (%lld in total)\n", info.nx, info.ny, info.nz,
            (long long)info.nx * info.ny * info.nz);
     printf("+- Starting simulation\n");
+    fflush(stdout);
     mafm_scan_stats ss{};
-    st = mafm_sim_run(sim, &ro, &ss);
-    if (st == MAFM_OK) st = mafm_sim_write_output(sim, 0);
+    if (!info.flexible && !deltaf) {
+        // files written while the scan runs, progress as the reference reports
+        // it: "Finished x %" at every tenth (simulation.cpp:131-137)
+        double next_report = 0.1;
+        auto report = [](double fraction, void* user) {
+            double& next = *static_cast<double*>(user);
+            while (fraction >= next - 1e-12 && next <= 1.0 + 1e-12) {
+                printf("+- Finished %4.1f %% of the simulation\n", 100 * next);
+                next += 0.1;
+            }
+            fflush(stdout);
+        };
+        st = mafm_sim_run_output(sim, &ro, 0, report, &next_report, &ss);
+    } else {
+        // flexible jobs and delta f images keep the device records: scan, then write
+        st = mafm_sim_run(sim, &ro, &ss);
+        if (st == MAFM_OK) st = mafm_sim_write_output(sim, 0);
+    }
     if (st == MAFM_OK) {
         // flexible on: the "+- Wrote state_..." lines of makeXYZFile (system.cpp:63)
         size_t need = 0;

@@ -1793,8 +1793,11 @@ mafm_status mafm_batch_run(int n, mafm_sim* const* sims, const mafm_run_opts* o,
     });
 }
 
-mafm_status mafm_batch_run_output(int n, mafm_sim* const* sims, const mafm_run_opts* o, int threads,
-                                  mafm_scan_stats* stats) {
+// One streamed launch of n jobs with their files written as the records
+// land; progress (if any) gets the fraction of positions handed to the
+// writers, from the polling thread, then 1 at the end.
+static mafm_status run_output(int n, mafm_sim* const* sims, const mafm_run_opts* o, int threads,
+                              mafm_progress_fn progress, void* user, mafm_scan_stats* stats) {
     if (n <= 0 || !sims || !sims[0]) return MAFM_ERR_INVALID;
     mafm_run_opts def{};
     def.precision = MAFM_PREC_FAST64;
@@ -1838,13 +1841,15 @@ mafm_status mafm_batch_run_output(int n, mafm_sim* const* sims, const mafm_run_o
         // [0, count) are visible once the count is); each chunk of columns
         // goes to the writers height by height, as its slowest column passes
         const int nz = d0.nz;
+        const double total_cells = (double)n * cols * nz;
+        double handed = 0;
         std::vector<char> job_done(n, 0);
         int complete = 0;
         bool finished = false;
         mafm_status sst = MAFM_OK;
         std::string serr;
         while (complete < n) {
-            bool progress = false;
+            bool moved = false;
             for (int m = 0; m < n; ++m) {
                 if (job_done[m]) continue;
                 const int32_t* f = done + cols * m;
@@ -1856,7 +1861,8 @@ mafm_status mafm_batch_run_output(int n, mafm_sim* const* sims, const mafm_run_o
                         h = std::min(h, __atomic_load_n(f + col, __ATOMIC_ACQUIRE));
                     if (h > q) {
                         writer.ready(m, c, h);
-                        progress = true;
+                        handed += (double)(h - q) * (writer.chunk_last(m, c) - writer.chunk_first(m, c));
+                        moved = true;
                     }
                 }
                 if (writer.complete(m)) {
@@ -1864,7 +1870,8 @@ mafm_status mafm_batch_run_output(int n, mafm_sim* const* sims, const mafm_run_o
                     ++complete;
                 }
             }
-            if (complete == n || progress) continue;
+            if (moved && progress) progress(handed / total_cells, user);
+            if (complete == n || moved) continue;
             if (finished) {
                 sst = MAFM_ERR_HIP;
                 serr = "the streamed scan finished with columns unmarked";
@@ -1898,8 +1905,21 @@ mafm_status mafm_batch_run_output(int n, mafm_sim* const* sims, const mafm_run_o
             sims[m]->sim.system_index_ = m;
             sims[m]->sim.last_stats_ = ss;
         }
+        if (progress) progress(1.0, user);
         if (stats) *stats = ss;
     });
+}
+
+mafm_status mafm_batch_run_output(int n, mafm_sim* const* sims, const mafm_run_opts* o, int threads,
+                                  mafm_scan_stats* stats) {
+    return run_output(n, sims, o, threads, nullptr, nullptr, stats);
+}
+
+mafm_status mafm_sim_run_output(mafm_sim* sim, const mafm_run_opts* o, int threads, mafm_progress_fn progress,
+                                void* user, mafm_scan_stats* stats) {
+    if (!sim) return MAFM_ERR_INVALID;
+    mafm_sim* one[1] = {sim};
+    return run_output(1, one, o, threads, progress, user, stats);
 }
 
 mafm_status mafm_batch_write_output(int n, mafm_sim* const* sims, int threads) {

@@ -74,6 +74,11 @@ def test_cli_writes_the_reference_files(tmp_path, precision):
                  "+- How many x,y points did each process handle:", "+-     Process  0:   1271 x,y points"):
         assert line + "\n" in run.stdout, line
     assert run.stdout.index(echo) < run.stdout.index("+- Starting simulation") < run.stdout.index("+- Statistics:")
+    # progress while the scan runs, at every tenth as the reference prints it
+    # (simulation.cpp:131-137, "+- Finished %4.1f %% of the simulation")
+    marks = [run.stdout.index(f"+- Finished {10.0 * q:4.1f} % of the simulation\n") for q in range(1, 11)]
+    assert marks == sorted(marks) and run.stdout.index("+- Starting simulation") < marks[0]
+    assert marks[-1] < run.stdout.index("+- Statistics:") and run.stdout.count("+- Finished") == 10
 
 
 @pytest.mark.parametrize("mutate,message", [
