@@ -437,6 +437,11 @@ typedef struct {
     double amplitude;      /* oscillation amplitude, A (0.50, :81) */
     double k_cantilever;   /* cantilever spring constant, N/m (1800, :82) */
     double f0;             /* resonance frequency, Hz (25e3, :83) */
+    int32_t fz_from_text;  /* 1: Fz as the script reads it back from the per-height
+                              files (:147, :183): printed "%8.4f" (the exact value
+                              rounded to 4 decimals, ties to even) and parsed by
+                              float(); 0: the records' Fz as they are */
+    int32_t reserved;
 } mafm_deltaf_desc;
 
 typedef struct {

@@ -116,7 +116,8 @@ mafm_status mafm_sim_write_output(mafm_sim* sim, int threads);
 mafm_status mafm_sim_log(mafm_sim* sim, char* buf, size_t size, size_t* needed);
 /* frequency-shift images of the last run (simple-vafm-image.py:76-204) as
  * deltaf-%06.3f.dat files in the output folder; desc NULL = the script's
- * amplitude, cantilever and frequency */
+ * amplitude, cantilever and frequency, and Fz as it reads it from the
+ * per-height files (fz_from_text) */
 mafm_status mafm_sim_write_deltaf(mafm_sim* sim, const mafm_deltaf_desc* desc);
 /* the warnings raised while reading the inputs and the input-settings echo,
  * in the reference's console format (messages.cpp:29-56, parse.cpp:407-473,

@@ -144,7 +144,8 @@ class RunOpts(ctypes.Structure):
 _P = ctypes.POINTER
 _vp = ctypes.c_void_p
 class DeltaFDesc(ctypes.Structure):
-    _fields_ = [("amplitude", ctypes.c_double), ("k_cantilever", ctypes.c_double), ("f0", ctypes.c_double)]
+    _fields_ = [("amplitude", ctypes.c_double), ("k_cantilever", ctypes.c_double), ("f0", ctypes.c_double),
+                ("fz_from_text", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class DeltaFWeights(ctypes.Structure):
@@ -382,10 +383,11 @@ class Context:
         return tuple(out)
 
     def delta_f(self, system: int, nx: int, ny: int, nz: int, zhigh: float, dz: float, amplitude: float = 0.0,
-                k_cantilever: float = 0.0, f0: float = 0.0):
+                k_cantilever: float = 0.0, f0: float = 0.0, fz_from_text: bool = False):
         """Frequency-shift images of `system` from the last scan's records
-        (mafm_delta_f): (df [n_images, nx, ny] in Hz, z_low [n_images])."""
-        d = DeltaFDesc(amplitude, k_cantilever, f0)
+        (mafm_delta_f): (df [n_images, nx, ny] in Hz, z_low [n_images]).
+        fz_from_text: Fz as the script reads it from the "%8.4f" files."""
+        d = DeltaFDesc(amplitude, k_cantilever, f0, int(fz_from_text), 0)
         w = deltaf_weights(nz, zhigh, dz, amplitude, k_cantilever, f0)
         df = np.zeros((w["n_images"], nx, ny))
         zl = np.zeros(w["n_images"])
@@ -592,8 +594,9 @@ class Simulation:
     def write_output(self, threads: int = 8) -> None:
         self.check(lib().mafm_sim_write_output(self.h, threads))
 
-    def write_deltaf(self, amplitude: float = 0.0, k_cantilever: float = 0.0, f0: float = 0.0) -> None:
-        d = DeltaFDesc(amplitude, k_cantilever, f0)
+    def write_deltaf(self, amplitude: float = 0.0, k_cantilever: float = 0.0, f0: float = 0.0,
+                     fz_from_text: bool = True) -> None:
+        d = DeltaFDesc(amplitude, k_cantilever, f0, int(fz_from_text), 0)
         self.check(lib().mafm_sim_write_deltaf(self.h, ctypes.byref(d)))
 
 
@@ -601,7 +604,7 @@ def deltaf_weights(nz: int, zhigh: float, dz: float, amplitude: float = 0.0, k_c
                    f0: float = 0.0) -> dict:
     """Window and constants of the frequency-shift images (mafm_deltaf_weights;
     host only, no device needed)."""
-    d = DeltaFDesc(amplitude, k_cantilever, f0)
+    d = DeltaFDesc(amplitude, k_cantilever, f0, 0, 0)
     o = DeltaFWeights()
     rc = lib().mafm_deltaf_weights(None, ctypes.byref(d), nz, zhigh, dz, ctypes.byref(o))
     if rc != MAFM_OK:

@@ -33,7 +33,7 @@ hipError_t launch_record_totals(const DevRecord* rec, long n, int64_t* out, hipS
 hipError_t noalloc_scratch_address(void** out);
 bool deltaf_weights(const mafm_deltaf_desc& d, int nz, double zhigh, double dz_input, mafm_deltaf_weights_out& o,
                     std::vector<double>* z_low, char* err, size_t err_len);
-hipError_t launch_delta_f(const DevRecord* rec, const mafm_deltaf_weights_out& w, int nx, int ny, int nz,
+hipError_t launch_delta_f(const DevRecord* rec, const mafm_deltaf_weights_out& w, int nx, int ny, int nz, bool text,
                           double* out, hipStream_t st);
 bool fft_length_supported(int n);
 void fft_twiddles(int n, std::vector<double>& out);
@@ -1414,7 +1414,7 @@ mafm_status mafm_delta_f(mafm_ctx* c, int system, const mafm_deltaf_desc* d, dou
     DevBuf<double> out;
     HIPCHK(c, out.reserve(n));
     const DevRecord* rec = c->d_rec.p + (c->last_flex ? 0 : c->mols[system].rec_off);
-    HIPCHK(c, launch_delta_f(rec, w, s.nx, s.ny, s.nz, out.p, c->stream));
+    HIPCHK(c, launch_delta_f(rec, w, s.nx, s.ny, s.nz, d->fz_from_text != 0, out.p, c->stream));
     HIPCHK(c, hipMemcpyAsync(df, out.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (z_low) memcpy(z_low, zl.data(), zl.size() * sizeof(double));

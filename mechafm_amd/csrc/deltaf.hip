@@ -132,7 +132,25 @@ struct DeltaFArgs {
     double neg_prefactor;
     double dz;
     int n_w, n_img, nx, ny, nz;
+    int text;
 };
+
+// v as printf("%8.4f") prints it and float() reads it back: the exact binary
+// value times 10^4 (p + err, err exact by FMA) rounded to an integer q, ties
+// to even, then q / 10^4 rounded once (float() of a decimal string is the
+// nearest double).  |v| 10^4 < 2^52 for any force here; larger values are
+// returned as they are.
+__device__ inline double text4(double v) {
+    const double a = fabs(v);
+    const double p = a * 1e4;
+    if (!(p < 4503599627370496.0)) return v;
+    const double err = fma(a, 1e4, -p);
+    double fl = floor(p);
+    const double t = ((p - fl) - 0.5) + err;
+    if (t > 0 || (t == 0 && fmod(fl, 2.0) == 1.0)) fl += 1.0;
+    const double q = fl / 1e4;
+    return signbit(v) ? -q : q;
+}
 
 // One lane per (image, pixel): image p covers ascending heights p .. p+n_w-1,
 // i.e. records k = nz-1-p .. nz-n_w-p of the pixel's column.
@@ -144,7 +162,10 @@ __global__ void __launch_bounds__(256) k_delta_f(const DevRecord* rec, DeltaFArg
     const int p = (int)(q / ((long)a.nx * a.ny));
     const DevRecord* col = rec + pix * a.nz;
     // np_pairwise_sum over the products, streamed (n_w <= 128: one block)
-    auto term = [&](int t) { return col[a.nz - 1 - p - t].fz * a.w[t]; };
+    auto term = [&](int t) {
+        const double fz = col[a.nz - 1 - p - t].fz;
+        return (a.text ? text4(fz) : fz) * a.w[t];
+    };
     const int n = a.n_w;
     double s;
     if (n < 8) {
@@ -164,9 +185,10 @@ __global__ void __launch_bounds__(256) k_delta_f(const DevRecord* rec, DeltaFArg
 
 }  // namespace
 
-hipError_t launch_delta_f(const DevRecord* rec, const mafm_deltaf_weights_out& w, int nx, int ny, int nz,
+hipError_t launch_delta_f(const DevRecord* rec, const mafm_deltaf_weights_out& w, int nx, int ny, int nz, bool text,
                           double* out, hipStream_t st) {
     DeltaFArgs a;
+    a.text = text ? 1 : 0;
     for (int t = 0; t < 128; ++t) a.w[t] = w.weights[t];
     a.neg_prefactor = -w.prefactor;
     a.dz = w.dz;

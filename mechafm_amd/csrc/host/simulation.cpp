@@ -1944,7 +1944,8 @@ mafm_status mafm_batch_write_output(int n, mafm_sim* const* sims, int threads) {
 
 mafm_status mafm_sim_write_deltaf(mafm_sim* s, const mafm_deltaf_desc* d) {
     if (!s) return MAFM_ERR_INVALID;
-    const mafm_deltaf_desc def{};
+    mafm_deltaf_desc def{};
+    def.fz_from_text = 1;   // the script reads Fz back from the per-height files
     return guarded(s, [&]() { s->sim.writeDeltaF(d ? *d : def); });
 }
 

@@ -66,6 +66,26 @@ def images(fz_ascending: np.ndarray, zpos: Sequence[float], **kw) -> Tuple[np.nd
     return out, np.array(zpos[:max(nshifts, 0)])
 
 
+def fz_text(fz: np.ndarray) -> np.ndarray:
+    """Fz as the script reads it (simple-vafm-image.py:147, 183): the value
+    printed "%8.4f" by the scan's writer (simulation.cpp:180-195) and parsed
+    back with float()."""
+    return np.array([float("%8.4f" % v) for v in np.ravel(fz)]).reshape(np.shape(fz))
+
+
+def fz_from_files(files: dict, nx: int, ny: int) -> np.ndarray:
+    """[nx, ny, nz] Fz in ascending height from per-height files given as
+    {name "scan-%06.3f.dat": lines}, as the script fills FZ (column 8 of a
+    line "k i j x y z fx fy fz ...")."""
+    names = sorted(files, key=lambda n: float(n[5:11]))
+    out = np.zeros((nx, ny, len(names)))
+    for z, name in enumerate(names):
+        for line in files[name]:
+            t = line.split()
+            out[int(t[1]), int(t[2]), z] = float(t[8])
+    return out
+
+
 def fz_from_records(fz_records: np.ndarray, nx: int, ny: int, nz: int) -> np.ndarray:
     """Fz of the records (index (i*ny + j)*nz + k, k = 0 at zhigh) as
     [nx, ny, nz] with ascending height."""
