@@ -408,17 +408,22 @@ __device__ inline void pair64(const SiteF64& st, double tx, double ty, double tz
 // B independent dependency chains).  The FIRE + midpoint kernels, built for
 // three waves per SIMD, use 2: with 4 the block's FP64 temporaries push ~45
 // values of the minimiser state into scratch, rewritten every loop trip.
-template <int SEG, bool SC, bool E, int B>
+// IMG = false (one image): the tip minus the image shift is formed once for
+// the whole segment instead of once per block; the same values.
+template <int SEG, bool SC, bool E, int B, bool IMG>
 __device__ inline void seg_sum64(cptr<DevMol> M, const Sites<SC, SiteF64>& s, int n, int gl, int S, double px,
                                  double py, double pz, double acc[4]) {
-    const int n_img = M->n_images;
+    const int n_img = IMG ? M->n_images : 1;
+    const double x0 = px - M->shift[0][0], y0 = py - M->shift[0][1], z0 = pz - M->shift[0][2];
     const int full = (n / (B * S)) * (B * S);
     for (int t0 = 0; t0 < full; t0 += B * S) {
         SiteF64 st[B];
 #pragma unroll
         for (int u = 0; u < B; ++u) st[u] = s[t0 + u * S + gl];
         for (int im = 0; im < n_img; ++im) {
-            const double tx = px - M->shift[im][0], ty = py - M->shift[im][1], tz = pz - M->shift[im][2];
+            const double tx = IMG ? px - M->shift[im][0] : x0;
+            const double ty = IMG ? py - M->shift[im][1] : y0;
+            const double tz = IMG ? pz - M->shift[im][2] : z0;
 #pragma unroll
             for (int u = 0; u < B; ++u) {
                 double fx, fy, fz, e;
@@ -431,7 +436,9 @@ __device__ inline void seg_sum64(cptr<DevMol> M, const Sites<SC, SiteF64>& s, in
     for (int t = full + gl; t < n; t += S) {
         const SiteF64 st = s[t];
         for (int im = 0; im < n_img; ++im) {
-            const double tx = px - M->shift[im][0], ty = py - M->shift[im][1], tz = pz - M->shift[im][2];
+            const double tx = IMG ? px - M->shift[im][0] : x0;
+            const double ty = IMG ? py - M->shift[im][1] : y0;
+            const double tz = IMG ? pz - M->shift[im][2] : z0;
             double fx, fy, fz, e;
             pair64<SEG>(st, tx, ty, tz, fx, fy, fz, e);
             acc[0] += fx; acc[1] += fy; acc[2] += fz;
@@ -440,22 +447,31 @@ __device__ inline void seg_sum64(cptr<DevMol> M, const Sites<SC, SiteF64>& s, in
     }
 }
 
+template <bool SC, bool E, int B, bool IMG>
+__device__ inline void seg_sums64(cptr<DevMol> M, const Sites<SC, SiteF64> s0, double px, double py, double pz,
+                                  int gl, int S, double acc[4]) {
+    Sites<SC, SiteF64> s = s0;
+    int n = M->n_seg[0];
+    seg_sum64<0, SC, E, B, IMG>(M, s, n, gl, S, px, py, pz, acc);
+    s.advance(n);
+    n = M->n_seg[1];
+    seg_sum64<1, SC, E, B, IMG>(M, s, n, gl, S, px, py, pz, acc);
+    s.advance(n);
+    n = M->n_seg[2];
+    if (n) seg_sum64<2, SC, E, B, IMG>(M, s, n, gl, S, px, py, pz, acc);
+    s.advance(n);
+    n = M->n_seg[3];
+    if (n) seg_sum64<3, SC, E, B, IMG>(M, s, n, gl, S, px, py, pz, acc);
+}
+
 template <bool SC, bool E, int B>
 __device__ void site_sums64(cptr<DevMol> M, const Sites<SC, SiteF64> s0, double px, double py, double pz, int gl,
                             int S, double out[4]) {
     double acc[4] = {0, 0, 0, 0};
-    Sites<SC, SiteF64> s = s0;
-    int n = M->n_seg[0];
-    seg_sum64<0, SC, E, B>(M, s, n, gl, S, px, py, pz, acc);
-    s.advance(n);
-    n = M->n_seg[1];
-    seg_sum64<1, SC, E, B>(M, s, n, gl, S, px, py, pz, acc);
-    s.advance(n);
-    n = M->n_seg[2];
-    if (n) seg_sum64<2, SC, E, B>(M, s, n, gl, S, px, py, pz, acc);
-    s.advance(n);
-    n = M->n_seg[3];
-    if (n) seg_sum64<3, SC, E, B>(M, s, n, gl, S, px, py, pz, acc);
+    if (M->n_images > 1)
+        seg_sums64<SC, E, B, true>(M, s0, px, py, pz, gl, S, acc);
+    else
+        seg_sums64<SC, E, B, false>(M, s0, px, py, pz, gl, S, acc);
     out[0] = acc[0];
     out[1] = acc[1];
     out[2] = acc[2];
