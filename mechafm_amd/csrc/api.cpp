@@ -1054,6 +1054,57 @@ static bool flex_tip_surface(const mafm_flex_term& t) {   // isTipSurface (inter
     return (t.kind == MAFM_FX_LJ || t.kind == MAFM_FX_MORSE || t.kind == MAFM_FX_COULOMB) && t.atom[0] == 1;
 }
 
+// Relative length of one term evaluation's dependent chain by kind (square
+// roots, IEEE divisions and libm calls set it: a dihedral has three roots,
+// seven divisions and an atan, an angle an acos, a Morse term an exp).
+static double flex_cost(int kind) {
+    switch (kind) {
+        case MAFM_FX_DIHEDRAL: return 16;
+        case MAFM_FX_ANGLE: return 8;
+        case MAFM_FX_MORSE: return 5;
+        case MAFM_FX_SUBSTRATE: return 4;
+        case MAFM_FX_XY_HARMONIC: return 1;
+        default: return 3;
+    }
+}
+
+// Which terms each thread of a `waves`-wavefront block evaluates (the
+// FlexArgs::sched table): the terms of each kind, in list order, are cut
+// into wave-rounds of 64, so the 64 lanes of a round run one code path, and
+// the rounds are dealt longest first, each to the wave with the least work
+// so far.  Returns the number of rounds.
+static int flex_schedule(const std::vector<mafm_flex_term>& eval, int waves, std::vector<int32_t>& sched) {
+    std::map<int, std::vector<int32_t>> by_kind;
+    for (int k = 0; k < (int)eval.size(); ++k) by_kind[eval[k].kind].push_back(k);
+    struct Round {
+        double cost;
+        const int32_t* terms;
+        int n;
+    };
+    std::vector<Round> rounds;
+    for (const auto& kv : by_kind)
+        for (size_t q = 0; q < kv.second.size(); q += 64)
+            rounds.push_back({flex_cost(kv.first), kv.second.data() + q, (int)std::min<size_t>(64, kv.second.size() - q)});
+    std::stable_sort(rounds.begin(), rounds.end(), [](const Round& x, const Round& y) { return x.cost > y.cost; });
+    std::vector<double> load(waves, 0.0);
+    std::vector<std::vector<int>> mine(waves);
+    for (int r = 0; r < (int)rounds.size(); ++r) {
+        const int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[w] += rounds[r].cost;
+        mine[w].push_back(r);
+    }
+    int n_rounds = 0;
+    for (const auto& m : mine) n_rounds = std::max(n_rounds, (int)m.size());
+    const int T = 64 * waves;
+    sched.assign((size_t)n_rounds * T, -1);
+    for (int w = 0; w < waves; ++w)
+        for (int r = 0; r < (int)mine[w].size(); ++r) {
+            const Round& R = rounds[mine[w][r]];
+            for (int l = 0; l < R.n; ++l) sched[(size_t)r * T + 64 * w + l] = R.terms[l];
+        }
+    return n_rounds;
+}
+
 mafm_status mafm_scan_flexible(mafm_ctx* c, const mafm_flex_desc* s, const mafm_scan_desc* d,
                                mafm_scan_stats* stats) {
     if (!c || !s || !d) return MAFM_ERR_INVALID;
@@ -1165,6 +1216,12 @@ mafm_status mafm_scan_flexible(mafm_ctx* c, const mafm_flex_desc* s, const mafm_
     int per_cu = 8 / waves;
     if (in_lds) per_cu = (int)std::max<size_t>(1, std::min<size_t>(per_cu, c->lds_per_cu / block_bytes));
     const int n_blocks = (int)std::min<long>(n_cols, (long)c->cus * per_cu);
+    // the term schedule goes behind the index tables (read once per thread,
+    // not copied to LDS)
+    std::vector<int32_t> sched;
+    const int sched_rounds = flex_schedule(eval, waves, sched);
+    const size_t sched_at = ints.size();
+    ints.insert(ints.end(), sched.begin(), sched.end());
 
     HIPCHK(c, c->d_fx_terms.reserve(std::max(n_eval, 1)));
     HIPCHK(c, c->d_fx_ints.reserve(ints.size()));
@@ -1185,6 +1242,8 @@ mafm_status mafm_scan_flexible(mafm_ctx* c, const mafm_flex_desc* s, const mafm_
 
     a.terms = c->d_fx_terms.p;
     a.idx = c->d_fx_ints.p;
+    a.sched = sched_rounds > 0 ? c->d_fx_ints.p + sched_at : nullptr;
+    a.sched_rounds = sched_rounds;
     a.pos0 = c->d_fx_pos.p;
     a.rec = c->d_rec.p;
     a.next_col = c->d_next.p;

@@ -27,6 +27,13 @@ struct FlexArgs {
     const int32_t* idx;
     int32_t n_csr, n_tip;
     int32_t tip_ts_slots, tip_ts_terms;   // the tip's slots / terms from tip-surface terms
+    // which terms each thread evaluates in one integrator stage: thread l of
+    // the block takes sched[r * block threads + l] (-1: none) in rounds
+    // r < sched_rounds; NULL: terms l, l + T, ...
+    // The schedule only places the work: every term still writes its own
+    // result slots, which are summed in list order.
+    const int32_t* sched;
+    int32_t sched_rounds;
     const double* pos0;               // n_atoms x 3
     int32_t tip_pbc;
     int32_t state_col;                // column whose positions are kept per height, -1 none

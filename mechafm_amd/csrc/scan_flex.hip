@@ -8,11 +8,13 @@
 // results and the index tables.
 //
 // One integrator stage evaluates the interaction list in two phases:
-//   1. term-parallel: thread l evaluates terms l, l + T, ... and writes one
-//      result slot {fx, fy, fz} per atom the term acts on, the force that
-//      atom receives (interactions.cpp:15-51, 252-362; `forces[i] -= f` is
-//      stored as -f, which adds with the same rounding), and the term's
-//      energy once (only the tip's energy is ever read);
+//   1. term-parallel: each thread evaluates the terms the host's schedule
+//      gives it (64 terms of one kind per wave-round, so a wave runs one
+//      code path at a time, the rounds spread over the waves by cost) and
+//      writes one result slot {fx, fy, fz} per atom the term acts on, the
+//      force that atom receives (interactions.cpp:15-51, 252-362;
+//      `forces[i] -= f` is stored as -f, which adds with the same rounding),
+//      and the term's energy once (only the tip's energy is ever read);
 //   2. atom-parallel: thread l sums the slots of its atoms in the
 //      interaction list's order, from zero -- the order in which the
 //      reference's Interaction::eval calls accumulate forces_[i] and
@@ -275,13 +277,26 @@ __device__ inline double seq_sum(const double* v, const int32_t* idx, int n) {
 // energy (into S.bc[0]): the interaction list evaluated as one integrator
 // stage does (integrators.cpp:31-33).  ts_only: only the tip-surface terms,
 // for the tip, as System::evalTipSurfaceForces does (system.cpp:41-51).
-// t / nthr: this thread and the block's threads; the block's last thread
-// sums the tip.
+// t / nthr: this thread and the block's threads; the terms follow the
+// schedule a.sched when there is one.  The block's last thread sums the tip.
 __device__ void evaluate(const FlexArgs& a, const State& S, const double* X, int t, int nthr, bool ts_only) {
-    const int ne = ts_only ? a.n_ts : a.n_eval;
-    for (int k = t; k < ne; k += nthr) {
-        const mafm_flex_term& term = a.terms[k];
-        eval_term(term, X, S.res + 3 * term.reserved, S.ener[k]);   // reserved: the term's first slot
+    if (ts_only || !a.sched) {
+        const int ne = ts_only ? a.n_ts : a.n_eval;
+        for (int k = t; k < ne; k += nthr) {
+            const mafm_flex_term& term = a.terms[k];
+            eval_term(term, X, S.res + 3 * term.reserved, S.ener[k]);   // reserved: the term's first slot
+        }
+    } else {
+        // the next round's term is loaded while this round's is evaluated
+        int k = a.sched[t];
+        for (int r = 0; r < a.sched_rounds; ++r) {
+            const int next = r + 1 < a.sched_rounds ? a.sched[(r + 1) * nthr + t] : -1;
+            if (k >= 0) {
+                const mafm_flex_term& term = a.terms[k];
+                eval_term(term, X, S.res + 3 * term.reserved, S.ener[k]);
+            }
+            k = next;
+        }
     }
     block_sync();
     if (!ts_only)
