@@ -349,7 +349,9 @@ mafm_status mafm_scan_noalloc(mafm_ctx* ctx, const mafm_scan_desc* desc, mafm_re
  * count with acquire ordering).  mafm_scan_host_query reports whether the launch
  * has finished without waiting; mafm_scan_host_wait ends it and fills the
  * statistics (MAFM_ERR_OUT_OF_GRID as mafm_scan).  Until then every other
- * scan on the context is refused with MAFM_ERR_STATE.  The host memory stays
+ * scan on the context (mafm_scan_flexible included) and mafm_set_systems are
+ * refused with MAFM_ERR_STATE; other calls may run and wait behind the
+ * launch, the scan's statistics are not affected.  The host memory stays
  * valid until the next streamed scan or mafm_destroy; the context holds no
  * device records afterwards (mafm_records / mafm_delta_f refuse).
  * MAFM_SCAN_MARGINS is refused. */

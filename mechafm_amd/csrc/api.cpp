@@ -185,6 +185,7 @@ struct mafm_ctx {
     // streamed scans (mafm_scan_host): records and per-column flags in pinned host memory
     HostBuf<DevRecord> h_rec;
     HostBuf<int32_t> h_done;
+    HostBuf<int64_t> h_counters;   // the streamed launch's counters, copied right behind it
     bool host_pending = false;     // launched, not yet ended by mafm_scan_host_wait
     PendingScan pending;
     // flexible scan: term list, index tables, initial positions, kept states, per-block work
@@ -589,6 +590,7 @@ static void destroy_now(mafm_ctx* c) {
     c->d_margins.release();
     c->h_rec.release();
     c->h_done.release();
+    c->h_counters.release();
     c->d_fx_terms.release();
     c->d_fx_ints.release();
     c->d_fx_pos.release();
@@ -767,6 +769,8 @@ mafm_status mafm_grid_build(mafm_ctx* c, const mafm_system_desc* sys, const mafm
 
 mafm_status mafm_set_systems(mafm_ctx* c, int n, const mafm_system_desc* sys) {
     if (!c || n <= 0 || !sys) return fail(c, MAFM_ERR_INVALID, "need at least one system");
+    if (c->host_pending)
+        return fail(c, MAFM_ERR_STATE, "a streamed scan of these systems is still running (mafm_scan_host_wait ends it)");
     HIPCHK(c, hipSetDevice(c->device));
     std::vector<DevMol> mols;
     std::vector<Site64> ref;
@@ -904,6 +908,7 @@ static mafm_status scan_impl(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_sta
         // the records of this scan live in pinned host memory only
         HIPCHK(c, c->h_rec.reserve(n_rec));
         HIPCHK(c, c->h_done.reserve((size_t)cols_per_mol * n_mols));
+        HIPCHK(c, c->h_counters.reserve(8));
         HIPCHK(c, hipStreamSynchronize(c->stream));   // no earlier work still reads the flags
         memset(c->h_done.p, 0, (size_t)cols_per_mol * n_mols * sizeof(int32_t));
         c->have_scan = false;
@@ -970,7 +975,10 @@ static mafm_status scan_impl(mafm_ctx* c, const mafm_scan_desc* d, mafm_scan_sta
     ps.W = plan.W;
     ps.n_rec = n_rec;
     if (to_host) {
-        // mafm_scan_host_wait finishes it
+        // mafm_scan_host_wait finishes it; the counters are copied out in
+        // stream order now, so later work on the context may reuse them
+        HIPCHK(c, hipMemcpyAsync(c->h_counters.p, a.counters, 8 * sizeof(int64_t), hipMemcpyDeviceToHost,
+                                 c->stream));
         c->pending = ps;
         c->host_pending = true;
         return MAFM_OK;
@@ -1016,9 +1024,9 @@ mafm_status mafm_scan_host_wait(mafm_ctx* c, mafm_scan_stats* stats) {
     if (!c->host_pending) return fail(c, MAFM_ERR_STATE, "no streamed scan is running");
     c->host_pending = false;
     HIPCHK(c, hipSetDevice(c->device));
-    int64_t counters[8];
-    HIPCHK(c, hipMemcpyAsync(counters, c->d_counters.p, sizeof counters, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    int64_t counters[8];
+    memcpy(counters, c->h_counters.p, sizeof counters);
     if (c->pending.fast) host_record_totals(c->h_rec.p, c->pending.n_rec, counters);
     return scan_stats(c, c->pending, counters, stats);
 }
@@ -1108,6 +1116,7 @@ static int flex_schedule(const std::vector<mafm_flex_term>& eval, int waves, std
 mafm_status mafm_scan_flexible(mafm_ctx* c, const mafm_flex_desc* s, const mafm_scan_desc* d,
                                mafm_scan_stats* stats) {
     if (!c || !s || !d) return MAFM_ERR_INVALID;
+    if (c->host_pending) return fail(c, MAFM_ERR_STATE, "a streamed scan is still running (mafm_scan_host_wait ends it)");
     mafm_status sd = check_scan_desc(c, d);
     if (sd != MAFM_OK) return sd;
     const int n = s->n_atoms;

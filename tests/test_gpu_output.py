@@ -3,7 +3,9 @@
 * mafm_scan_host (the streamed scan) through the C ABI: the records it
   stores in pinned host memory equal mafm_scan's bit for bit (reproducible
   mode) in every precision, every column's height count is nz once the scan
-  ends, and the context refuses another scan while one is streaming.
+  ends, and the context refuses another scan (flexible too) and new systems
+  while one is streaming; a call that reuses the device counters while it
+  streams leaves the scan's statistics as mafm_scan reports them.
 * mafm_batch_run_output: the 64 molecules of the batched workload (2 x 2
   pixels each, all 40 heights) are scanned in one launch while a pool of
   writers formats every chunk of columns, height by height, as it lands.
@@ -41,6 +43,17 @@ def test_streamed_scan_equals_scan(gpu_ctx, tmp_path, precision):
         recs, flags = gpu_ctx.scan_host(d, len(descs))
         with pytest.raises(abi.MafmError, match="still running"):
             gpu_ctx.scan(d)
+        if rep == 0:
+            # the flexible scan and new systems wait for the end; other calls
+            # run behind the launch and leave its statistics alone (this one
+            # reuses the context's device counters)
+            with pytest.raises(abi.MafmError, match="still running"):
+                gpu_ctx.set_systems(descs)
+            with pytest.raises(abi.MafmError, match="still running"):
+                gpu_ctx.scan_flexible(2, np.zeros((2, 3)), np.ones(2, dtype=np.int32),
+                                      np.zeros(0, dtype=abi.FLEX_TERM_DTYPE), d)
+            fe = gpu_ctx.eval_tip_surface(0, np.array([[0.1, 0.2, 8.0]]))
+            assert np.isfinite(fe).all()
         t0 = time.time()
         while not gpu_ctx.scan_host_query():   # the launch ends by itself
             assert time.time() - t0 < 60
