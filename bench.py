@@ -533,7 +533,7 @@ def main():
         head = timed_scan(abi, sims, args.precision, args, dev, options, positions, args.precision)
         st = head["stats"]
         out = {
-            "metric": "relaxed tip positions/s at 1e-4 A error; % FP32 VALU peak vs plain FP64",
+            "metric": "relaxed tip positions/s at 1e-4 Å error; % FP32 VALU peak vs plain FP64",
             "value": head["value"], "unit": "positions/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": {"fast32": "f32"}.get(args.precision, "f64"), "data": "synthetic",
