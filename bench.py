@@ -480,6 +480,9 @@ def main():
                     help="time the flexible scan of the reference's example job at full size instead")
     ap.add_argument("--margins", action="store_true",
                     help="decision-margin analysis of the FP32 scan against FP64 (no timing line)")
+    ap.add_argument("--valu-calibration", action="store_true",
+                    help="only run the VALU calibration chains (FP32 FMA, FP64 FMA, FP64 rsq + add), one launch each "
+                         "after a warm-up, and print their wave instructions and times")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (pixel columns would shard "
@@ -495,6 +498,20 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": n * 4, "launches": 3}))
+        return
+
+    if args.valu_calibration:
+        from mechafm_amd import abi
+        ctx = abi.Context(0)
+        out = {}
+        for kind, name in ((0, "fp32_fma"), (1, "fp64_fma"), (2, "fp64_rsq_add")):
+            ctx.valu_calibration(kind)                     # warm-up launch
+            r = ctx.valu_calibration(kind)
+            # wave instructions per SIMD per nanosecond over the launch (the
+            # counters give the clock: SQ_BUSY_CYCLES)
+            r["wave_insts_per_simd_ns"] = r["wave_instructions"] / (r["ms"] * 1e6) / (4 * ctx.device_info()["compute_units"])
+            out[name] = r
+        print(json.dumps({"valu_calibration": out, "launches_per_kind": 2}), flush=True)
         return
 
     from mechafm_amd import abi

@@ -369,6 +369,13 @@ mafm_status mafm_records(mafm_ctx* ctx, size_t first, size_t count, mafm_record*
  * iterations (+inf where none was counted), then 0.  No reference
  * counterpart: a diagnostic of the FP32 arithmetic. */
 mafm_status mafm_margins(mafm_ctx* ctx, size_t first, size_t count, float* out);
+/* VALU issue-rate calibration.  No reference counterpart: a measurement aid
+ * for the device's performance counters.  One launch of 32 768 wave64s, each
+ * lane running 8 independent chains of `iterations` steps: kind 0 FP32 FMAs,
+ * 1 FP64 FMAs, 2 FP64 v_rsq_f64 each followed by an addition.  *ms is the
+ * launch's time (HIP events), *wave_instructions the VALU wave instructions
+ * of its chains (known exactly, to set against SQ_INSTS_VALU). */
+mafm_status mafm_valu_calibration(mafm_ctx* ctx, int kind, int iterations, double* ms, int64_t* wave_instructions);
 mafm_status mafm_eval_tip_surface(mafm_ctx* ctx, int system, size_t n, const double* pos, double* fe);
 /* The scan with a flexible surface: the pixel loop of Simulation::run
  * (simulation.cpp:78-140) over one System copy per column, whose surface

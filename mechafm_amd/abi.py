@@ -174,6 +174,8 @@ SIGNATURES = {
     "mafm_grid_interpolate": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_size_t, _P(ctypes.c_double),
                                              _P(ctypes.c_double)]),
     "mafm_margins": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t, _P(ctypes.c_float)]),
+    "mafm_valu_calibration": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _P(ctypes.c_double),
+                                             _P(ctypes.c_int64)]),
     "mafm_scan_flexible": (ctypes.c_int, [_vp, _P(FlexDesc), _P(ScanDesc), _P(ScanStats)]),
     "mafm_system_eval": (ctypes.c_int, [_vp, _P(InteractionList), ctypes.c_int64, _P(ctypes.c_double),
                                         _P(ctypes.c_double), _P(ctypes.c_double)]),
@@ -368,6 +370,14 @@ class Context:
         out = np.zeros((count, 4), dtype=np.float32)
         self.check(lib().mafm_margins(self.h, first, count, out.ctypes.data_as(_P(ctypes.c_float))))
         return out
+
+    def valu_calibration(self, kind: int, iterations: int = 4096) -> dict:
+        """One launch of the VALU calibration chains (mafm_valu_calibration):
+        kind 0 FP32 FMAs, 1 FP64 FMAs, 2 FP64 v_rsq + add."""
+        ms = ctypes.c_double()
+        insts = ctypes.c_int64()
+        self.check(lib().mafm_valu_calibration(self.h, kind, iterations, ctypes.byref(ms), ctypes.byref(insts)))
+        return {"ms": ms.value, "wave_instructions": insts.value}
 
     def adjoint(self, system: int, n_sites: int, tip: np.ndarray, dummy: np.ndarray, g: np.ndarray,
                 precision: str = "fast64"):

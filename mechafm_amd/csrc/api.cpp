@@ -24,6 +24,8 @@ namespace mafm {
 hipError_t launch_scan_plain(const ScanArgs& a, hipStream_t st);
 hipError_t launch_eval_plain(const ScanArgs& a, int mol, long n, const double* pos, double* out, hipStream_t st);
 hipError_t launch_grid_interp(const ScanArgs& a, int grid, long n, const double* pos, double* out, hipStream_t st);
+size_t valu_calibration_doubles();
+hipError_t launch_valu_calibration(int kind, int n, double* out, int64_t* wave_insts, hipStream_t st);
 hipError_t launch_grid_build(const ScanArgs& a, int mol, const int* n, const double* s, const double* o,
                              double4* out, hipStream_t st);
 hipError_t launch_scan_fast(const ScanArgs& a, bool f32, int comp, int G, int n_blocks, int W, int lds_sites,
@@ -678,6 +680,25 @@ mafm_status mafm_grid_interpolate(mafm_ctx* c, int handle, size_t n, const doubl
     if (counters[3] != 0)
         return fail(c, MAFM_ERR_OUT_OF_GRID, "%lld positions needed a grid point past the grid",
                     (long long)counters[3]);
+    return MAFM_OK;
+}
+
+mafm_status mafm_valu_calibration(mafm_ctx* c, int kind, int iterations, double* ms, int64_t* wave_instructions) {
+    if (!c || !ms || !wave_instructions) return MAFM_ERR_INVALID;
+    if (kind < 0 || kind > 2) return fail(c, MAFM_ERR_INVALID, "calibration kind %d (0 FP32 FMA, 1 FP64 FMA, 2 FP64 rsq)", kind);
+    if (iterations < 1 || iterations > (1 << 20)) return fail(c, MAFM_ERR_INVALID, "iterations is %d", iterations);
+    if (c->host_pending) return fail(c, MAFM_ERR_STATE, "a streamed scan is still running (mafm_scan_host_wait ends it)");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, c->d_tmp.reserve(valu_calibration_doubles()));
+    int64_t insts = 0;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, launch_valu_calibration(kind, iterations, c->d_tmp.p, &insts, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float t = 0;
+    HIPCHK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
+    *ms = t;
+    *wave_instructions = insts;
     return MAFM_OK;
 }
 

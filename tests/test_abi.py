@@ -62,6 +62,7 @@ def test_null_arguments_rejected(lib):
     assert lib.mafm_scan_host(None, None, None, None) == 1
     assert lib.mafm_scan_host_query(None, None) == 1
     assert lib.mafm_scan_host_wait(None, None) == 1
+    assert lib.mafm_valu_calibration(None, 0, 1, None, None) == 1
 
 
 STRUCTS = [("mafm_site", "Site"), ("mafm_tip_dummy", "TipDummy"), ("mafm_system_desc", "SystemDesc"),
