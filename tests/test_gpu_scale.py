@@ -9,7 +9,10 @@ regenerated from the seeded generators and checked by SHA-256):
 * c4_full -- the 2000-atom periodic slab (9 images, 18 000 site terms per
   evaluation, the FP32 table staged whole in 64 KiB of LDS) with the Hartree
   potential of a 160 x 160 x 80 cube, 4 x 4 pixels x 40 heights, plus 512
-  tip-surface evaluations against the reference's.
+  tip-surface evaluations against the reference's;
+* the batched workload at its full size (10 485 760 positions), which no
+  reference run covers: fast64 against plain64 on every position, and two
+  reproducible fast64 launches bit for bit.
 
 Bars as in test_gpu_scan: plain64 identical step counts (bit for bit for
 LJ / Coulomb, 1e-9 where the FFT enters), fast64 1e-4 A / 1e-3 Fz on every
@@ -97,3 +100,34 @@ def test_c4_full_tip_surface_evaluations(gpu_ctx, c4_full):
     gpu_ctx.set_systems([c4_full.system()])
     got = gpu_ctx.eval_tip_surface(0, d["pos"])
     assert np.abs(got - d["fe"]).max() <= 1e-9 * max(1.0, np.abs(d["fe"]).max())
+
+
+def test_c3_full_size_fast64_against_plain64(gpu_ctx, tmp_path):
+    """The batched workload at its full size (64 molecules x 64 x 64 pixels x
+    40 heights, 10 485 760 positions), which no reference run covers, through
+    properties that do not depend on size: every fast64 position within
+    1e-4 A / 1e-3 relative Fz of the plain FP64 scan, no unconverged
+    positions in either, and two reproducible fast64 launches equal bit for
+    bit."""
+    from mechafm_amd import abi
+    from mechafm_amd import workloads as W
+    sims = [abi.Simulation(w.input_path, gpu_ctx) for w in W.make_c3(str(tmp_path))]
+    st_plain = abi.batch_run(sims, "plain64")
+    plain = [s.records() for s in sims]
+    st = abi.batch_run(sims, "fast64", options=abi.SCAN_REPRODUCIBLE)
+    fast = [s.records() for s in sims]
+    assert st["positions"] == st_plain["positions"] == 64 * 64 * 64 * 40
+    assert st["unconverged"] == st_plain["unconverged"] == 0
+    worst_dr, worst_fz = 0.0, 0.0
+    for p, f in zip(plain, fast):
+        dr = np.sqrt((f["rx"] - p["rx"]) ** 2 + (f["ry"] - p["ry"]) ** 2 + (f["rz"] - p["rz"]) ** 2)
+        dfz = np.abs(f["fz"] - p["fz"]) / np.maximum(np.abs(p["fz"]), 1e-2)
+        worst_dr, worst_fz = max(worst_dr, float(dr.max())), max(worst_fz, float(dfz.max()))
+    print(f"full c3: max |dr| {worst_dr:.2e} A, max rel dFz {worst_fz:.2e}, "
+          f"fast64 {st['kernel_ms']:.1f} ms, plain64 {st_plain['kernel_ms']:.1f} ms")
+    assert worst_dr <= 1e-4 and worst_fz <= 1e-3
+    abi.batch_run(sims, "fast64", options=abi.SCAN_REPRODUCIBLE)
+    for s, f in zip(sims, fast):
+        assert (s.records() == f).all()
+    for s in sims:
+        s.close()
