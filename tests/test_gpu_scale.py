@@ -12,7 +12,8 @@ regenerated from the seeded generators and checked by SHA-256):
   tip-surface evaluations against the reference's;
 * the batched workload at its full size (10 485 760 positions), which no
   reference run covers: fast64 against plain64 on every position, and two
-  reproducible fast64 launches bit for bit.
+  reproducible fast64 launches bit for bit; the C96 flake, the force-grid
+  scan and the 150-atom scan at their full sizes the same way.
 
 Bars as in test_gpu_scan: plain64 identical step counts (bit for bit for
 LJ / Coulomb, 1e-9 where the FFT enters), fast64 1e-4 A / 1e-3 Fz on every
@@ -131,3 +132,28 @@ def test_c3_full_size_fast64_against_plain64(gpu_ctx, tmp_path):
         assert (s.records() == f).all()
     for s in sims:
         s.close()
+
+
+@pytest.mark.parametrize("config", ["c1", "c2", "c5"])
+def test_full_size_fast64_against_plain64(gpu_ctx, tmp_path, config):
+    """The single-system workloads at their full size (the C96 flake, the
+    40-atom force-grid scan, the 150-atom LJ + Coulomb scan): every fast64
+    position within 1e-4 A / 1e-3 relative Fz of the plain FP64 scan, the
+    same convergence in both, and plain64 deterministic (two launches bit
+    for bit)."""
+    from mechafm_amd import abi
+    from mechafm_amd import workloads as W
+    sim = abi.Simulation(W.MAKERS[config](str(tmp_path)).input_path, gpu_ctx)
+    st_plain = sim.run("plain64")
+    plain = sim.records()
+    st = sim.run("fast64")
+    fast = sim.records()
+    assert st["unconverged"] == st_plain["unconverged"]
+    dr = np.sqrt((fast["rx"] - plain["rx"]) ** 2 + (fast["ry"] - plain["ry"]) ** 2 +
+                 (fast["rz"] - plain["rz"]) ** 2)
+    dfz = np.abs(fast["fz"] - plain["fz"]) / np.maximum(np.abs(plain["fz"]), 1e-2)
+    print(f"{config} full size: {len(plain)} positions, max |dr| {dr.max():.2e} A, max rel dFz {dfz.max():.2e}")
+    assert dr.max() <= 1e-4 and dfz.max() <= 1e-3
+    sim.run("plain64")
+    assert (sim.records() == plain).all()
+    sim.close()
