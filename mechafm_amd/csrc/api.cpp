@@ -418,6 +418,11 @@ struct FastPlan {
     bool scalar_sites = false, mid_family = false;
 };
 
+// LDS bytes the fast scan may stage one molecule's sites in
+static size_t scan_lds_limit(const mafm_ctx* c) {
+    return c->lds_per_block > 1024 ? c->lds_per_block - 1024 : 0;
+}
+
 static mafm_status plan_fast(mafm_ctx* c, const mafm_scan_desc* d, bool f32, const std::vector<long>& cols, int lanes,
                              int waves_per_simd, int waves_per_group, ScanArgs& a, FastPlan& p,
                              ScanScratch* host_ws = nullptr, ScanScratch* dev_ws = nullptr) {
@@ -444,13 +449,17 @@ static mafm_status plan_fast(mafm_ctx* c, const mafm_scan_desc* d, bool f32, con
         // two pixels' share of work, and a lane keeps >= 8 sites
         while (p.G < 64 && (double)total_cols * p.G < 2.0 * lanes_total && p.G * 8 <= max_sites) p.G <<= 1;
     }
-    // a workgroup stages its molecule's sites in LDS (64 KiB at most); larger
-    // molecules are streamed with scalar loads, one lane per pixel
+    // a workgroup stages its molecule's sites in LDS (up to what one
+    // workgroup may allocate, less 1 KiB for the kernel's static LDS: 159 KiB
+    // on gfx950); larger molecules are streamed with scalar loads, one lane
+    // per pixel
     const size_t site_bytes = f32 ? sizeof(Site32) : sizeof(SiteF64);
-    p.scalar_sites = (size_t)max_sites * site_bytes > 64 * 1024;
+    const size_t lds_limit = scan_lds_limit(c);
+    p.scalar_sites = (size_t)max_sites * site_bytes > lds_limit;
     if (p.scalar_sites && p.G > 1) {
         if (lanes > 0)
-            return fail(c, MAFM_ERR_INVALID, "lanes_per_pixel > 1 needs the sites of one molecule to fit 64 KiB of LDS");
+            return fail(c, MAFM_ERR_INVALID, "lanes_per_pixel > 1 needs the sites of one molecule to fit %zu bytes of LDS",
+                        lds_limit);
         p.G = 1;
     }
     p.lds_sites = (int)max_sites;

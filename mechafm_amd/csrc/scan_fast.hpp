@@ -1071,13 +1071,26 @@ __global__ void __launch_bounds__(512, MID ? MAFM_F64_MID_WAVES : 2) k_scan_fast
     scan_fast_body<G, false, 0, SC, MID>(a);
 }
 
+// More than 64 KiB of dynamic LDS per workgroup (a molecule's sites staged
+// whole; gfx950 allows up to 160 KiB) has to be granted per kernel first.
 template <int G, bool F32, int COMP, bool MID, bool SC = false>
 hipError_t launch_one(const ScanArgs& a, int n_blocks, int W, size_t lds, hipStream_t st) {
     const size_t bytes = SC ? 0 : lds;
-    if constexpr (F32)
+    if constexpr (F32) {
+        if (bytes > 64 * 1024) {
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_scan_fast32<G, COMP, SC, MID>),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+            if (e != hipSuccess) return e;
+        }
         hipLaunchKernelGGL((k_scan_fast32<G, COMP, SC, MID>), dim3(n_blocks), dim3(64 * W), bytes, st, a);
-    else
+    } else {
+        if (bytes > 64 * 1024) {
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_scan_fast64<G, SC, MID>),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+            if (e != hipSuccess) return e;
+        }
         hipLaunchKernelGGL((k_scan_fast64<G, SC, MID>), dim3(n_blocks), dim3(64 * W), bytes, st, a);
+    }
     return hipGetLastError();
 }
 

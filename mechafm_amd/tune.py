@@ -25,7 +25,7 @@ from . import abi
 
 DEFAULT_LANES = (1, 2, 4, 8, 16)
 WAVES = (2, 3)
-LDS_BYTES = 64 * 1024                      # one molecule's sites staged per workgroup
+LDS_BYTES = 160 * 1024                     # LDS one workgroup may take (gfx950), when the device is not given
 SITE_BYTES = {"fast32": 32, "fast64": 64}  # bytes per site in LDS
 CACHE_ENV = "MAFM_TUNE_CACHE"
 
@@ -46,16 +46,18 @@ def signature(sims: Sequence[abi.Simulation], precision: str, compensation: int 
               device: Optional[dict] = None) -> str:
     """Key of a workload: device, arithmetic, scan geometry, minimiser, the
     size distribution of the systems (sites per system rounded to two
-    significant digits) and whether every system's sites fit the 64 KiB LDS
-    copy (otherwise the kernel streams them with one lane per pixel, so a
-    shape tuned on a system that fits does not carry over)."""
+    significant digits) and whether every system's sites fit the LDS copy a
+    workgroup may take (the device's per-workgroup LDS less 1 KiB; otherwise
+    the kernel streams them with one lane per pixel, so a shape tuned on a
+    system that fits does not carry over)."""
     info = sims[0].info()
     d = sims[0].scan_desc(precision)
     systems = []
     lds_fit = True
+    lds_limit = int((device or {}).get("lds_per_block") or LDS_BYTES) - 1024
     for s in sims:
         sd = s.system()
-        lds_fit &= sd.force_grid >= 0 or sd.n_sites * SITE_BYTES.get(precision, 64) <= LDS_BYTES
+        lds_fit &= sd.force_grid >= 0 or sd.n_sites * SITE_BYTES.get(precision, 64) <= lds_limit
         systems.append((int(_round2(sd.n_sites)), int(sd.n_images or 1), int(sd.force_grid >= 0),
                         int(sd.potential_grid >= 0)))
     systems.sort()

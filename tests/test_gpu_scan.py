@@ -326,15 +326,42 @@ def test_output_files_match_reference_format(gpu_ctx, tmp_path, gz):
         assert sorted(got) == sorted(want)
 
 
-@pytest.mark.parametrize("precision,n_atoms,area", [("fast64", 1100, "0.3 0.3"), ("fast32", 2100, "2.0 2.0")])
-def test_large_molecule_streams_sites(gpu_ctx, tmp_path, precision, n_atoms, area):
-    """A molecule whose sites exceed 64 KiB of LDS (1024 FP64 / 2048 FP32
-    sites) is streamed with scalar loads, one lane per pixel.  (The FP32 case
-    scans 441 columns so that one column whose FIRE steps FP32 rounding
-    shifts stays well inside the 1 % the bar allows.)"""
+def _large_molecule(tmp_path, n_atoms, area, cell):
+    from mechafm_amd import workloads as W
+    rng = np.random.default_rng(7)
+    mol = W.random_slab(n_atoms, (cell, cell), 4.0, rng)
+    opts = W._base_options(coulomb="on", area=area, zhigh="10.0", zlow="9.0")
+    return W._emit("large", str(tmp_path), mol, opts, True)
+
+
+@pytest.mark.parametrize("precision,n_atoms,area", [("fast64", 1500, "0.3 0.3"), ("fast32", 3000, "2.0 2.0")])
+def test_large_molecule_staged_in_lds(gpu_ctx, tmp_path, precision, n_atoms, area):
+    """A molecule whose sites take more than 64 KiB (96 000 B here) is still
+    staged whole in LDS, up to what one workgroup may take (159 KiB on
+    gfx950), and scanned with several lanes per pixel."""
+    from mechafm_amd import abi
+    sim = abi.Simulation(_large_molecule(tmp_path, n_atoms, area, 40.0).input_path, gpu_ctx)
+    sim.run("plain64")
+    ref = sim.records().copy()
+    st = sim.run(precision)
+    got = sim.records()
+    assert st["lanes_per_pixel"] > 1
+    dpos = np.abs(np.stack([got[c] - ref[c] for c in ("rx", "ry", "rz")])).max(0)
+    dfz = np.abs(got["fz"] - ref["fz"]) / np.maximum(np.abs(ref["fz"]), 1e-2)
+    ok = (dpos <= 1e-4) & (dfz <= 1e-3)
+    assert ok.mean() >= (1.0 if precision == "fast64" else 0.99)
+
+
+@pytest.mark.parametrize("precision,n_atoms,area,cell", [("fast64", 2600, "0.3 0.3", 40.0),
+                                                         ("fast32", 5200, "2.0 2.0", 60.0)])
+def test_large_molecule_streams_sites(gpu_ctx, tmp_path, precision, n_atoms, area, cell):
+    """A molecule whose sites exceed the LDS one workgroup may take (159 KiB:
+    2544 FP64 / 5088 FP32 sites) is streamed with scalar loads, one lane per
+    pixel.  (The FP32 case scans 441 columns so that one column whose FIRE
+    steps FP32 rounding shifts stays well inside the 1 % the bar allows.)"""
     from mechafm_amd import abi, workloads as W
     rng = np.random.default_rng(7)
-    mol = W.random_slab(n_atoms, (40.0, 40.0), 4.0, rng)
+    mol = W.random_slab(n_atoms, (cell, cell), 4.0, rng)
     opts = W._base_options(coulomb="on", area=area, zhigh="10.0", zlow="9.0")
     wl = W._emit("large", str(tmp_path), mol, opts, True)
     sim = abi.Simulation(wl.input_path, gpu_ctx)

@@ -60,8 +60,13 @@ def test_signature_separates_workloads():
     assert a != tune.signature([FakeSim(400)], "fast32")
     assert a != tune.signature([FakeSim(120)], "fast64")
     assert a != tune.signature([FakeSim(120, 9)], "fast32")
-    # 2040 and 2049 FP32 sites round alike, but only the first fits 64 KiB of LDS
-    assert tune.signature([FakeSim(2040)], "fast32") != tune.signature([FakeSim(2049)], "fast32")
+    # 5080 and 5100 FP32 sites round alike, but only the first fits the LDS a
+    # workgroup may take (160 KiB less 1 KiB of static LDS)
+    assert tune.signature([FakeSim(5080)], "fast32") != tune.signature([FakeSim(5100)], "fast32")
+    # on a device that reports 64 KiB per workgroup the split moves with it
+    small = {"name": "x", "compute_units": 1, "lds_per_block": 64 * 1024}
+    assert tune.signature([FakeSim(2000)], "fast32", device=small) != \
+        tune.signature([FakeSim(2049)], "fast32", device=small)
 
 
 def test_plain_has_nothing_to_tune():
