@@ -13,7 +13,8 @@ regenerated from the seeded generators and checked by SHA-256):
 * the batched workload at its full size (10 485 760 positions), which no
   reference run covers: fast64 against plain64 on every position, and two
   reproducible fast64 launches bit for bit; the C96 flake, the force-grid
-  scan and the 150-atom scan at their full sizes the same way.
+  scan and the 150-atom scan at their full sizes the same way; the periodic
+  slab at full size through its FP32 and FP64 scans against each other.
 
 Bars as in test_gpu_scan: plain64 identical step counts (bit for bit for
 LJ / Coulomb, 1e-9 where the FFT enters), fast64 1e-4 A / 1e-3 Fz on every
@@ -156,4 +157,31 @@ def test_full_size_fast64_against_plain64(gpu_ctx, tmp_path, config):
     assert dr.max() <= 1e-4 and dfz.max() <= 1e-3
     sim.run("plain64")
     assert (sim.records() == plain).all()
+    sim.close()
+
+
+def test_c4_full_size_fast32_against_fast64(gpu_ctx, tmp_path):
+    """The 2000-atom periodic slab at its full size (256 x 256 pixels x 40
+    heights, 2 621 440 positions; the plain FP64 scan would take minutes):
+    the FP32 pair sums (their 64 000 B of sites in LDS) within 1e-4 A /
+    1e-3 relative Fz of the FP64 ones (their 128 000 B of sites in LDS, 8
+    waves sharing it) on at least 99 % of the positions, and two reproducible
+    fast64 launches bit for bit."""
+    from mechafm_amd import abi
+    from mechafm_amd import workloads as W
+    sim = abi.Simulation(W.make_c4(str(tmp_path)).input_path, gpu_ctx)
+    st64 = sim.run("fast64", options=abi.SCAN_REPRODUCIBLE)
+    f64 = sim.records()
+    assert st64["positions"] == 256 * 256 * 40 and st64["lanes_per_pixel"] > 1
+    st32 = sim.run("fast32", options=abi.SCAN_REPRODUCIBLE)
+    f32 = sim.records()
+    assert st32["unconverged"] == st64["unconverged"] == 0
+    dr = np.sqrt((f32["rx"] - f64["rx"]) ** 2 + (f32["ry"] - f64["ry"]) ** 2 + (f32["rz"] - f64["rz"]) ** 2)
+    dfz = np.abs(f32["fz"] - f64["fz"]) / np.maximum(np.abs(f64["fz"]), 1e-2)
+    ok = (dr <= 1e-4) & (dfz <= 1e-3)
+    print(f"c4 full size: fast32 within the bar of fast64 on {ok.mean() * 100:.4f} %, "
+          f"fast64 {st64['kernel_ms']:.0f} ms, fast32 {st32['kernel_ms']:.0f} ms")
+    assert ok.mean() >= 0.99
+    sim.run("fast64", options=abi.SCAN_REPRODUCIBLE)
+    assert (sim.records() == f64).all()
     sim.close()
