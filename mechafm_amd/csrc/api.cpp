@@ -472,11 +472,15 @@ static mafm_status plan_fast(mafm_ctx* c, const mafm_scan_desc* d, bool f32, con
         const size_t lds_bytes = std::max<size_t>((size_t)max_sites * site_bytes, 1);
         const long blocks_per_cu = (long)(c->lds_per_cu / lds_bytes);
         while (p.W < 8 && blocks_per_cu * p.W < 4L * p.wps) ++p.W;
-        // FIRE + midpoint kernels: at least four waves per workgroup (one
-        // per SIMD), which measured faster than two on the batched workload
-        // (fast64 531-537 -> 516-518 ms, fast32 -0.8 %); any W that holds the
-        // LDS constraint above also holds it at 4
-        if (p.mid_family) p.W = std::max(p.W, 4);
+        // FIRE + midpoint kernels over site tables: at least four waves per
+        // workgroup (one per SIMD), which measured faster than two on the
+        // batched workload (fast64 531-537 -> 516-518 ms, fast32 -0.8 %); any
+        // W that holds the LDS constraint above also holds it at 4.  Force-
+        // grid scans stage nothing and keep the smallest W (four measured
+        // 6 % slower there).
+        bool stages_sites = false;
+        for (int m = 0; m < n_mols; ++m) stages_sites |= cols[m] > 0 && c->mols[m].force_grid < 0;
+        if (p.mid_family && stages_sites) p.W = std::max(p.W, 4);
     }
     const long groups_per_wave = 64 / p.G;
     const long need_waves = std::max<long>((total_cols + groups_per_wave - 1) / groups_per_wave, 1);
